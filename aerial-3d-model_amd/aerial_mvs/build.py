@@ -1,0 +1,72 @@
+"""Build recipes for the native libraries (run on the CPU; hipcc cross-compiles gfx950).
+
+- ``libaerial_mvs.so``: the HIP engine behind ``include/aerial_mvs.h``.
+- ``libsynth.so``: the seeded synthetic-survey generator (test/benchmark workload).
+
+The oracle under ``oracle/`` is built by ``oracle/build.py``; this module does not
+touch it.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+SRC_ROOT = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(SRC_ROOT, "csrc")
+REPO = os.path.dirname(SRC_ROOT)
+
+LIB_MVS = os.path.join(PKG_DIR, "libaerial_mvs.so")
+LIB_SYNTH = os.path.join(PKG_DIR, "libsynth.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: the reference (Rust) never fuses a*b+c; the kernels match
+# its rounding operation by operation. IEEE-correct f32 division/sqrt are the
+# HIP defaults and are requested explicitly. Packed-FP32 instructions are
+# disabled for the whole build.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-rdc",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+C_FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+           "-shared", "-Wall", "-pthread"]
+
+
+def _run(cmd):
+    print("+ " + " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_mvs(extra=None):
+    srcs = [os.path.join(CSRC, "mvs_kernels.hip"), os.path.join(CSRC, "mvs_api.hip")]
+    tmp = LIB_MVS + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + (extra or []) + srcs + ["-o", tmp])
+    os.replace(tmp, LIB_MVS)
+    return LIB_MVS
+
+
+def build_synth():
+    tmp = LIB_SYNTH + ".tmp"
+    _run(["gcc"] + C_FLAGS + [os.path.join(CSRC, "synth.c"), "-o", tmp, "-lm"])
+    os.replace(tmp, LIB_SYNTH)
+    return LIB_SYNTH
+
+
+def build_all():
+    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
+        raise RuntimeError("hipcc not found; the HIP engine cannot be built")
+    build_synth()
+    build_mvs()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "mvs": build_mvs, "synth": build_synth}[w]()

@@ -1,0 +1,332 @@
+"""ctypes binding of ``libaerial_mvs.so`` (the C ABI declared in include/aerial_mvs.h).
+
+This is glue for the tests and the benchmark; the engine itself is the HIP
+library. Loading fails loudly when the library is missing — there is no other
+implementation behind these calls.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = _build.LIB_MVS
+
+MVS_OK = 0
+MVS_NO_NEIGHBOURS = 1
+MVS_GRAY_U8 = 0
+MVS_GRAY_F32 = 1
+MVS_CONS_REFERENCE = 0
+
+EXPORTS = [
+    "mvs_default_params", "mvs_dense_params_for_profile", "mvs_select_neighbours",
+    "mvs_create", "mvs_destroy", "mvs_last_error", "mvs_synchronize", "mvs_depth_map",
+    "mvs_ncc_cost", "mvs_patchmatch_steps", "mvs_survey_create", "mvs_survey_destroy",
+    "mvs_survey_set_view", "mvs_survey_depth_map", "mvs_survey_depth_maps",
+    "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
+    "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth",
+]
+
+
+class MvsView(C.Structure):
+    _fields_ = [("w", C.c_int32), ("h", C.c_int32), ("stride", C.c_int32),
+                ("format", C.c_int32), ("pixels", C.c_void_p),
+                ("f", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("k1", C.c_double), ("k2", C.c_double),
+                ("R", C.c_double * 9), ("C", C.c_double * 3)]
+
+
+class MvsParams(C.Structure):
+    _fields_ = [("taps", C.c_int32), ("stride", C.c_int32), ("iterations", C.c_int32),
+                ("texture_min", C.c_float), ("cost_max", C.c_float),
+                ("reserved", C.c_int32 * 3)]
+
+
+class MvsStats(C.Structure):
+    _fields_ = [("sweep_launches", C.c_int64), ("sweep_ms", C.c_double),
+                ("init_launches", C.c_int64), ("init_ms", C.c_double),
+                ("evals", C.c_int64), ("pixels", C.c_int64),
+                ("init_evals", C.c_int64), ("sweep_evals", C.c_int64)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)
+
+_lib = None
+
+
+class MvsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"aerial_mvs error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP engine not built ({LIB_PATH} missing): run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        P = C.POINTER
+        L.mvs_default_params.argtypes = [P(MvsParams)]
+        L.mvs_dense_params_for_profile.argtypes = [i32, P(C.c_int64)]
+        L.mvs_select_neighbours.argtypes = [P(C.c_double), i32, i32, P(C.c_int32), i32, f64, i32,
+                                            P(C.c_int32)]
+        L.mvs_select_neighbours.restype = i32
+        L.mvs_create.argtypes = [i32, P(vp), C.c_char_p, C.c_size_t]
+        L.mvs_destroy.argtypes = [vp]
+        L.mvs_destroy.restype = None
+        L.mvs_last_error.argtypes = [vp]
+        L.mvs_last_error.restype = C.c_char_p
+        L.mvs_synchronize.argtypes = [vp]
+        L.mvs_depth_map.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams),
+                                    vp, vp, vp, vp]
+        L.mvs_ncc_cost.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
+                                   vp, vp, vp, vp]
+        L.mvs_patchmatch_steps.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64,
+                                           P(MvsParams), i32, i32, vp, vp, P(C.c_int64)]
+        L.mvs_survey_create.argtypes = [vp, i32, P(vp)]
+        L.mvs_survey_destroy.argtypes = [vp]
+        L.mvs_survey_destroy.restype = None
+        L.mvs_survey_set_view.argtypes = [vp, i32, P(MvsView)]
+        L.mvs_survey_depth_map.argtypes = [vp, i32, vp, i32, f64, f64, P(MvsParams)]
+        L.mvs_survey_depth_maps.argtypes = [vp, i32, vp, vp, vp, vp, P(MvsParams), vp,
+                                            PROGRESS_FN, vp]
+        L.mvs_survey_download.argtypes = [vp, i32, vp, vp, vp]
+        L.mvs_survey_consistency.argtypes = [vp, i32, vp, i32, i32, vp, P(C.c_int64)]
+        L.mvs_set_timing.argtypes = [vp, i32]
+        L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
+        L.mvs_reset_stats.argtypes = [vp]
+        L.mvs_save_depth.argtypes = [C.c_char_p, i32, i32, vp]
+        L.mvs_load_depth.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, i64]
+        _lib = L
+    return _lib
+
+
+def params(taps=5, stride=2, iterations=4, texture_min=0.008, cost_max=0.45):
+    p = MvsParams()
+    lib().mvs_default_params(C.byref(p))
+    p.taps, p.stride, p.iterations = taps, stride, iterations
+    p.texture_min, p.cost_max = texture_min, cost_max
+    return p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def make_view(v, keep):
+    """mvs_view of a survey View (u8 grey, or f32 grey when `gray_f32` is set)."""
+    mv = MvsView()
+    g32 = getattr(v, "gray_f32", None)
+    if g32 is not None:
+        arr = np.ascontiguousarray(g32, np.float32)
+        mv.format = MVS_GRAY_F32
+    else:
+        arr = np.ascontiguousarray(v.gray, np.uint8)
+        mv.format = MVS_GRAY_U8
+    keep.append(arr)
+    mv.h, mv.w = arr.shape
+    mv.stride = arr.shape[1]
+    mv.pixels = arr.ctypes.data
+    mv.f, mv.cx, mv.cy, mv.k1, mv.k2 = v.f, v.cx, v.cy, v.k1, v.k2
+    mv.R[:] = [float(a) for a in np.asarray(v.R, np.float64).ravel()]
+    mv.C[:] = [float(a) for a in np.asarray(v.C, np.float64).ravel()]
+    return mv
+
+
+def dense_params_for_profile(kind):
+    out = (C.c_int64 * 4)()
+    rc = lib().mvs_dense_params_for_profile(kind, out)
+    if rc:
+        raise MvsError(rc, "unknown profile")
+    return dict(size=out[0], iterations=out[1], neighbours=out[2], max_points=out[3])
+
+
+def select_neighbours(centres, ci, neighbours, range0, take):
+    centres = np.ascontiguousarray(centres, np.float64)
+    nb = np.ascontiguousarray(neighbours, np.int32)
+    out = np.zeros(max(len(nb), 1), np.int32)
+    k = lib().mvs_select_neighbours(
+        centres.ctypes.data_as(C.POINTER(C.c_double)), len(centres), ci,
+        nb.ctypes.data_as(C.POINTER(C.c_int32)), len(nb), range0, take,
+        out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out[:k].tolist()
+
+
+def save_depth(path, depth):
+    d = np.ascontiguousarray(depth, np.float32)
+    rc = lib().mvs_save_depth(os.fsencode(path), d.shape[1], d.shape[0], d.ctypes.data)
+    if rc:
+        raise MvsError(rc, "save_depth failed")
+
+
+def load_depth(path):
+    w, h = C.c_int32(), C.c_int32()
+    rc = lib().mvs_load_depth(os.fsencode(path), C.byref(w), C.byref(h), None, 0)
+    if rc:
+        return None
+    d = np.zeros((h.value, w.value), np.float32)
+    rc = lib().mvs_load_depth(os.fsencode(path), C.byref(w), C.byref(h), d.ctypes.data, d.size)
+    return None if rc else d
+
+
+class Context:
+    """One mvs_ctx: a HIP stream and its device buffers on one GPU."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib().mvs_create(device, C.byref(self._h), err, len(err))
+        if rc:
+            raise MvsError(rc, err.value.decode())
+
+    def close(self):
+        if self._h:
+            lib().mvs_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc, allow=(MVS_OK,)):
+        if rc not in allow:
+            raise MvsError(rc, lib().mvs_last_error(self._h).decode())
+        return rc
+
+    def synchronize(self):
+        self.check(lib().mvs_synchronize(self._h))
+
+    def _views(self, ref, srcs):
+        keep = []
+        r = make_view(ref, keep)
+        arr = (MvsView * max(len(srcs), 1))()
+        for i, s in enumerate(srcs):
+            arr[i] = make_view(s, keep)
+        return r, arr, keep
+
+    def depth_map(self, ref, srcs, range0, range1, prm, cancel=None):
+        """Returns (depth, normal, cost) or None when there are no neighbours."""
+        r, arr, keep = self._views(ref, srcs)
+        h, w = ref.h, ref.w
+        d = np.zeros((h, w), np.float32)
+        n = np.zeros((h, w, 3), np.float32)
+        c = np.zeros((h, w), np.float32)
+        cflag = None if cancel is None else C.cast(C.pointer(cancel), C.c_void_p)
+        rc = lib().mvs_depth_map(self._h, C.byref(r), arr, len(srcs), range0, range1,
+                                 C.byref(prm), d.ctypes.data, n.ctypes.data, c.ctypes.data, cflag)
+        if self.check(rc, (MVS_OK, MVS_NO_NEIGHBOURS)) == MVS_NO_NEIGHBOURS:
+            return None
+        return d, n, c
+
+    def ncc_cost(self, ref, srcs, range0, range1, prm, xs, ys, hyps):
+        r, arr, keep = self._views(ref, srcs)
+        xs = np.ascontiguousarray(xs, np.int32)
+        ys = np.ascontiguousarray(ys, np.int32)
+        hy = np.ascontiguousarray(hyps, np.float32).reshape(-1, 4)
+        out = np.zeros(len(xs), np.float32)
+        self.check(lib().mvs_ncc_cost(self._h, C.byref(r), arr, len(srcs), range0, range1,
+                                      C.byref(prm), len(xs), xs.ctypes.data, ys.ctypes.data,
+                                      hy.ctypes.data, out.ctypes.data))
+        return out
+
+    def patchmatch_steps(self, ref, srcs, range0, range1, prm, first_step, n_steps, state=None):
+        """Returns (state (h,w,5), textured, evals). first_step = -1 initialises."""
+        r, arr, keep = self._views(ref, srcs)
+        h, w = ref.h, ref.w
+        st = (np.zeros((h, w, 5), np.float32) if state is None
+              else np.array(state, np.float32, order="C", copy=True))
+        tex = np.zeros((h, w), np.uint8)
+        ev = C.c_int64()
+        self.check(lib().mvs_patchmatch_steps(self._h, C.byref(r), arr, len(srcs), range0, range1,
+                                              C.byref(prm), first_step, n_steps, st.ctypes.data,
+                                              tex.ctypes.data, C.byref(ev)))
+        return st, tex, ev.value
+
+    def set_timing(self, on=True):
+        self.check(lib().mvs_set_timing(self._h, 1 if on else 0))
+
+    def stats(self):
+        s = MvsStats()
+        self.check(lib().mvs_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in MvsStats._fields_}
+
+    def reset_stats(self):
+        self.check(lib().mvs_reset_stats(self._h))
+
+
+class Survey:
+    """Views resident in HBM; depth maps stay on the device."""
+
+    def __init__(self, ctx, views):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx.check(lib().mvs_survey_create(ctx.handle, len(views), C.byref(self._h)))
+        self.shapes = [(v.h, v.w) for v in views]
+        for i, v in enumerate(views):
+            keep = []
+            mv = make_view(v, keep)
+            ctx.check(lib().mvs_survey_set_view(self._h, i, C.byref(mv)))
+
+    def close(self):
+        if self._h:
+            lib().mvs_survey_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def depth_map(self, ref, srcs, range0, range1, prm):
+        s = np.ascontiguousarray(srcs, np.int32)
+        return self.ctx.check(lib().mvs_survey_depth_map(self._h, ref, s.ctypes.data, len(s),
+                                                         range0, range1, C.byref(prm)),
+                              (MVS_OK, MVS_NO_NEIGHBOURS))
+
+    def depth_maps(self, jobs, prm, progress=None):
+        """jobs: list of (ref, [srcs], (range0, range1))."""
+        refs = np.array([j[0] for j in jobs], np.int32)
+        off = np.zeros(len(jobs) + 1, np.int32)
+        for i, j in enumerate(jobs):
+            off[i + 1] = off[i] + len(j[1])
+        srcs = np.array([s for j in jobs for s in j[1]] or [0], np.int32)
+        rng = np.array([x for j in jobs for x in j[2]], np.float64)
+        cb = PROGRESS_FN(progress) if progress else PROGRESS_FN()
+        self.ctx.check(lib().mvs_survey_depth_maps(self._h, len(jobs), refs.ctypes.data,
+                                                   off.ctypes.data, srcs.ctypes.data,
+                                                   rng.ctypes.data, C.byref(prm), None, cb, None))
+
+    def download(self, view):
+        h, w = self.shapes[view]
+        d = np.zeros((h, w), np.float32)
+        n = np.zeros((h, w, 3), np.float32)
+        c = np.zeros((h, w), np.float32)
+        self.ctx.check(lib().mvs_survey_download(self._h, view, d.ctypes.data, n.ctypes.data,
+                                                 c.ctypes.data))
+        return d, n, c
+
+    def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE):
+        h, w = self.shapes[ref]
+        nb = np.ascontiguousarray(nbrs, np.int32)
+        agree = np.zeros((h, w), np.int32)
+        n = C.c_int64()
+        self.ctx.check(lib().mvs_survey_consistency(self._h, ref, nb.ctypes.data, len(nb), rule,
+                                                    agree.ctypes.data, C.byref(n)))
+        return agree, n.value

@@ -1,0 +1,231 @@
+"""Seeded synthetic aerial surveys (workload for tests and the benchmark).
+
+Scene: fractal heightfield (smooth value-noise fBm) textured with band-limited
+value noise, rendered by the C generator ``libsynth.so`` with pinhole
+projection, per-view Gaussian sensor noise, and the FP64 camera-z depth of every
+pixel (ground truth). Cameras follow the reference's convention and its
+``Pose::from_angles`` (crates/stitch-core/src/recon/geometry.rs:76-93); the
+lawnmower layout with alternating yaw follows its synthetic BA survey
+(recon/ba.rs:411-421). Neighbours are nearest by camera position (GPS), best
+first, at most 8 as in recon/sparse.rs:536.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import build as _build
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def _synth():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_build.LIB_SYNTH):
+            raise RuntimeError("libsynth.so not built")
+        L = C.CDLL(_build.LIB_SYNTH)
+        L.synth_scene_create.argtypes = [C.c_double] * 6 + [C.c_int32, C.c_double, C.c_double,
+                                                            C.c_uint64, C.c_double, C.c_int32,
+                                                            C.c_int32]
+        L.synth_scene_create.restype = C.c_void_p
+        L.synth_scene_add_boxes.argtypes = [C.c_void_p, _f64p, C.c_int32]
+        L.synth_scene_destroy.argtypes = [C.c_void_p]
+        L.synth_scene_info.argtypes = [C.c_void_p, _f64p]
+        L.synth_height.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.synth_height.restype = C.c_double
+        L.synth_render.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [C.c_double] * 5 + [
+            _f64p, _f64p, C.c_uint64, C.c_double, _u8p, C.c_void_p, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def rot_axis(axis, angle):
+    """geometry.rs:104-108 (Rodrigues about a unit axis)."""
+    a = np.asarray(axis, np.float64)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]], np.float64)
+    return np.eye(3) + K * np.sin(angle) + K @ K * (1.0 - np.cos(angle))
+
+
+def from_angles(c, yaw_deg, pitch_deg, roll_deg):
+    """Pose::from_angles (geometry.rs:76-93): returns (R world->camera, C)."""
+    sy, cy = np.sin(np.radians(yaw_deg)), np.cos(np.radians(yaw_deg))
+    sp, cp = np.sin(np.radians(pitch_deg)), np.cos(np.radians(pitch_deg))
+    heading = np.array([sy, cy, 0.0])
+    up = np.array([0.0, 0.0, 1.0])
+    x = np.array([cy, -sy, 0.0])
+    z = heading * cp + up * sp
+    y = np.cross(z, x)
+    r_wc = np.stack([x, y, z], axis=1)
+    if abs(roll_deg) > 1e-9:
+        r_wc = r_wc @ rot_axis([0.0, 0.0, 1.0], np.radians(roll_deg))
+    return np.ascontiguousarray(r_wc.T), np.asarray(c, np.float64)
+
+
+@dataclass
+class View:
+    gray: np.ndarray                 # (h, w) uint8
+    f: float
+    cx: float
+    cy: float
+    k1: float
+    k2: float
+    R: np.ndarray                    # world -> camera
+    C: np.ndarray
+    depth_gt: Optional[np.ndarray] = None   # (h, w) float64 camera z, 0 = no hit
+
+    @property
+    def w(self):
+        return self.gray.shape[1]
+
+    @property
+    def h(self):
+        return self.gray.shape[0]
+
+
+@dataclass
+class SurveySpec:
+    name: str
+    width: int
+    height: int
+    focal: float
+    altitude: float
+    lines: int
+    per_line: int
+    forward_overlap: float = 0.75
+    side_overlap: float = 0.65
+    patch: tuple = (200.0, 150.0)
+    relief: float = 15.0
+    grid: float = 0.1
+    base_lambda: float = 50.0
+    octaves: int = 6
+    persistence: float = 0.5
+    tex_lambda0: float = 0.2
+    tex_octaves: int = 5
+    pos_sigma: float = 0.05
+    ang_sigma_deg: float = 0.05
+    noise_dn: float = 1.0
+    depth_range: tuple = (40.0, 80.0)
+    neighbours: int = 4
+    taps: int = 7
+    stride: int = 1
+    iterations: int = 4
+    k1: float = 0.0
+    k2: float = 0.0
+    seed: int = 0xA3D00000
+
+
+# The reference-window shape: 5 taps at stride 2 (dense.rs:170).
+SPECS = {
+    # 16 nadir views 1024x768, 4 neighbours, 7x7, 4 iterations
+    "c1": SurveySpec("c1", 1024, 768, 900.0, 60.0, 4, 4, neighbours=4, iterations=4,
+                     seed=0xA3D00001),
+    # 77 nadir views 2000x1500, 8 neighbours, 7x7, 6 iterations;
+    # focal scaled with the width to keep configs[0]'s field of view; 60 m
+    # altitude, depth range [40, 80], the 7 x 11 block over the scene centre.
+    "c2": SurveySpec("c2", 2000, 1500, 900.0 * 2000 / 1024, 60.0, 7, 11, patch=(500.0, 350.0),
+                     neighbours=8, iterations=6, seed=0xA3D00002),
+    # small surveys for tests: the c1 camera geometry with a cropped sensor
+    "tiny": SurveySpec("tiny", 160, 120, 900.0, 60.0, 3, 3, patch=(60.0, 50.0), neighbours=4,
+                       taps=5, stride=2, iterations=3, seed=0xA3D0000A),
+    "small": SurveySpec("small", 320, 240, 900.0, 60.0, 3, 4, patch=(80.0, 60.0), neighbours=4,
+                        taps=7, stride=1, iterations=4, seed=0xA3D0000B),
+}
+
+
+@dataclass
+class Survey:
+    spec: SurveySpec
+    views: List[View]
+    neighbours: List[List[int]]          # up to 8 per view, nearest first
+    ranges: List[tuple]
+    ground: float
+    extra: dict = field(default_factory=dict)
+
+    def centres(self):
+        return np.array([v.C for v in self.views], np.float64)
+
+    def reference_index(self):
+        """The view nearest the survey centre (lowest index on ties)."""
+        c = self.centres()[:, :2]
+        m = c.mean(axis=0)
+        d = np.hypot(c[:, 0] - m[0], c[:, 1] - m[1])
+        return int(np.argmin(d))
+
+
+def _layout(spec, rng):
+    fw = spec.width / spec.focal * spec.altitude      # across track (image x)
+    fh = spec.height / spec.focal * spec.altitude     # along track (image y)
+    along = (1.0 - spec.forward_overlap) * fh
+    across = (1.0 - spec.side_overlap) * fw
+    cx0, cy0 = spec.patch[0] / 2, spec.patch[1] / 2
+    poses = []
+    for r in range(spec.lines):
+        yaw = 90.0 if r % 2 == 0 else 270.0
+        cols = range(spec.per_line) if r % 2 == 0 else range(spec.per_line - 1, -1, -1)
+        for cidx in cols:
+            x = cx0 + (cidx - (spec.per_line - 1) / 2) * along
+            y = cy0 + (r - (spec.lines - 1) / 2) * across
+            poses.append((x, y, yaw))
+    out = []
+    for (x, y, yaw) in poses:
+        dp = rng.normal(0.0, spec.pos_sigma, 3)
+        da = rng.normal(0.0, spec.ang_sigma_deg, 3)
+        out.append(((x + dp[0], y + dp[1], dp[2]), yaw + da[0], -90.0 + da[1], da[2]))
+    return out
+
+
+def nearest_neighbours(centres, k=8):
+    out = []
+    for i in range(len(centres)):
+        d = np.hypot(centres[:, 0] - centres[i, 0], centres[:, 1] - centres[i, 1])
+        order = sorted((float(d[j]), j) for j in range(len(centres)) if j != i)
+        out.append([j for _, j in order[:k]])
+    return out
+
+
+def make_survey(spec, nthreads=None, with_depth=True, views=None):
+    """Render a survey. `views` (optional) restricts which views get pixels
+    (others keep an empty image), to save time when only some are needed."""
+    if isinstance(spec, str):
+        spec = SPECS[spec]
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    L = _synth()
+    margin = spec.altitude
+    scene = L.synth_scene_create(-margin, -margin, spec.patch[0] + 2 * margin,
+                                 spec.patch[1] + 2 * margin, spec.grid, spec.base_lambda,
+                                 spec.octaves, spec.persistence, spec.relief, spec.seed,
+                                 spec.tex_lambda0, spec.tex_octaves, nthreads)
+    try:
+        info = np.zeros(6)
+        L.synth_scene_info(scene, info)
+        ground = 0.5 * (info[0] + info[1])
+        rng = np.random.default_rng(spec.seed + 1)
+        layout = _layout(spec, rng)
+        cx, cy = spec.width / 2.0, spec.height / 2.0
+        vs = []
+        for vi, (c, yaw, pitch, roll) in enumerate(layout):
+            R, Cc = from_angles((c[0], c[1], ground + spec.altitude + c[2]), yaw, pitch, roll)
+            img = np.zeros((spec.height, spec.width), np.uint8)
+            dep = np.zeros((spec.height, spec.width), np.float64) if with_depth else None
+            if views is None or vi in views:
+                L.synth_render(scene, spec.width, spec.height, spec.focal, cx, cy, spec.k1, spec.k2,
+                               np.ascontiguousarray(R.ravel()), Cc, spec.seed + 1000 + vi,
+                               spec.noise_dn, img.ravel(),
+                               None if dep is None else dep.ctypes.data, nthreads)
+            vs.append(View(img, spec.focal, cx, cy, spec.k1, spec.k2, R, Cc, dep))
+    finally:
+        L.synth_scene_destroy(scene)
+    cent = np.array([v.C for v in vs])
+    nb = nearest_neighbours(cent, 8)
+    ranges = [tuple(spec.depth_range)] * len(vs)
+    return Survey(spec, vs, nb, ranges, ground)
+
+
+def params_for(spec, api_params):
+    return api_params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)

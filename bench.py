@@ -1,0 +1,190 @@
+"""Benchmark: dense-MVS depth maps on one MI355X.
+
+Workload (the survey the metric is quoted on): 77 nadir views at 2000x1500 u8
+grey, seeded synthetic heightfield survey; each view is a reference view with
+its 8 nearest-GPS neighbours, 7x7 NCC window, 6 PatchMatch iterations, then the
+geometric-consistency filter. A "step" is one reference view's depth map
+(initialisation + 12 half-sweeps + threshold); the timed region covers K steps
+enqueued back to back plus the consistency filter of those K views, with all
+images already resident in HBM.
+
+Prints one JSON line (rank 0 of one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "aerial-3d-model_amd")]
+
+METRIC = ("depth Mpix/s (7x7 NCC, 8 nbrs) batched+refined GPU vs plain GPU vs CPU; depth err")
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_per_eval(taps, k):
+    # declared algorithmic count of one (pixel, hypothesis) cost evaluation:
+    # per window sample 12 flops of plane/reference work, and per sample and
+    # source 48 (3x3 apply 15, divide 2, projection 13, bilinear 13, sums 5)
+    n = taps * taps
+    return n * (12 + 48 * k)
+
+
+def depth_errors(depth, gt):
+    valid = (depth > 0) & (gt > 0)
+    if not valid.any():
+        return dict(valid=0.0)
+    rel = np.abs(depth[valid] - gt[valid]) / gt[valid]
+    return dict(valid=float(valid.mean()), within_0p5=float(np.mean(rel <= 0.005)),
+                within_0p2=float(np.mean(rel <= 0.002)), median_rel=float(np.median(rel)))
+
+
+def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm):
+    """The CPU oracle on a bounded sample: a centred crop of the first
+    reference view (its intrinsics shifted accordingly) against the same 8
+    full-size sources, I and window as the GPU run."""
+    import copy
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    ref, srcs, rng = jobs[0]
+    v = surv.views[ref]
+    x0, y0 = (v.w - crop_w) // 2, (v.h - crop_h) // 2
+    cv = copy.copy(v)
+    cv.gray = np.ascontiguousarray(v.gray[y0:y0 + crop_h, x0:x0 + crop_w])
+    cv.cx, cv.cy = v.cx - x0, v.cy - y0
+    sv = [surv.views[j] for j in srcs]
+    o = orc.OracleDepth(cv, sv, rng[0], rng[1], taps=spec.taps, stride=spec.stride)
+    t = time.perf_counter()
+    od, _, _, ev = o.depth_map(iterations=spec.iterations, nthreads=nthreads)
+    dt = time.perf_counter() - t
+    # the GPU on the same crop: parity of the sample
+    gd, _, _ = gpu_ctx.depth_map(cv, sv, rng[0], rng[1], prm)
+    both = (gd > 0) & (od > 0)
+    within = float(np.mean(np.abs(gd[both] - od[both]) / od[both] <= 0.005)) if both.any() else 0.0
+    mpix = crop_w * crop_h / 1e6
+    return dict(value=mpix / dt, unit="depth Mpix/s", cores=nthreads, kind="port",
+                sample=(f"oracle C restatement of recon/dense.rs, {crop_w}x{crop_h} centre crop "
+                        f"of view {ref} vs its {len(srcs)} full-size neighbours, "
+                        f"{spec.taps}x{spec.taps} window, {spec.iterations} iterations: "
+                        f"{dt:.1f} s, {ev} cost evaluations"),
+                seconds=dt, sample_gpu_within_0p5=within,
+                sample_identical=float(np.mean(gd == od)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=77)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-crop", default="640x480")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this depth engine runs on a single MI355X "
+              "(multi-GPU sharding by reference view is designed but not built); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from aerial_mvs import capi, survey
+
+    spec = survey.SPECS[args.config]
+    t = time.perf_counter()
+    surv = survey.make_survey(spec, nthreads=args.threads)
+    log(f"survey {spec.name}: {len(surv.views)} views {spec.width}x{spec.height} "
+        f"rendered in {time.perf_counter() - t:.1f} s")
+    centres = surv.centres()
+    jobs = []
+    for i in range(len(surv.views)):
+        nb = capi.select_neighbours(centres, i, surv.neighbours[i], spec.depth_range[0],
+                                    spec.neighbours)
+        jobs.append((i, nb, spec.depth_range))
+    steps = max(1, min(args.steps, len(jobs)))
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
+
+    ctx = capi.Context(0)
+    sv = capi.Survey(ctx, surv.views)
+    log("views resident in HBM")
+    for w in range(args.warmup):
+        sv.depth_maps([jobs[w % len(jobs)]], prm)
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    timed = jobs[:steps]
+    done = set(j[0] for j in timed)
+
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    sv.depth_maps(timed, prm)
+    n_cons = 0
+    for (i, _, _) in timed:
+        nbrs = [j for j in surv.neighbours[i][:spec.neighbours + 1] if j in done]
+        if nbrs:
+            n_cons += sv.consistency(i, nbrs)[1]
+    ctx.synchronize()
+    t1 = time.perf_counter()
+
+    st = ctx.stats()
+    pixels = sum(surv.views[j[0]].w * surv.views[j[0]].h for j in timed)
+    elapsed = t1 - t0
+    value = pixels / elapsed / 1e6
+    log(f"timed {steps} views: {elapsed:.2f} s, {value:.2f} depth Mpix/s; stats {st}")
+
+    # roofline of the dominant kernel (the half-sweep)
+    fpe = flops_per_eval(spec.taps, spec.neighbours)
+    avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
+    evals_per_launch = st["sweep_evals"] / max(1, st["sweep_launches"])
+    achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
+    roofline = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
+                    kernel="k_patchmatch<7,8,pinhole,sweep>", avg_launch_ms=avg_ms,
+                    evals_per_launch=evals_per_launch, flops_per_eval=fpe,
+                    launches=st["sweep_launches"])
+
+    # depth error against the FP64 ground truth, all timed views
+    errs = []
+    for (i, _, _) in timed:
+        d, _, _ = sv.download(i)
+        errs.append(depth_errors(d, surv.views[i].depth_gt))
+    err = {k: float(np.mean([e.get(k, 0.0) for e in errs])) for k in errs[0]}
+    err["consistent_pixels"] = int(n_cons)
+    err["reference"] = "FP64 ray-cast ground truth of the synthetic scene"
+
+    cpu = None
+    if not args.no_cpu:
+        cw, ch = (int(a) for a in args.cpu_crop.split("x"))
+        log("CPU baseline (oracle) ...")
+        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm)
+        log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
+
+    line = dict(
+        metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,
+        warmup=args.warmup, ms_per_step=elapsed * 1e3 / steps, higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(workload=(f"{len(surv.views)} nadir views {spec.width}x{spec.height} u8 grey, "
+                              f"{spec.neighbours} nearest-GPS neighbours, {spec.taps}x{spec.taps} "
+                              f"NCC window, {spec.iterations} PatchMatch iterations, "
+                              "geometric-consistency filter"),
+                    views=len(surv.views), width=spec.width, height=spec.height,
+                    neighbours=spec.neighbours, window=f"{spec.taps}x{spec.taps}/stride{spec.stride}",
+                    iterations=spec.iterations, timed_views=steps, path="plain GPU"),
+        roofline=roofline, cpu_baseline=cpu, depth_error=err,
+        gpu_vs_cpu=(value / cpu["value"]) if cpu else None,
+        evals_per_pixel=st["evals"] / max(1, pixels),
+    )
+    print(json.dumps(line), flush=True)
+    sv.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+/*
+ * aerial_mvs — MI355X (gfx950) dense multi-view-stereo depth engine, C ABI.
+ *
+ * Drop-in for the PatchMatch depth-map stage of stitch-core
+ * (crates/stitch-core/src/recon/dense.rs). The reference has no FFI of its own:
+ * its seam is the private Rust function
+ *     fn depth_map(dir, sp, ci, prm) -> Result<Option<(usize, usize, Vec<f32>)>>
+ * (dense.rs:315-321), called once per view from reconstruct_dense
+ * (dense.rs:530-540), plus the `.view` / `.depth` cache files (dense.rs:88-154,
+ * 311-313, 481-508). Every entry point below names the reference code it
+ * replaces. Plain C types only; all pointers are host pointers unless stated.
+ *
+ * Conventions (geometry.rs:1-5): world = local east/north/up metres, camera
+ * x right / y down / z forward, R is world -> camera (row-major), a world point
+ * X lands at R (X - C). Intrinsics are in the view's own pixels (already scaled
+ * to the working size, dense.rs:138-145).
+ *
+ * Return codes: MVS_OK (0); MVS_NO_NEIGHBOURS (1) is the reference's Ok(None)
+ * (dense.rs:335-337); negative values are errors whose text mvs_last_error()
+ * returns (the reference's Err). There is no CPU fallback: without a usable
+ * gfx950 device every compute entry point fails with MVS_ERR_DEVICE.
+ *
+ * Threading: an mvs_ctx owns one HIP stream and its device buffers; use it
+ * from one thread at a time (the reference runs depth maps one view at a time,
+ * dense.rs:528-530). Different contexts may run concurrently.
+ */
+#ifndef AERIAL_MVS_H
+#define AERIAL_MVS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MVS_ABI_VERSION 1
+
+enum {
+    MVS_OK = 0,
+    MVS_NO_NEIGHBOURS = 1,
+    MVS_ERR_ARG = -1,
+    MVS_ERR_DEVICE = -2,
+    MVS_ERR_CANCELLED = -3,
+    MVS_ERR_NOMEM = -4,
+    MVS_ERR_IO = -5,
+    MVS_ERR_FORMAT = -6
+};
+
+/* Pixel formats of a view's grey plane. U8 is converted with the reference's
+ * grey formula for R = G = B (dense.rs:108); F32 holds the reference's own grey
+ * values (the `.view` cache, dense.rs:104-111) and is used unchanged. */
+enum { MVS_GRAY_U8 = 0, MVS_GRAY_F32 = 1 };
+
+/* One working-size view (the reference's View + Camera, dense.rs:57-65,
+ * sparse.rs:26-38, geometry.rs:13-53). */
+typedef struct {
+    int32_t w, h;
+    int32_t stride;          /* pixels between rows of `pixels` (>= w) */
+    int32_t format;          /* MVS_GRAY_U8 or MVS_GRAY_F32 */
+    const void *pixels;      /* host memory, borrowed for the call */
+    double f, cx, cy, k1, k2;
+    double R[9];             /* world -> camera, row-major */
+    double C[3];             /* camera centre */
+} mvs_view;
+
+/* Knobs of the depth stage. mvs_default_params() gives the reference's
+ * constants: 5 taps at stride 2 (dense.rs:170), TEXTURE_MIN 0.008 (:173),
+ * output threshold 0.45 (:471), Standard-profile iterations 4 (:41-46). */
+typedef struct {
+    int32_t taps;            /* window taps per side: 5 (reference), 7 or 11 */
+    int32_t stride;          /* pixels between taps (reference 2) */
+    int32_t iterations;      /* PatchMatch iterations */
+    float texture_min;
+    float cost_max;
+    int32_t reserved[3];     /* zero */
+} mvs_params;
+
+void mvs_default_params(mvs_params *p);
+
+/* DenseParams::for_profile (dense.rs:32-55). kind: 0 Lite, 1 Standard, 2 High.
+ * out = {size, iterations, neighbours, max_points}. Returns MVS_ERR_ARG for an
+ * unknown kind. */
+int mvs_dense_params_for_profile(int32_t kind, int64_t out[4]);
+
+/* Neighbour choice of depth_map (dense.rs:324-334): keep cam.neighbours in
+ * order whose baseline |C_j - C_ci| > 0.02 max(range0, 1), first `take` of them.
+ * centres: ncams x 3 doubles. Writes indices to out, returns how many. */
+int32_t mvs_select_neighbours(const double *centres, int32_t ncams, int32_t ci,
+                              const int32_t *neighbours, int32_t nn, double range0,
+                              int32_t take, int32_t *out);
+
+typedef struct mvs_ctx mvs_ctx;
+
+/* Create a context on HIP device `device`. On failure returns a negative code
+ * and writes a message into err (if given). */
+int mvs_create(int32_t device, mvs_ctx **out, char *err, size_t errlen);
+void mvs_destroy(mvs_ctx *ctx);
+const char *mvs_last_error(const mvs_ctx *ctx);
+int mvs_synchronize(mvs_ctx *ctx);
+
+/* Progress callback: stage "depth" per finished view, like Ctx::report
+ * (dense.rs:539). Return nonzero to cancel. */
+typedef int (*mvs_progress_fn)(void *user, const char *stage, int32_t done, int32_t total);
+
+/* ---- one depth map from host views: replaces depth_map (dense.rs:316-479) ----
+ * ref + nsrc sources (the already-filtered neighbours, dense.rs:324-341),
+ * depth range = cam.depth_range (dense.rs:362). Outputs are caller-allocated
+ * host arrays of w*h (normal: 3*w*h, xyz per pixel); out_normal / out_cost may
+ * be NULL. out_depth follows dense.rs:468-477 (0 = invalid). cancel (nullable)
+ * is polled between half-sweeps. */
+int mvs_depth_map(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                  double range0, double range1, const mvs_params *prm, float *out_depth,
+                  float *out_normal, float *out_cost, const volatile int32_t *cancel);
+
+/* ---- parity entry points (lock-step driver) ----
+ * Cost of explicit (pixel, hypothesis) pairs: Ctx2::cost (dense.rs:181-263).
+ * hyp4: n x {d, n0, n1, n2}. */
+int mvs_ncc_cost(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                 double range0, double range1, const mvs_params *prm, int64_t n,
+                 const int32_t *px, const int32_t *py, const float *hyp4, float *out_cost);
+
+/* Run PatchMatch in steps on a host state of w*h x {d, n0, n1, n2, cost}
+ * (Hyp, dense.rs:156-161). first_step = -1 runs the initialisation
+ * (dense.rs:365-402) before the sweeps; half-sweep s covers iteration s/2,
+ * parity s%2 (dense.rs:413-467). textured (w*h, nullable) receives the mask.
+ * evals (nullable) receives the number of cost evaluations performed. */
+int mvs_patchmatch_steps(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                         double range0, double range1, const mvs_params *prm,
+                         int32_t first_step, int32_t n_steps, float *state, uint8_t *textured,
+                         int64_t *evals);
+
+/* ---- resident survey: views uploaded once, depth maps kept in HBM ---- */
+typedef struct mvs_survey mvs_survey;
+
+int mvs_survey_create(mvs_ctx *ctx, int32_t nviews, mvs_survey **out);
+void mvs_survey_destroy(mvs_survey *sv);
+/* Upload view i (pixels converted to device grey once). */
+int mvs_survey_set_view(mvs_survey *sv, int32_t i, const mvs_view *v);
+
+/* One reference view of the survey (the body of the loop dense.rs:530-540,
+ * without file I/O). Neighbour indices are survey view indices, already
+ * filtered. The maps stay on the device until mvs_survey_download. */
+int mvs_survey_depth_map(mvs_survey *sv, int32_t ref, const int32_t *srcs, int32_t nsrc,
+                         double range0, double range1, const mvs_params *prm);
+
+/* Many reference views in one call; jobs are described in CSR form:
+ * refs[j], srcs[src_off[j] .. src_off[j+1]), ranges[2j], ranges[2j+1].
+ * Progress/cancel per view as in dense.rs:531-539. */
+int mvs_survey_depth_maps(mvs_survey *sv, int32_t njobs, const int32_t *refs,
+                          const int32_t *src_off, const int32_t *srcs, const double *ranges,
+                          const mvs_params *prm, const volatile int32_t *cancel,
+                          mvs_progress_fn progress, void *user);
+
+int mvs_survey_download(mvs_survey *sv, int32_t view, float *depth, float *normal, float *cost);
+
+/* Geometric consistency of view ref's depth map against its neighbours' maps
+ * (fusion filter, dense.rs:575-615): neighbours = the first prm.neighbours+1 of
+ * cam.neighbours that have a depth map (the caller passes that list). rule
+ * MVS_CONS_REFERENCE: relative depth difference < 1% in >= 2 maps (>= 1 when
+ * fewer than 3 neighbours). Writes per-pixel agree counts (-1 where depth <= 0)
+ * and returns the count of consistent pixels via n_consistent. */
+enum { MVS_CONS_REFERENCE = 0 };
+int mvs_survey_consistency(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int32_t nn,
+                           int32_t rule, int32_t *agree, int64_t *n_consistent);
+
+/* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
+typedef struct {
+    int64_t sweep_launches;
+    double sweep_ms;          /* summed device time of the half-sweep kernel */
+    int64_t init_launches;
+    double init_ms;
+    int64_t evals;            /* cost evaluations counted on the device */
+    int64_t pixels;           /* reference pixels processed */
+    int64_t init_evals;       /* of which in the initialisation */
+    int64_t sweep_evals;      /* of which in the half-sweeps */
+} mvs_stats;
+
+int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
+int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
+int mvs_reset_stats(mvs_ctx *ctx);
+
+/* ---- `.depth` cache files (dense.rs:481-508) ---- */
+/* Writes u32 LE w, u32 LE h, w*h f32 LE to path + ".tmp", then renames. */
+int mvs_save_depth(const char *path, int32_t w, int32_t h, const float *depth);
+/* Reads a `.depth` file; returns MVS_ERR_IO / MVS_ERR_FORMAT like load_depth's
+ * None (wrong length). With depth == NULL only w and h are read. */
+int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,618 @@
+/*
+ * CPU oracle for the dense-MVS depth-map stage — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of crates/stitch-core/src/recon/dense.rs (PatchMatch
+ * depth maps, NCC cost, geometric consistency) and of the camera model in
+ * crates/stitch-core/src/recon/geometry.rs. Every function cites the lines it
+ * follows. Used only by the tests, the smoke check and the CPU-baseline leg of
+ * the benchmark; the product library never links or calls it.
+ *
+ * Build: -O3 -ffp-contract=off, no fast-math (the Rust reference never fuses
+ * a*b+c and evaluates left to right; C with FLT_EVAL_METHOD 0 does the same).
+ *
+ * Parity pins (see tests/test_oracle_*.py): rnd/seeds and random_normal by
+ * closed-form known answers; distort/undistort by the reference's own round-trip
+ * test (geometry.rs:335-351); gray_at, mean_sd, cost and the sweep by closed
+ * forms, invariants and brute force; the full trajectory cannot be pinned
+ * against the reference itself (Rust, not buildable here).
+ */
+#include "dense_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#if defined(__FP_FAST_FMA) && !defined(ORC_ALLOW_FMA)
+/* fine: fma availability does not matter, only contraction does */
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* scalar building blocks                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* dense.rs:108 — g = (0.299 R + 0.587 G + 0.114 B) / 255 in f32, left to right. */
+float orc_gray_rgb(uint8_t r, uint8_t g, uint8_t b) {
+    float v = 0.299f * (float)r + 0.587f * (float)g;
+    v = v + 0.114f * (float)b;
+    return v / 255.0f;
+}
+
+/* A grey u8 pixel is R = G = B (image::to_rgb8 of a luma image). */
+void orc_gray_lut(float lut[256]) {
+    for (int i = 0; i < 256; ++i) lut[i] = orc_gray_rgb((uint8_t)i, (uint8_t)i, (uint8_t)i);
+}
+
+/* dense.rs:273-279 — xorshift64 (13, 7, 17); top 24 bits / 2^24. */
+float orc_rnd(uint64_t *s) {
+    *s ^= *s << 13;
+    *s ^= *s >> 7;
+    *s ^= *s << 17;
+    return (float)(*s >> 40) / (float)(1u << 24);
+}
+
+/* dense.rs:374 */
+uint64_t orc_seed_init(uint64_t i) { return (i * 0x9E3779B97F4A7C15ull) | 1ull; }
+
+/* dense.rs:446-448: ((i << 8) ^ (2 it + parity)) * C | 1, wrapping. */
+uint64_t orc_seed_refine(uint64_t i, uint64_t it, uint64_t parity) {
+    return (((i << 8) ^ (it * 2ull + parity)) * 0x2545F4914F6CDD1Dull) | 1ull;
+}
+
+/* dense.rs:281-297 with Some(around) (every caller passes one: :376, :456). */
+void orc_random_normal(uint64_t *s, const float ray[2], const float around[3], float spread,
+                       float out[3]) {
+    float n[3];
+    for (int k = 0; k < 3; ++k) {
+        float r = orc_rnd(s);
+        float t = (r - 0.5f) * 2.0f;
+        t = t * spread;
+        n[k] = around[k] + t;
+    }
+    float facing = n[0] * ray[0] + n[1] * ray[1];
+    facing = facing + n[2];
+    if (facing > 0.0f) {
+        n[0] = -n[0];
+        n[1] = -n[1];
+        n[2] = -n[2];
+    }
+    float l2 = n[0] * n[0] + n[1] * n[1];
+    l2 = l2 + n[2] * n[2];
+    float l = sqrtf(l2);
+    if (!(l >= 1e-9f)) l = 1e-9f; /* f32::max(l, 1e-9); l is never NaN here */
+    out[0] = n[0] / l;
+    out[1] = n[1] / l;
+    out[2] = n[2] / l;
+}
+
+/* geometry.rs:26-32 — normalised -> distorted pixel, f64. */
+void orc_distort(double f, double cx, double cy, double k1, double k2, double x, double y,
+                 double out[2]) {
+    double r2 = x * x + y * y;
+    double d = 1.0 + k1 * r2;
+    d = d + (k2 * r2) * r2;
+    double fd = f * d;
+    out[0] = fd * x + cx;
+    out[1] = fd * y + cy;
+}
+
+/* geometry.rs:34-45 — pixel -> normalised, 8 fixed-point steps, f64. */
+void orc_undistort(double f, double cx, double cy, double k1, double k2, double u, double v,
+                   double out[2]) {
+    double qx = (u - cx) / f, qy = (v - cy) / f;
+    double px = qx, py = qy;
+    for (int i = 0; i < 8; ++i) {
+        double r2 = px * px + py * py;
+        double d = 1.0 + k1 * r2;
+        d = d + (k2 * r2) * r2;
+        px = qx / d;
+        py = qy / d;
+    }
+    out[0] = px;
+    out[1] = py;
+}
+
+/* dense.rs:299-309 — R = Rs * Rr^T, t = Rs * (Cr - Cs), f64 then cast to f32. */
+void orc_relative(const double Rr[9], const double Cr[3], const double Rs[9], const double Cs[3],
+                  float R[9], float t[3]) {
+    double dc[3] = {Cr[0] - Cs[0], Cr[1] - Cs[1], Cr[2] - Cs[2]};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            double acc = Rs[i * 3 + 0] * Rr[j * 3 + 0];
+            acc = acc + Rs[i * 3 + 1] * Rr[j * 3 + 1];
+            acc = acc + Rs[i * 3 + 2] * Rr[j * 3 + 2];
+            R[i * 3 + j] = (float)acc;
+        }
+        double a = Rs[i * 3 + 0] * dc[0];
+        a = a + Rs[i * 3 + 1] * dc[1];
+        a = a + Rs[i * 3 + 2] * dc[2];
+        t[i] = (float)a;
+    }
+}
+
+/* dense.rs:69-85 — bilinear grey at pixel-centre coordinates. Returns 0 for
+ * None. Non-finite coordinates are treated as None (the reference would read
+ * pixel 0 for NaN and panic for values beyond usize; neither occurs for finite
+ * projections inside the image). */
+int orc_gray_at(const float *g, int32_t w, int32_t h, float x, float y, float *out) {
+    float fx = x - 0.5f;
+    float fy = y - 0.5f;
+    if (!(fx >= 0.0f) || !(fy >= 0.0f)) return 0;
+    if (!(fx < 1.8e19f) || !(fy < 1.8e19f)) return 0;
+    uint64_t x0 = (uint64_t)fx, y0 = (uint64_t)fy;
+    if (x0 + 1 >= (uint64_t)w || y0 + 1 >= (uint64_t)h) return 0;
+    float ax = fx - (float)x0, ay = fy - (float)y0;
+    size_t i = (size_t)y0 * (size_t)w + (size_t)x0;
+    float top = g[i] * (1.0f - ax) + g[i + 1] * ax;
+    float bot = g[i + (size_t)w] * (1.0f - ax) + g[i + (size_t)w + 1] * ax;
+    *out = top * (1.0f - ay) + bot * ay;
+    return 1;
+}
+
+/* dense.rs:266-271 — two-pass mean and population standard deviation, with the
+ * window size generalised from 25 to n (sums in index order). */
+void orc_mean_sd(const float *v, int32_t n, float *m, float *sd) {
+    float s = 0.0f;
+    for (int32_t i = 0; i < n; ++i) s = s + v[i];
+    float mean = s / (float)n;
+    float var = 0.0f;
+    for (int32_t i = 0; i < n; ++i) {
+        float d = v[i] - mean;
+        var = var + d * d;
+    }
+    var = var / (float)n;
+    *m = mean;
+    *sd = sqrtf(var);
+}
+
+/* dense.rs:324-334 — neighbours in list order whose baseline exceeds
+ * 0.02 * max(range0, 1), first `take` of them. centres: ncams x 3 (f64).
+ * Returns the count written to out. */
+int orc_select_neighbours(const double *centres, int32_t ncams, int32_t ci,
+                          const int32_t *neighbours, int32_t nn, double range0, int32_t take,
+                          int32_t *out) {
+    (void)ncams;
+    int32_t k = 0;
+    double lim = 0.02 * (range0 > 1.0 ? range0 : 1.0);
+    for (int32_t a = 0; a < nn && k < take; ++a) {
+        int32_t j = neighbours[a];
+        const double *cj = centres + 3 * (size_t)j, *cc = centres + 3 * (size_t)ci;
+        double dx = cj[0] - cc[0], dy = cj[1] - cc[1], dz = cj[2] - cc[2];
+        double base = sqrt(dx * dx + dy * dy + dz * dz);
+        if (base > lim) out[k++] = j;
+    }
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* depth-map context                                                        */
+/* ------------------------------------------------------------------------ */
+
+struct orc_ctx {
+    orc_view rf;
+    int32_t nsrc;
+    orc_view *srcs;
+    float *rays;   /* w*h*2 */
+    float *Rs;     /* nsrc*9 */
+    float *ts;     /* nsrc*3 */
+    int32_t n;     /* window samples */
+    int32_t half;  /* window half footprint in pixels */
+    int32_t *wdx, *wdy;
+    orc_params prm;
+    float dmin, dmax, imin, imax;
+    float up[3];
+};
+
+orc_ctx *orc_ctx_create(const orc_view *ref, const orc_view *srcs, int32_t nsrc,
+                        const orc_params *prm, double range0, double range1) {
+    if (prm->taps < 1 || (prm->taps & 1) == 0 || prm->taps * prm->taps > ORC_MAX_TAPS)
+        return NULL;
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->rf = *ref;
+    c->nsrc = nsrc;
+    c->srcs = (orc_view *)malloc(sizeof(orc_view) * (size_t)(nsrc > 0 ? nsrc : 1));
+    memcpy(c->srcs, srcs, sizeof(orc_view) * (size_t)nsrc);
+    c->prm = *prm;
+    int32_t w = ref->w, h = ref->h;
+    /* dense.rs:343-349 — undistorted ray per pixel centre, f64 -> f32. */
+    c->rays = (float *)malloc(sizeof(float) * 2 * (size_t)w * (size_t)h);
+    for (int64_t i = 0; i < (int64_t)w * h; ++i) {
+        double p[2];
+        orc_undistort(ref->f, ref->cx, ref->cy, ref->k1, ref->k2, (double)(i % w) + 0.5,
+                      (double)(i / w) + 0.5, p);
+        c->rays[2 * i] = (float)p[0];
+        c->rays[2 * i + 1] = (float)p[1];
+    }
+    c->Rs = (float *)malloc(sizeof(float) * 9 * (size_t)(nsrc > 0 ? nsrc : 1));
+    c->ts = (float *)malloc(sizeof(float) * 3 * (size_t)(nsrc > 0 ? nsrc : 1));
+    for (int32_t s = 0; s < nsrc; ++s)
+        orc_relative(ref->R, ref->C, srcs[s].R, srcs[s].C, c->Rs + 9 * s, c->ts + 3 * s);
+    /* dense.rs:170 generalised: taps at -(taps/2)*stride .. +(taps/2)*stride. */
+    c->n = prm->taps * prm->taps;
+    c->half = (prm->taps / 2) * prm->stride;
+    c->wdx = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->n);
+    c->wdy = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->n);
+    int32_t k = 0;
+    for (int32_t a = 0; a < prm->taps; ++a)
+        for (int32_t b = 0; b < prm->taps; ++b) {
+            c->wdy[k] = (a - prm->taps / 2) * prm->stride;
+            c->wdx[k] = (b - prm->taps / 2) * prm->stride;
+            ++k;
+        }
+    /* dense.rs:362-363 */
+    c->dmin = (float)(range0 > 0.5 ? range0 : 0.5);
+    c->dmax = (float)range1;
+    c->imin = 1.0f / c->dmax;
+    c->imax = 1.0f / c->dmin;
+    /* dense.rs:367-370 — R_ref * (0,0,1) is R's third column. */
+    c->up[0] = (float)ref->R[2];
+    c->up[1] = (float)ref->R[5];
+    c->up[2] = (float)ref->R[8];
+    return c;
+}
+
+void orc_ctx_destroy(orc_ctx *c) {
+    if (!c) return;
+    free(c->srcs);
+    free(c->rays);
+    free(c->Rs);
+    free(c->ts);
+    free(c->wdx);
+    free(c->wdy);
+    free(c);
+}
+
+const float *orc_ctx_rays(const orc_ctx *c) { return c->rays; }
+
+void orc_ctx_relpose(const orc_ctx *c, int32_t s, float R[9], float t[3]) {
+    memcpy(R, c->Rs + 9 * s, 9 * sizeof(float));
+    memcpy(t, c->ts + 3 * s, 3 * sizeof(float));
+}
+
+void orc_ctx_limits(const orc_ctx *c, float out[4]) {
+    out[0] = c->dmin;
+    out[1] = c->dmax;
+    out[2] = c->imin;
+    out[3] = c->imax;
+}
+
+static int cmp_f32(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* dense.rs:181-263 — Ctx2::cost. */
+float orc_cost(const orc_ctx *c, int32_t x, int32_t y, const float hyp[4]) {
+    const int32_t w = c->rf.w, h = c->rf.h, N = c->n;
+    const float *ray = c->rays + 2 * ((size_t)y * (size_t)w + (size_t)x);
+    const float d = hyp[0], n0 = hyp[1], n1 = hyp[2], n2 = hyp[3];
+    const float xp0 = ray[0] * d, xp1 = ray[1] * d, xp2 = d;
+    float cc = n0 * xp0 + n1 * xp1;
+    cc = cc + n2 * xp2;
+    float refv[ORC_MAX_TAPS], p0[ORC_MAX_TAPS], p1[ORC_MAX_TAPS], p2[ORC_MAX_TAPS];
+    for (int32_t k = 0; k < N; ++k) {
+        int32_t qx = x + c->wdx[k], qy = y + c->wdy[k];
+        if (qx < 0 || qy < 0 || qx >= w || qy >= h) return 2.0f;
+        size_t qi = (size_t)qy * (size_t)w + (size_t)qx;
+        refv[k] = c->rf.gray[qi];
+        const float r0 = c->rays[2 * qi], r1 = c->rays[2 * qi + 1];
+        float den = n0 * r0 + n1 * r1;
+        den = den + n2;
+        if (fabsf(den) < 1e-6f) return 2.0f;
+        float dq = cc / den;
+        if (dq <= 0.0f) return 2.0f;
+        p0[k] = r0 * dq;
+        p1[k] = r1 * dq;
+        p2[k] = dq;
+    }
+    float mr, sr;
+    orc_mean_sd(refv, N, &mr, &sr);
+    if (sr < c->prm.texture_min) return 2.0f;
+    float costs_stack[16];
+    float *costs = c->nsrc <= 16 ? costs_stack : (float *)malloc(sizeof(float) * (size_t)c->nsrc);
+    float sv[ORC_MAX_TAPS];
+    for (int32_t si = 0; si < c->nsrc; ++si) {
+        costs[si] = 2.0f;
+        const float *R = c->Rs + 9 * si, *t = c->ts + 3 * si;
+        const orc_view *v = &c->srcs[si];
+        int ok = 1;
+        for (int32_t j = 0; j < N; ++j) {
+            float xs0 = R[0] * p0[j] + R[1] * p1[j];
+            xs0 = xs0 + R[2] * p2[j];
+            xs0 = xs0 + t[0];
+            float xs1 = R[3] * p0[j] + R[4] * p1[j];
+            xs1 = xs1 + R[5] * p2[j];
+            xs1 = xs1 + t[1];
+            float xs2 = R[6] * p0[j] + R[7] * p1[j];
+            xs2 = xs2 + R[8] * p2[j];
+            xs2 = xs2 + t[2];
+            if (xs2 <= 0.0f) {
+                ok = 0;
+                break;
+            }
+            double u[2];
+            orc_distort(v->f, v->cx, v->cy, v->k1, v->k2, (double)(xs0 / xs2), (double)(xs1 / xs2),
+                        u);
+            if (!orc_gray_at(v->gray, v->w, v->h, (float)u[0], (float)u[1], &sv[j])) {
+                ok = 0;
+                break;
+            }
+        }
+        if (!ok) continue;
+        float ms, ss;
+        orc_mean_sd(sv, N, &ms, &ss);
+        if (ss < 1e-3f) continue;
+        float cov = 0.0f;
+        for (int32_t j = 0; j < N; ++j) cov = cov + (refv[j] - mr) * (sv[j] - ms);
+        float den = (float)N * sr;
+        den = den * ss;
+        costs[si] = 1.0f - cov / den;
+    }
+    int32_t m = c->nsrc;
+    float r;
+    if (m <= 0) {
+        r = 2.0f;
+    } else {
+        qsort(costs, (size_t)m, sizeof(float), cmp_f32);
+        r = m >= 2 ? (costs[0] + costs[1]) * 0.5f : costs[0];
+    }
+    if (costs != costs_stack) free(costs);
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* threading: rows split over pthreads (the rayon row model, dense.rs:417)   */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*row_fn)(void *arg, int32_t y0, int32_t y1, int64_t *counter);
+typedef struct {
+    row_fn fn;
+    void *arg;
+    int32_t y0, y1;
+    int64_t count;
+} row_job;
+
+static void *row_job_main(void *p) {
+    row_job *j = (row_job *)p;
+    j->fn(j->arg, j->y0, j->y1, &j->count);
+    return NULL;
+}
+
+static int64_t run_rows(row_fn fn, void *arg, int32_t rows, int32_t nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > rows) nthreads = rows > 0 ? rows : 1;
+    row_job *jobs = (row_job *)calloc((size_t)nthreads, sizeof(row_job));
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    for (int32_t t = 0; t < nthreads; ++t) {
+        jobs[t].fn = fn;
+        jobs[t].arg = arg;
+        jobs[t].y0 = (int32_t)((int64_t)rows * t / nthreads);
+        jobs[t].y1 = (int32_t)((int64_t)rows * (t + 1) / nthreads);
+    }
+    for (int32_t t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, row_job_main, &jobs[t]);
+    row_job_main(&jobs[0]);
+    int64_t total = jobs[0].count;
+    for (int32_t t = 1; t < nthreads; ++t) {
+        pthread_join(th[t], NULL);
+        total += jobs[t].count;
+    }
+    free(jobs);
+    free(th);
+    return total;
+}
+
+typedef struct {
+    const orc_ctx *c;
+    int64_t n;
+    const int32_t *px, *py;
+    const float *hyp4;
+    float *out;
+} batch_arg;
+
+static void batch_rows(void *p, int32_t a, int32_t b, int64_t *cnt) {
+    batch_arg *ba = (batch_arg *)p;
+    for (int64_t i = a; i < b; ++i) ba->out[i] = orc_cost(ba->c, ba->px[i], ba->py[i], ba->hyp4 + 4 * i);
+    *cnt += b - a;
+}
+
+void orc_cost_batch(const orc_ctx *c, int64_t n, const int32_t *px, const int32_t *py,
+                    const float *hyp4, float *out, int32_t nthreads) {
+    batch_arg a = {c, n, px, py, hyp4, out};
+    run_rows(batch_rows, &a, (int32_t)n, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* PatchMatch                                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const orc_ctx *c;
+    float *state;
+    uint8_t *textured;
+    const uint8_t *tex_ro;
+    int32_t it, parity;
+} sweep_arg;
+
+/* dense.rs:371-381 (init) and :385-402 (texture mask). */
+static void init_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
+    sweep_arg *a = (sweep_arg *)p;
+    const orc_ctx *c = a->c;
+    const int32_t w = c->rf.w, h = c->rf.h, half = c->half;
+    float v[ORC_MAX_TAPS];
+    for (int32_t y = y0; y < y1; ++y)
+        for (int32_t x = 0; x < w; ++x) {
+            size_t i = (size_t)y * (size_t)w + (size_t)x;
+            uint64_t s = orc_seed_init((uint64_t)i);
+            float r = orc_rnd(&s);
+            float dd = c->imin + r * (c->imax - c->imin);
+            float *st = a->state + 5 * i;
+            st[0] = 1.0f / dd;
+            orc_random_normal(&s, c->rays + 2 * i, c->up, 0.3f, st + 1);
+            st[4] = orc_cost(c, x, y, st);
+            *cnt += 1;
+            uint8_t tx = 0;
+            if (!(x < half || y < half || x >= w - half || y >= h - half)) {
+                for (int32_t k = 0; k < c->n; ++k)
+                    v[k] = c->rf.gray[(size_t)(y + c->wdy[k]) * (size_t)w + (size_t)(x + c->wdx[k])];
+                float m, sd;
+                orc_mean_sd(v, c->n, &m, &sd);
+                tx = sd >= c->prm.texture_min;
+            }
+            a->textured[i] = tx;
+        }
+}
+
+void orc_init(const orc_ctx *c, float *state, uint8_t *textured, int32_t nthreads) {
+    sweep_arg a = {c, state, textured, textured, 0, 0};
+    run_rows(init_rows, &a, c->rf.h, nthreads);
+}
+
+static const int32_t CAND[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1},
+                                   {-5, 0}, {5, 0}, {0, -5}, {0, 5}}; /* dense.rs:403-412 */
+
+/* dense.rs:413-467 — one half-sweep (one parity of one iteration). Candidates
+ * are all of the opposite colour, so reading the live state equals reading the
+ * snapshot the reference clones at :416. Returns cost evaluations done. */
+static void sweep_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
+    sweep_arg *a = (sweep_arg *)p;
+    const orc_ctx *c = a->c;
+    const int32_t w = c->rf.w, h = c->rf.h;
+    const float range = ldexpf(1.0f, -a->it); /* 0.5f32.powi(it), exact */
+    const float span = c->imax - c->imin;
+    for (int32_t y = y0; y < y1; ++y)
+        for (int32_t x = 0; x < w; ++x) {
+            if ((x + y) % 2 != a->parity) continue;
+            size_t i = (size_t)y * (size_t)w + (size_t)x;
+            if (!a->tex_ro[i]) continue;
+            float best[5];
+            memcpy(best, a->state + 5 * i, sizeof best);
+            for (int k = 0; k < 8; ++k) {
+                int32_t nx = x + CAND[k][0], ny = y + CAND[k][1];
+                if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+                const float *cand = a->state + 5 * ((size_t)ny * (size_t)w + (size_t)nx);
+                if (cand[0] == best[0] && cand[1] == best[1] && cand[2] == best[2] &&
+                    cand[3] == best[3])
+                    continue;
+                float cost = orc_cost(c, x, y, cand);
+                *cnt += 1;
+                if (cost < best[4]) {
+                    memcpy(best, cand, 4 * sizeof(float));
+                    best[4] = cost;
+                }
+            }
+            uint64_t s = orc_seed_refine((uint64_t)i, (uint64_t)a->it, (uint64_t)a->parity);
+            for (int rep = 0; rep < 2; ++rep) {
+                float r = orc_rnd(&s);
+                float step = (r - 0.5f) * range;
+                step = step * 0.25f;
+                step = step * span;
+                float id = 1.0f / best[0] + step;
+                float hyp[4];
+                hyp[0] = (id > c->imin * 0.5f && id < c->imax * 1.5f) ? 1.0f / id : best[0];
+                orc_random_normal(&s, c->rays + 2 * i, best + 1, 0.25f * range, hyp + 1);
+                float cost = orc_cost(c, x, y, hyp);
+                *cnt += 1;
+                if (cost < best[4]) {
+                    memcpy(best, hyp, 4 * sizeof(float));
+                    best[4] = cost;
+                }
+            }
+            memcpy(a->state + 5 * i, best, sizeof best);
+        }
+}
+
+int64_t orc_half_sweep(const orc_ctx *c, float *state, const uint8_t *textured, int32_t it,
+                       int32_t parity, int32_t nthreads) {
+    sweep_arg a = {c, state, NULL, textured, it, parity};
+    return run_rows(sweep_rows, &a, c->rf.h, nthreads);
+}
+
+/* dense.rs:468-477 */
+void orc_threshold(const orc_ctx *c, const float *state, float *depth) {
+    const float lo = c->dmin * 0.7f, hi = c->dmax * 1.4f;
+    for (int64_t i = 0; i < (int64_t)c->rf.w * c->rf.h; ++i) {
+        const float *st = state + 5 * i;
+        depth[i] = (st[4] < c->prm.cost_max && st[0] >= lo && st[0] <= hi) ? st[0] : 0.0f;
+    }
+}
+
+int64_t orc_depth_map(const orc_ctx *c, int32_t iterations, int32_t nthreads, float *state,
+                      uint8_t *textured, float *depth) {
+    int64_t evals = 0;
+    orc_init(c, state, textured, nthreads);
+    for (int32_t it = 0; it < iterations; ++it)
+        for (int32_t parity = 0; parity < 2; ++parity)
+            evals += orc_half_sweep(c, state, textured, it, parity, nthreads);
+    if (depth) orc_threshold(c, state, depth);
+    return evals;
+}
+
+/* ------------------------------------------------------------------------ */
+/* geometric consistency (fusion's filter)                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const orc_view *ref;
+    const float *depth;
+    const orc_nbr_depth *nbrs;
+    int32_t nnbr;
+    int32_t *agree;
+    int32_t need;
+} cons_arg;
+
+/* dense.rs:585-615 */
+static void cons_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
+    cons_arg *a = (cons_arg *)p;
+    const orc_view *rf = a->ref;
+    const int32_t w = rf->w;
+    for (int32_t y = y0; y < y1; ++y)
+        for (int32_t x = 0; x < w; ++x) {
+            size_t i = (size_t)y * (size_t)w + (size_t)x;
+            float di = a->depth[i];
+            if (di <= 0.0f) {
+                a->agree[i] = -1;
+                continue;
+            }
+            double pp[2];
+            orc_undistort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, (double)x + 0.5, (double)y + 0.5,
+                          pp);
+            double xc[3] = {pp[0] * (double)di, pp[1] * (double)di, (double)di};
+            double xw[3];
+            for (int r = 0; r < 3; ++r) {
+                /* rwc = R^T: row r of R^T is column r of R */
+                double acc = rf->R[0 * 3 + r] * xc[0];
+                acc = acc + rf->R[1 * 3 + r] * xc[1];
+                acc = acc + rf->R[2 * 3 + r] * xc[2];
+                xw[r] = acc + rf->C[r];
+            }
+            int32_t agree = 0;
+            for (int32_t k = 0; k < a->nnbr; ++k) {
+                const orc_view *v = a->nbrs[k].view;
+                double dx[3] = {xw[0] - v->C[0], xw[1] - v->C[1], xw[2] - v->C[2]};
+                double xs[3];
+                for (int r = 0; r < 3; ++r) {
+                    double acc = v->R[r * 3 + 0] * dx[0];
+                    acc = acc + v->R[r * 3 + 1] * dx[1];
+                    acc = acc + v->R[r * 3 + 2] * dx[2];
+                    xs[r] = acc;
+                }
+                if (xs[2] <= 0.0) continue;
+                double u[2];
+                orc_distort(v->f, v->cx, v->cy, v->k1, v->k2, xs[0] / xs[2], xs[1] / xs[2], u);
+                double fx = floor(u[0]), fy = floor(u[1]);
+                if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)v->w) || !(fy < (double)v->h))
+                    continue;
+                int64_t ux = (int64_t)fx, uy = (int64_t)fy;
+                double dn = (double)a->nbrs[k].depth[(size_t)uy * (size_t)v->w + (size_t)ux];
+                if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) agree += 1;
+            }
+            a->agree[i] = agree;
+            if (agree >= a->need) *cnt += 1;
+        }
+}
+
+int64_t orc_consistency(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                        int32_t nnbr, int32_t *agree, int32_t nthreads) {
+    cons_arg a = {ref, depth, nbrs, nnbr, agree, nnbr >= 3 ? 2 : 1};
+    return run_rows(cons_rows, &a, ref->h, nthreads);
+}

@@ -1,0 +1,103 @@
+/*
+ * CPU oracle for the dense-MVS depth-map stage.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's
+ * PatchMatch depth-map algorithm (crates/stitch-core/src/recon/dense.rs) and of
+ * the camera model it uses (crates/stitch-core/src/recon/geometry.rs). It is the
+ * checker the GPU engine is compared against and the timed CPU baseline; it is
+ * never linked into, loaded by or called from the product library.
+ *
+ * Arithmetic follows the reference operation by operation: f32 where the Rust
+ * code uses f32, f64 where it uses f64, left-to-right evaluation, no fused
+ * multiply-add (build with -ffp-contract=off, no fast-math).
+ *
+ * Generalisations beyond the reference (defaults reproduce it exactly):
+ *   - window: `taps` per side at spacing `stride` (reference: 5 taps, stride 2,
+ *     dense.rs:170);
+ *   - any number of sources K >= 1 (reference: at most 8, dense.rs:217).
+ */
+#ifndef AERIAL_DENSE_ORACLE_H
+#define AERIAL_DENSE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_TAPS 121 /* 11 x 11 */
+
+/* A working-size view: grey plane plus camera (dense.rs:57-65, 123-154). */
+typedef struct {
+    int32_t w, h;
+    const float *gray;          /* w*h grey levels, 0..1 (dense.rs:108) */
+    double f, cx, cy, k1, k2;   /* intrinsics already scaled to w (dense.rs:139-145) */
+    double R[9];                /* world -> camera rotation, row-major (geometry.rs:47-53) */
+    double C[3];                /* camera centre, world (local ENU) metres */
+} orc_view;
+
+typedef struct {
+    int32_t taps;        /* window taps per side, odd (reference 5) */
+    int32_t stride;      /* tap spacing in pixels (reference 2) */
+    int32_t iterations;  /* PatchMatch iterations (DenseParams.iterations) */
+    float texture_min;   /* dense.rs:173 (0.008) */
+    float cost_max;      /* dense.rs:471 (0.45) */
+} orc_params;
+
+typedef struct orc_ctx orc_ctx;
+
+/* ---- scalar building blocks (each cites the reference line it restates) ---- */
+void orc_gray_lut(float lut[256]);
+float orc_gray_rgb(uint8_t r, uint8_t g, uint8_t b);
+float orc_rnd(uint64_t *s);
+uint64_t orc_seed_init(uint64_t i);
+uint64_t orc_seed_refine(uint64_t i, uint64_t it, uint64_t parity);
+void orc_random_normal(uint64_t *s, const float ray[2], const float around[3], float spread,
+                       float out[3]);
+void orc_distort(double f, double cx, double cy, double k1, double k2, double x, double y,
+                 double out[2]);
+void orc_undistort(double f, double cx, double cy, double k1, double k2, double u, double v,
+                   double out[2]);
+void orc_relative(const double Rr[9], const double Cr[3], const double Rs[9], const double Cs[3],
+                  float R[9], float t[3]);
+int orc_gray_at(const float *gray, int32_t w, int32_t h, float x, float y, float *out);
+void orc_mean_sd(const float *v, int32_t n, float *m, float *sd);
+int orc_select_neighbours(const double *centres, int32_t ncams, int32_t ci,
+                          const int32_t *neighbours, int32_t nn, double range0, int32_t take,
+                          int32_t *out);
+
+/* ---- depth-map context ---- */
+orc_ctx *orc_ctx_create(const orc_view *ref, const orc_view *srcs, int32_t nsrc,
+                        const orc_params *prm, double range0, double range1);
+void orc_ctx_destroy(orc_ctx *c);
+const float *orc_ctx_rays(const orc_ctx *c);
+void orc_ctx_relpose(const orc_ctx *c, int32_t s, float R[9], float t[3]);
+void orc_ctx_limits(const orc_ctx *c, float out[4]); /* dmin, dmax, imin, imax */
+
+/* 1 - NCC over the best two sources, 2.0 when undefined (dense.rs:181-263). */
+float orc_cost(const orc_ctx *c, int32_t x, int32_t y, const float hyp[4]);
+void orc_cost_batch(const orc_ctx *c, int64_t n, const int32_t *px, const int32_t *py,
+                    const float *hyp4, float *out, int32_t nthreads);
+
+/* state = 5 floats per pixel: d, n0, n1, n2, cost (Hyp, dense.rs:156-161). */
+void orc_init(const orc_ctx *c, float *state, uint8_t *textured, int32_t nthreads);
+int64_t orc_half_sweep(const orc_ctx *c, float *state, const uint8_t *textured, int32_t it,
+                       int32_t parity, int32_t nthreads);
+void orc_threshold(const orc_ctx *c, const float *state, float *depth);
+int64_t orc_depth_map(const orc_ctx *c, int32_t iterations, int32_t nthreads, float *state,
+                      uint8_t *textured, float *depth);
+
+/* Geometric consistency of one reference depth map against neighbour maps
+ * (dense.rs:585-615). Writes the agree count per pixel (-1 where depth <= 0)
+ * and returns the number of pixels with agree >= need. */
+typedef struct {
+    const orc_view *view;   /* camera of the neighbour (gray unused) */
+    const float *depth;     /* its depth map (w*h) */
+} orc_nbr_depth;
+int64_t orc_consistency(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                        int32_t nnbr, int32_t *agree, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
