@@ -25,7 +25,7 @@ EXPORTS = [
     "mvs_ncc_cost", "mvs_patchmatch_steps", "mvs_survey_create", "mvs_survey_destroy",
     "mvs_survey_set_view", "mvs_survey_depth_map", "mvs_survey_depth_maps",
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
-    "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth",
+    "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
 ]
 
 
@@ -99,6 +99,7 @@ def lib():
         L.mvs_set_timing.argtypes = [vp, i32]
         L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
         L.mvs_reset_stats.argtypes = [vp]
+        L.mvs_selftest.argtypes = [vp, i32, i64, C.c_uint64, P(C.c_int64)]
         L.mvs_save_depth.argtypes = [C.c_char_p, i32, i32, vp]
         L.mvs_load_depth.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, i64]
         _lib = L
@@ -268,6 +269,11 @@ class Context:
 
     def reset_stats(self):
         self.check(lib().mvs_reset_stats(self._h))
+
+    def selftest_division(self, n, seed=1):
+        bad = C.c_int64()
+        self.check(lib().mvs_selftest(self._h, 0, n, seed, C.byref(bad)))
+        return bad.value
 
 
 class Survey:

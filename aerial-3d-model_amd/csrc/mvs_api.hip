@@ -827,6 +827,27 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
     return MVS_OK;
 }
 
+int mvs_selftest(mvs_ctx *c, int32_t which, int64_t n, uint64_t seed, int64_t *mismatches) {
+    if (!c || !mismatches || n < 0 || n > (1ll << 34)) return MVS_ERR_ARG;
+    if (which != MVS_SELFTEST_DIVISION) return fail(c, MVS_ERR_ARG, "unknown self-test");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->ok_count.ensure(sizeof(unsigned long long)));
+    HIP_TRY(c, hipMemsetAsync(c->ok_count.p, 0, sizeof(unsigned long long), c->st));
+    const int64_t chunk = 1ll << 28;
+    for (int64_t done = 0; done < n; done += chunk) {
+        const int64_t m = n - done < chunk ? n - done : chunk;
+        launch_selftest_div(m, seed + (uint64_t)done * 0x9E3779B97F4A7C15ull,
+                            c->ok_count.as<unsigned long long>(), c->st);
+        HIP_TRY(c, hipGetLastError());
+    }
+    unsigned long long bad = 0;
+    HIP_TRY(c, hipMemcpyAsync(&bad, c->ok_count.p, sizeof bad, hipMemcpyDeviceToHost, c->st));
+    int rc = sync_ctx(c);
+    if (rc) return rc;
+    *mismatches = (int64_t)bad;
+    return MVS_OK;
+}
+
 int mvs_set_timing(mvs_ctx *c, int32_t enable) {
     if (!c) return MVS_ERR_ARG;
     c->timing = enable != 0;

@@ -77,6 +77,7 @@ void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                         const NbrCam *nbrs, int32_t nn, int32_t need, int32_t *agree,
                         unsigned long long *n_ok, hipStream_t st);
+void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st);
 // Lanes per reference pixel for nsrc sources (one lane per source).
 int lanes_for(int32_t nsrc);
 bool taps_supported(int32_t taps);

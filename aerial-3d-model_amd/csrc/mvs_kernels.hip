@@ -72,6 +72,36 @@ __device__ __forceinline__ void random_normal(uint64_t &s, float r0, float r1, f
     o2 = n2 / l;
 }
 
+// a0/b and a1/b, each correctly rounded, sharing one reciprocal. These are the
+// steps of the compiler's IEEE f32 division (reciprocal estimate, one
+// Newton-Raphson refinement, two residual corrections) without the range
+// scaling of v_div_scale/v_div_fixup, which is the identity for a normal
+// divisor and a normal, finite quotient. Outside that range (denormal or
+// overflowing quotients) the projected coordinate is off the image or equal to
+// the principal point whichever way the quotient rounds; mvs_selftest checks
+// the equality with `/` on the device.
+__device__ __forceinline__ void div2_shared(const float a0, const float a1, const float b,
+                                            float &q0, float &q1) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = fmaf(-b, r, 1.0f);
+    r = fmaf(e, r, r);
+    float q = a0 * r;
+    float res = fmaf(-b, q, a0);
+    q = fmaf(res, r, q);
+    res = fmaf(-b, q, a0);
+    q0 = fmaf(res, r, q);
+    q = a1 * r;
+    res = fmaf(-b, q, a1);
+    q = fmaf(res, r, q);
+    res = fmaf(-b, q, a1);
+    q1 = fmaf(res, r, q);
+}
+
+// Global-address-space view of a device pointer (global_load rather than
+// flat_load).
+typedef const __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ gfloat *as_global(const float *p) { return (gfloat *)p; }
+
 // float-float helpers for the general (distorted / non-f32 intrinsics) path
 struct ff {
     float hi, lo;
@@ -366,6 +396,11 @@ __global__ __launch_bounds__(64) void k_patchmatch(RefJob job, int32_t it, int32
         float cs = has_src ? 2.0f : INFINITY;
         if (act && !gfail && has_src) {
             // dense.rs:218-253 for source l.
+            // The reference stops at a source's first failing sample
+            // (dense.rs:227-240) and scores it 2.0. Here every sample is
+            // computed without branches (a failing one reads a safe address)
+            // and `ok` collects the failures, which yields the same cost while
+            // letting the gathers of successive taps overlap.
             const float4 *P = Ps[g];
             float sv[N];
             float sum = 0.0f;
@@ -382,25 +417,24 @@ __global__ __launch_bounds__(64) void k_patchmatch(RefJob job, int32_t it, int32
                 float xs2 = R6 * p.x + R7 * p.y;
                 xs2 = xs2 + R8 * p.z;
                 xs2 = xs2 + T2;
-                if (!(xs2 > 0.0f)) {
-                    ok = false;
-                    break;
-                }
-                const float q0 = xs0 / xs2, q1 = xs1 / xs2;
+                float q0, q1;
+                div2_shared(xs0, xs1, xs2, q0, q1);
                 float u, v;
                 if (PINHOLE)
                     project_pinhole(q0, q1, F, CX, CY, u, v);
                 else
                     project_general(q0, q1, FF, FCX, FCY, FK1, FK2, u, v);
-                // gray_at, dense.rs:69-85
+                // gray_at, dense.rs:69-85: None unless 0 <= fx < w-1 and
+                // 0 <= fy < h-1 (NaN fails too)
                 const float fx = u - 0.5f, fy = v - 0.5f;
-                if (!(fx >= 0.0f && fy >= 0.0f && fx < wm1 && fy < hm1)) {
-                    ok = false;
-                    break;
-                }
-                const int x0 = (int)fx, y0 = (int)fy;
-                const float ax = fx - (float)x0, ay = fy - (float)y0;
-                const float *gp = G + ((int64_t)y0 * sw + x0);
+                const bool good =
+                    (xs2 > 0.0f) && fx >= 0.0f && fy >= 0.0f && fx < wm1 && fy < hm1;
+                ok = ok && good;
+                const float sx = good ? fx : 0.0f, sy = good ? fy : 0.0f;
+                const int x0 = (int)sx, y0 = (int)sy;
+                // fx >= 0: fract is fx - floor(fx) = fx - x0, exactly
+                const float ax = __builtin_amdgcn_fractf(sx), ay = __builtin_amdgcn_fractf(sy);
+                gfloat *gp = as_global(G) + (y0 * sw + x0);
                 const float g00 = gp[0], g10 = gp[1], g01 = gp[sw], g11 = gp[sw + 1];
                 const float bx = 1.0f - ax, by = 1.0f - ay;
                 float top = g00 * bx;
@@ -508,6 +542,42 @@ __global__ __launch_bounds__(64) void k_patchmatch(RefJob job, int32_t it, int32
         for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off);
         if (lane == 0 && e) atomicAdd(evals, e);
     }
+}
+
+// Device self-check of div2_shared against IEEE `/`: n hashed (a0, a1, b)
+// triples, half drawn log-uniformly over [2^-40, 2^40] with random signs on
+// the numerators, half from the projection's working range (|a| < 4000,
+// b in [0.05, 2000]). Counts quotients that differ in any bit.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float unit24(uint64_t h) { return (float)(uint32_t)(h >> 40) * 0x1p-24f; }
+
+__global__ void k_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nbad = 0;
+    if (i < n) {
+        const uint64_t h0 = mix64(seed ^ (uint64_t)i), h1 = mix64(h0), h2 = mix64(h1),
+                       h3 = mix64(h2);
+        float a0, a1, b;
+        if (i & 1) {
+            a0 = exp2f(80.0f * unit24(h0) - 40.0f) * ((h3 & 1) ? -1.0f : 1.0f);
+            a1 = exp2f(80.0f * unit24(h1) - 40.0f) * ((h3 & 2) ? -1.0f : 1.0f);
+            b = exp2f(80.0f * unit24(h2) - 40.0f);
+        } else {
+            a0 = 8000.0f * unit24(h0) - 4000.0f;
+            a1 = 8000.0f * unit24(h1) - 4000.0f;
+            b = 0.05f + 2000.0f * unit24(h2);
+        }
+        float q0, q1;
+        div2_shared(a0, a1, b, q0, q1);
+        const float r0 = a0 / b, r1 = a1 / b;
+        nbad = (__float_as_uint(q0) != __float_as_uint(r0)) + (__float_as_uint(q1) != __float_as_uint(r1));
+    }
+    if (nbad) atomicAdd(bad, nbad);
 }
 
 // dense.rs:468-477, plus the normals of the final hypotheses.
@@ -692,6 +762,11 @@ int launch_cost_list(const RefJob &job, int32_t taps, int64_t n, const int32_t *
                      const int32_t *py, const float4 *hyp, float *out, hipStream_t st) {
     return dispatch_pm<MODE_COST_LIST>(job, taps, n, 0, 0, 1.0f, nullptr, px, py, hyp, out, st,
                                        job.pinhole != 0);
+}
+
+void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st) {
+    hipLaunchKernelGGL(k_selftest_div, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                       seed, bad);
 }
 
 void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_t st) {

@@ -180,6 +180,14 @@ int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
 int mvs_reset_stats(mvs_ctx *ctx);
 
+/* ---- device self-checks (no reference counterpart) ----
+ * MVS_SELFTEST_DIVISION: the kernels' shared-reciprocal division against IEEE
+ * f32 division on n hashed operand triples; *mismatches = differing quotients
+ * (0 expected). Lets a deployment confirm the arithmetic contract on its own
+ * device and driver. */
+enum { MVS_SELFTEST_DIVISION = 0 };
+int mvs_selftest(mvs_ctx *ctx, int32_t which, int64_t n, uint64_t seed, int64_t *mismatches);
+
 /* ---- `.depth` cache files (dense.rs:481-508) ---- */
 /* Writes u32 LE w, u32 LE h, w*h f32 LE to path + ".tmp", then renames. */
 int mvs_save_depth(const char *path, int32_t w, int32_t h, const float *depth);

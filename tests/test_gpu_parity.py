@@ -232,3 +232,11 @@ def test_edge_cases(gpu_ctx, tiny_survey):
     flat.gray = np.full_like(rv.gray, 90)
     d, n, c = gpu_ctx.depth_map(flat, [s.views[j] for j in s.neighbours[ref][:4]], 40, 80, prm)
     assert np.all(d == 0) and np.all(c == 2.0)
+
+
+def test_shared_reciprocal_division_is_ieee(gpu_ctx):
+    """The kernels' two-quotient division equals IEEE f32 `/` bit for bit."""
+    _say("case selftest division")
+    bad = gpu_ctx.selftest_division(1 << 30, seed=12345)
+    _say(f"division mismatches: {bad} of {2 << 30}")
+    assert bad == 0
