@@ -44,10 +44,28 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+UNITS = ["mvs_api.hip", "mvs_kernels.hip", "mvs_pm_t5.hip", "mvs_pm_t7.hip", "mvs_pm_t11.hip"]
+
+
 def build_mvs(extra=None):
-    srcs = [os.path.join(CSRC, "mvs_kernels.hip"), os.path.join(CSRC, "mvs_api.hip")]
+    """Compile the units in parallel into objects, then link the library."""
+    objdir = os.path.join(SRC_ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"] + (extra or [])
+    procs = []
+    for u in UNITS:
+        obj = os.path.join(objdir, u.replace(".hip", ".o"))
+        cmd = [HIPCC] + cflags + ["-c", os.path.join(CSRC, u), "-o", obj]
+        print("+ " + " ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), obj))
+    objs = []
+    for p, obj in procs:
+        if p.wait() != 0:
+            raise RuntimeError(f"hipcc failed for {obj}")
+        objs.append(obj)
     tmp = LIB_MVS + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + (extra or []) + srcs + ["-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-fno-gpu-rdc"] + objs +
+         ["-o", tmp])
     os.replace(tmp, LIB_MVS)
     return LIB_MVS
 

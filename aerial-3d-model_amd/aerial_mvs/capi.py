@@ -26,6 +26,7 @@ EXPORTS = [
     "mvs_survey_set_view", "mvs_survey_depth_map", "mvs_survey_depth_maps",
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
+    "mvs_set_schedule",
 ]
 
 
@@ -47,7 +48,8 @@ class MvsStats(C.Structure):
     _fields_ = [("sweep_launches", C.c_int64), ("sweep_ms", C.c_double),
                 ("init_launches", C.c_int64), ("init_ms", C.c_double),
                 ("evals", C.c_int64), ("pixels", C.c_int64),
-                ("init_evals", C.c_int64), ("sweep_evals", C.c_int64)]
+                ("init_evals", C.c_int64), ("sweep_evals", C.c_int64),
+                ("split", C.c_int32), ("gpc", C.c_int32)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)
@@ -99,6 +101,7 @@ def lib():
         L.mvs_set_timing.argtypes = [vp, i32]
         L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
         L.mvs_reset_stats.argtypes = [vp]
+        L.mvs_set_schedule.argtypes = [vp, i32, i32, i32]
         L.mvs_selftest.argtypes = [vp, i32, i64, C.c_uint64, P(C.c_int64)]
         L.mvs_save_depth.argtypes = [C.c_char_p, i32, i32, vp]
         L.mvs_load_depth.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, i64]
@@ -269,6 +272,9 @@ class Context:
 
     def reset_stats(self):
         self.check(lib().mvs_reset_stats(self._h))
+
+    def set_schedule(self, split=0, gpc=0, autotune=True):
+        self.check(lib().mvs_set_schedule(self._h, split, gpc, 1 if autotune else 0))
 
     def selftest_division(self, n, seed=1):
         bad = C.c_int64()

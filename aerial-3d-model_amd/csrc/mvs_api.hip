@@ -1,12 +1,14 @@
 // C-ABI layer of the dense-MVS depth engine: contexts, device buffers, the
-// resident survey, and the host logic of the reference's depth stage
-// (crates/stitch-core/src/recon/dense.rs) around the gfx950 kernels.
+// resident survey, batching of reference views, schedule choice, and the host
+// logic of the reference's depth stage (crates/stitch-core/src/recon/dense.rs)
+// around the gfx950 kernels.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <tuple>
@@ -61,7 +63,27 @@ struct MapDev {
     bool valid = false;
 };
 
+// Work buffers of one job slot of a batch.
+struct JobWork {
+    DevBuf hyp, cost, textured, list, counts, srcs;
+};
+
+// One reference view to process.
+struct JobSpec {
+    const ViewDev *ref = nullptr;
+    std::vector<const ViewDev *> srcs;
+    double range0 = 0, range1 = 0;
+    MapDev *out = nullptr;  // thresholded outputs (nullable)
+};
+
 typedef std::tuple<double, double, double, double, double, int32_t, int32_t> RayKey;
+// taps, source lanes, pinhole, log2 pixels, iteration
+typedef std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t> TuneKey;
+
+struct Schedule {
+    int32_t split = 1;
+    int32_t gpc = 8;
+};
 
 struct TimedLaunch {
     hipEvent_t a, b;
@@ -72,16 +94,23 @@ struct TimedLaunch {
 
 struct mvs_ctx {
     int32_t device = 0;
+    int32_t num_cu = 256;
     hipStream_t st = nullptr;
     std::string err;
-    // work buffers of the current reference view
-    DevBuf hyp, cost, textured, list, counts, srcs, evals, ok_count;
-    DevBuf up_u8;  // staging of u8 uploads
-    // host-view calls
-    std::vector<ViewDev> tmp_views;
+    std::vector<JobWork> work;  // job slots
+    DevBuf jobs_dev, prefix, counters, evals, ok_count;
+    DevBuf up_u8;                    // staging of u8 uploads
+    std::vector<ViewDev> tmp_views;  // host-view calls
     MapDev tmp_map;
-    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree;
+    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
     std::map<RayKey, DevBuf *> rays;
+    // host copies that must outlive the asynchronous uploads of a call
+    std::vector<RefJob> host_jobs;
+    std::vector<std::vector<SrcCam>> host_tables;
+    // schedule: forced (split > 0) or tuned per class
+    Schedule forced{0, 0};
+    bool autotune = true;
+    std::map<TuneKey, Schedule> tuned;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -93,7 +122,6 @@ struct mvs_survey {
     mvs_ctx *ctx = nullptr;
     std::vector<ViewDev> views;
     std::vector<MapDev> maps;
-    std::vector<DevBuf> src_tables;  // per reference view
 };
 
 namespace {
@@ -108,10 +136,16 @@ int hip_fail(mvs_ctx *c, hipError_t e, const char *what) {
     return fail(c, MVS_ERR_DEVICE, m);
 }
 
-#define HIP_TRY(c, expr)                                   \
-    do {                                                   \
-        hipError_t _e = (expr);                            \
+#define HIP_TRY(c, expr)                                     \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
         if (_e != hipSuccess) return hip_fail(c, _e, #expr); \
+    } while (0)
+
+#define TRY(expr)            \
+    do {                     \
+        int _rc = (expr);    \
+        if (_rc) return _rc; \
     } while (0)
 
 hipEvent_t take_event(mvs_ctx *c) {
@@ -155,6 +189,8 @@ int sync_ctx(mvs_ctx *c) {
     c->stats.init_evals += (int64_t)ev[0];
     c->stats.sweep_evals += (int64_t)ev[1];
     c->stats.evals += (int64_t)(ev[0] + ev[1]);
+    c->host_jobs.clear();
+    c->host_tables.clear();
     return MVS_OK;
 }
 
@@ -173,32 +209,34 @@ int check_view(mvs_ctx *c, const mvs_view *v, const char *what) {
 int check_params(mvs_ctx *c, const mvs_params *p) {
     if (!p) return fail(c, MVS_ERR_ARG, "null params");
     if (!taps_supported(p->taps))
-        return fail(c, MVS_ERR_ARG, "window taps must be 5 or 7 (got " + std::to_string(p->taps) + ")");
+        return fail(c, MVS_ERR_ARG,
+                    "window taps must be 5, 7 or 11 (got " + std::to_string(p->taps) + ")");
     if (p->stride < 1 || p->stride > 8) return fail(c, MVS_ERR_ARG, "tap stride must be 1..8");
-    if (p->iterations < 0 || p->iterations > 64) return fail(c, MVS_ERR_ARG, "iterations out of range");
+    if (p->iterations < 0 || p->iterations > 64)
+        return fail(c, MVS_ERR_ARG, "iterations out of range");
     return MVS_OK;
 }
 
 // Upload a host view into a ViewDev (grey converted to f32 on the device).
 int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v) {
-    int rc = check_view(c, v, "view");
-    if (rc) return rc;
+    TRY(check_view(c, v, "view"));
     const size_t n = (size_t)v->w * v->h;
     HIP_TRY(c, dv.gray.ensure(n * sizeof(float)));
     if (v->format == MVS_GRAY_U8) {
         const size_t bytes = (size_t)v->stride * v->h;
         HIP_TRY(c, c->up_u8.ensure(bytes));
         HIP_TRY(c, hipMemcpyAsync(c->up_u8.p, v->pixels, bytes, hipMemcpyHostToDevice, c->st));
-        launch_gray_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride, dv.gray.as<float>(), c->st);
+        launch_gray_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride, dv.gray.as<float>(),
+                            c->st);
         HIP_TRY(c, hipGetLastError());
-        // staging buffer is reused by the next upload: wait for the conversion
-        HIP_TRY(c, hipStreamSynchronize(c->st));
     } else {
         HIP_TRY(c, hipMemcpy2DAsync(dv.gray.p, (size_t)v->w * sizeof(float), v->pixels,
-                                    (size_t)v->stride * sizeof(float), (size_t)v->w * sizeof(float),
-                                    v->h, hipMemcpyHostToDevice, c->st));
-        HIP_TRY(c, hipStreamSynchronize(c->st));
+                                    (size_t)v->stride * sizeof(float),
+                                    (size_t)v->w * sizeof(float), v->h, hipMemcpyHostToDevice,
+                                    c->st));
     }
+    // the staging buffer and the caller's pixels are reused after return
+    HIP_TRY(c, hipStreamSynchronize(c->st));
     dv.w = v->w;
     dv.h = v->h;
     dv.f = v->f;
@@ -251,7 +289,7 @@ void relative_pose(const double Rr[9], const double Cr[3], const double Rs[9], c
     }
 }
 
-void split(double v, float &hi, float &lo) {
+void split_f64(double v, float &hi, float &lo) {
     hi = (float)v;
     lo = (float)(v - (double)hi);
 }
@@ -260,11 +298,11 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
     SrcCam c;
     memset(&c, 0, sizeof c);
     relative_pose(ref.R, ref.C, s.R, s.C, c.R, c.t);
-    split(s.f, c.f, c.f_lo);
-    split(s.cx, c.cx, c.cx_lo);
-    split(s.cy, c.cy, c.cy_lo);
-    split(s.k1, c.k1, c.k1_lo);
-    split(s.k2, c.k2, c.k2_lo);
+    split_f64(s.f, c.f, c.f_lo);
+    split_f64(s.cx, c.cx, c.cx_lo);
+    split_f64(s.cy, c.cy, c.cy_lo);
+    split_f64(s.k1, c.k1, c.k1_lo);
+    split_f64(s.k2, c.k2, c.k2_lo);
     c.gray = s.gray.as<float>();
     c.w = s.w;
     c.h = s.h;
@@ -273,115 +311,6 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
                           ? 1
                           : 0;
     return c;
-}
-
-// Everything of one reference view's PatchMatch, enqueued on the context
-// stream. first_step = -1 initialises; the half-sweeps run from max(first, 0)
-// for n_steps. Uses the context work buffers; `table` receives the source table.
-int run_map(mvs_ctx *c, const ViewDev &ref, const std::vector<const ViewDev *> &srcs,
-            double range0, double range1, const mvs_params *prm, int32_t first_step,
-            int32_t n_steps, DevBuf &table, std::vector<SrcCam> &host_table,
-            const volatile int32_t *cancel, RefJob *job_out) {
-    const int32_t K = (int32_t)srcs.size();
-    if (K < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
-    if (lanes_for(K) < 0) return fail(c, MVS_ERR_ARG, "at most 16 source views per reference view");
-    const int32_t w = ref.w, h = ref.h;
-    const int64_t n = (int64_t)w * h;
-    const int32_t half = (prm->taps / 2) * prm->stride;
-    if (w <= 2 * half || h <= 2 * half) return fail(c, MVS_ERR_ARG, "image smaller than the window");
-    const int64_t cap = (n + 1) / 2;
-    HIP_TRY(c, c->hyp.ensure(n * sizeof(float4)));
-    HIP_TRY(c, c->cost.ensure(n * sizeof(float)));
-    HIP_TRY(c, c->textured.ensure(n));
-    HIP_TRY(c, c->list.ensure(2 * cap * sizeof(int32_t)));
-    HIP_TRY(c, c->counts.ensure(4 * sizeof(int32_t)));
-    if (!c->evals.p) {
-        HIP_TRY(c, c->evals.ensure(2 * sizeof(unsigned long long)));
-        HIP_TRY(c, hipMemset(c->evals.p, 0, 2 * sizeof(unsigned long long)));
-    }
-    host_table.clear();
-    bool pinhole = true;
-    for (const ViewDev *s : srcs) {
-        host_table.push_back(make_src(ref, *s));
-        pinhole = pinhole && host_table.back().pinhole_exact;
-    }
-    HIP_TRY(c, table.ensure(sizeof(SrcCam) * (size_t)K));
-    HIP_TRY(c, hipMemcpyAsync(table.p, host_table.data(), sizeof(SrcCam) * (size_t)K,
-                              hipMemcpyHostToDevice, c->st));
-    const float2 *rays = nullptr;
-    int rc = get_rays(c, ref, &rays);
-    if (rc) return rc;
-
-    RefJob job;
-    memset(&job, 0, sizeof job);
-    job.gray = ref.gray.as<float>();
-    job.rays = rays;
-    job.hyp = c->hyp.as<float4>();
-    job.cost = c->cost.as<float>();
-    job.textured = c->textured.as<uint8_t>();
-    job.list = c->list.as<int32_t>();
-    job.counts = c->counts.as<int32_t>();
-    job.srcs = table.as<SrcCam>();
-    job.w = w;
-    job.h = h;
-    job.nsrc = K;
-    job.list_cap = (int32_t)cap;
-    // dense.rs:362-363
-    job.dmin = (float)(range0 > 0.5 ? range0 : 0.5);
-    job.dmax = (float)range1;
-    job.imin = 1.0f / job.dmax;
-    job.imax = 1.0f / job.dmin;
-    // dense.rs:367-370: R_ref * (0, 0, 1)
-    job.up0 = (float)ref.R[2];
-    job.up1 = (float)ref.R[5];
-    job.up2 = (float)ref.R[8];
-    job.texture_min = prm->texture_min;
-    job.cost_max = prm->cost_max;
-    job.stride = prm->stride;
-    job.half = half;
-    job.pinhole = pinhole ? 1 : 0;
-    if (job_out) *job_out = job;
-
-    unsigned long long *evals = c->evals.as<unsigned long long>();
-    HIP_TRY(c, hipMemsetAsync(c->counts.p, 0, 4 * sizeof(int32_t), c->st));
-    {
-        hipEvent_t a = nullptr, b = nullptr;
-        if (c->timing) {
-            a = take_event(c);
-            b = take_event(c);
-            HIP_TRY(c, hipEventRecord(a, c->st));
-        }
-        launch_init(job, prm->taps, first_step < 0 ? 1 : 0, c->st);
-        HIP_TRY(c, hipGetLastError());
-        if (first_step < 0) {
-            if (launch_eval_current(job, prm->taps, evals + 0, c->st))
-                return fail(c, MVS_ERR_ARG, "unsupported window/sources combination");
-            HIP_TRY(c, hipGetLastError());
-        }
-        if (c->timing) {
-            HIP_TRY(c, hipEventRecord(b, c->st));
-            c->pending.push_back({a, b, 1});
-        }
-    }
-    const int32_t s0 = first_step < 0 ? 0 : first_step;
-    for (int32_t s = s0; s < s0 + n_steps; ++s) {
-        if (cancel && *cancel) return fail(c, MVS_ERR_CANCELLED, "cancelled");
-        hipEvent_t a = nullptr, b = nullptr;
-        if (c->timing) {
-            a = take_event(c);
-            b = take_event(c);
-            HIP_TRY(c, hipEventRecord(a, c->st));
-        }
-        if (launch_sweep(job, prm->taps, s / 2, s % 2, evals + 1, c->st))
-            return fail(c, MVS_ERR_ARG, "unsupported window/sources combination");
-        HIP_TRY(c, hipGetLastError());
-        if (c->timing) {
-            HIP_TRY(c, hipEventRecord(b, c->st));
-            c->pending.push_back({a, b, 0});
-        }
-    }
-    c->stats.pixels += n;
-    return MVS_OK;
 }
 
 int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
@@ -394,24 +323,337 @@ int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
     return MVS_OK;
 }
 
-// Threshold (dense.rs:468-477) into a map and keep the final costs.
-int finish_map(mvs_ctx *c, const RefJob &job, MapDev &m) {
-    int rc = ensure_map(c, m, job.w, job.h);
-    if (rc) return rc;
-    launch_threshold(job, m.depth.as<float>(), m.normal.as<float>(), c->st);
+// A batch prepared on the device: jobs of equal kernel class are contiguous
+// in the device job table.
+struct KClass {
+    int32_t srcl = 0, pinhole = 1, first = 0, count = 0;
+    int64_t cap_sweep = 0, cap_eval = 0;  // upper bounds of the list lengths
+    int32_t size = 0;                     // log2 pixels of the first job
+    Schedule sch;
+};
+
+int32_t size_bucket(int64_t px) {
+    int32_t b = 0;
+    while ((1ll << (b + 1)) <= px) ++b;
+    return b;
+}
+
+PmArgs base_args(mvs_ctx *c, const KClass &k) {
+    PmArgs a;
+    memset(&a, 0, sizeof a);
+    a.jobs = c->jobs_dev.as<RefJob>() + k.first;
+    a.njobs = k.count;
+    a.prefix = c->prefix.as<int32_t>();
+    a.counters = c->counters.as<int32_t>();
+    a.gpc = k.sch.gpc;
+    a.range = 1.0f;
+    return a;
+}
+
+int32_t grid_for(mvs_ctx *c, int64_t items, int32_t srcl, const Schedule &s) {
+    const int64_t pix = 64 / (srcl * s.split);
+    const int64_t chunks = (items + pix * s.gpc - 1) / (pix * s.gpc);
+    const int64_t resident = (int64_t)c->num_cu * 16;
+    return (int32_t)std::max<int64_t>(1, std::min(chunks, resident));
+}
+
+int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs &a,
+              int64_t items) {
+    const int32_t grid = grid_for(c, items, k.srcl, k.sch);
+    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, a, grid, c->st))
+        return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
     HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipMemcpyAsync(m.cost.p, job.cost, sizeof(float) * (size_t)job.w * job.h,
-                              hipMemcpyDeviceToDevice, c->st));
-    m.valid = true;
+    return MVS_OK;
+}
+
+// Candidate schedules of a class: tap split x chunk size.
+std::vector<Schedule> candidates(int32_t taps, int32_t srcl) {
+    int32_t sp[4];
+    const int ns = splits_for(taps, sp);
+    std::vector<Schedule> cand;
+    for (int i = 0; i < ns; ++i) {
+        if (srcl * sp[i] > 64) continue;
+        for (int32_t gpc : {4, 16}) cand.push_back(Schedule{sp[i], gpc});
+    }
+    return cand;
+}
+
+// Time the half-sweep (it, parity) of the first job of a class per candidate
+// schedule on a copy of its state, keep the fastest. All schedules give
+// bit-identical results; only their speed differs, and it differs between the
+// first iterations (random planes, scattered gathers) and the converged ones,
+// so the choice is made per iteration.
+int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_t it,
+                   int32_t parity) {
+    const std::vector<Schedule> cand = candidates(taps, k.srcl);
+    if (cand.empty()) return fail(c, MVS_ERR_ARG, "no schedule for this class");
+    if (cand.size() == 1) {
+        k.sch = cand[0];
+        return MVS_OK;
+    }
+    const size_t n = (size_t)j0.w * j0.h;
+    HIP_TRY(c, c->save_hyp.ensure(n * sizeof(float4)));
+    HIP_TRY(c, c->save_cost.ensure(n * sizeof(float)));
+    HIP_TRY(c, hipMemcpyAsync(c->save_hyp.p, j0.hyp, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                              c->st));
+    HIP_TRY(c, hipMemcpyAsync(c->save_cost.p, j0.cost, n * sizeof(float), hipMemcpyDeviceToDevice,
+                              c->st));
+    hipEvent_t ea = take_event(c), eb = take_event(c);
+    float best_ms = 1e30f;
+    Schedule best = cand[0];
+    KClass one = k;
+    one.count = 1;
+    one.cap_sweep = ((int64_t)j0.w * j0.h + 1) / 2;
+    for (const Schedule &s : cand) {
+        one.sch = s;
+        PmArgs a = base_args(c, one);
+        a.it = it;
+        a.parity = parity;
+        a.range = ldexpf(1.0f, -it);
+        HIP_TRY(c, hipMemcpyAsync(j0.hyp, c->save_hyp.p, n * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, c->st));
+        HIP_TRY(c, hipMemcpyAsync(j0.cost, c->save_cost.p, n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, c->st));
+        HIP_TRY(c, hipEventRecord(ea, c->st));
+        TRY(pm_launch(c, PM_SWEEP, taps, one, a, one.cap_sweep));
+        HIP_TRY(c, hipEventRecord(eb, c->st));
+        HIP_TRY(c, hipEventSynchronize(eb));
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, ea, eb));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = s;
+        }
+    }
+    HIP_TRY(c, hipMemcpyAsync(j0.hyp, c->save_hyp.p, n * sizeof(float4), hipMemcpyDeviceToDevice,
+                              c->st));
+    HIP_TRY(c, hipMemcpyAsync(j0.cost, c->save_cost.p, n * sizeof(float), hipMemcpyDeviceToDevice,
+                              c->st));
+    c->free_events.push_back(ea);
+    c->free_events.push_back(eb);
+    k.sch = best;
+    return MVS_OK;
+}
+
+Schedule forced_schedule(mvs_ctx *c, int32_t taps, int32_t srcl) {
+    const std::vector<Schedule> cand = candidates(taps, srcl);
+    Schedule s = c->forced;
+    if (s.gpc <= 0) s.gpc = 8;
+    bool ok = false;
+    for (const Schedule &x : cand) ok = ok || x.split == s.split;
+    if (!ok) s.split = cand.empty() ? 1 : cand[0].split;
+    return s;
+}
+
+// Enqueue PatchMatch for a batch of reference views. first_step = -1 runs the
+// initialisation (dense.rs:365-402) then n_steps half-sweeps from 0;
+// first_step >= 0 keeps the state already in the job slots and runs half-sweeps
+// first_step .. first_step + n_steps - 1.
+int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *prm,
+              int32_t first_step, int32_t n_steps, const volatile int32_t *cancel,
+              std::vector<KClass> *classes_out) {
+    const int32_t nj = (int32_t)jobs.size();
+    if (nj == 0) return MVS_OK;
+    const int32_t taps = prm->taps, half = (taps / 2) * prm->stride;
+    for (const JobSpec &j : jobs) {
+        const int32_t K = (int32_t)j.srcs.size();
+        if (K < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
+        if (lanes_for(K) < 0)
+            return fail(c, MVS_ERR_ARG, "at most 16 source views per reference view");
+        if (j.ref->w <= 2 * half || j.ref->h <= 2 * half)
+            return fail(c, MVS_ERR_ARG, "image smaller than the window");
+    }
+    if ((int32_t)c->work.size() < nj) c->work.resize((size_t)nj);
+    if (!c->evals.p) {
+        HIP_TRY(c, c->evals.ensure(2 * sizeof(unsigned long long)));
+        HIP_TRY(c, hipMemset(c->evals.p, 0, 2 * sizeof(unsigned long long)));
+    }
+    HIP_TRY(c, c->prefix.ensure(sizeof(int32_t) * 9 * ((size_t)nj + 1)));
+    HIP_TRY(c, c->counters.ensure(sizeof(int32_t) * 8));
+
+    // host job descriptions
+    std::vector<RefJob> hj((size_t)nj);
+    std::vector<int32_t> srcl((size_t)nj), pin((size_t)nj);
+    for (int32_t i = 0; i < nj; ++i) {
+        const JobSpec &js = jobs[(size_t)i];
+        JobWork &wk = c->work[(size_t)i];
+        const ViewDev &ref = *js.ref;
+        const int64_t n = (int64_t)ref.w * ref.h, cap = (n + 1) / 2;
+        const int32_t K = (int32_t)js.srcs.size();
+        HIP_TRY(c, wk.hyp.ensure((size_t)n * sizeof(float4)));
+        HIP_TRY(c, wk.cost.ensure((size_t)n * sizeof(float)));
+        HIP_TRY(c, wk.textured.ensure((size_t)n));
+        HIP_TRY(c, wk.list.ensure((size_t)(2 * cap) * sizeof(int32_t)));
+        HIP_TRY(c, wk.counts.ensure(4 * sizeof(int32_t)));
+        HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
+        c->host_tables.emplace_back();
+        std::vector<SrcCam> &tab = c->host_tables.back();
+        bool pinhole = true;
+        for (const ViewDev *s : js.srcs) {
+            tab.push_back(make_src(ref, *s));
+            pinhole = pinhole && tab.back().pinhole_exact;
+        }
+        HIP_TRY(c, hipMemcpyAsync(wk.srcs.p, tab.data(), sizeof(SrcCam) * (size_t)K,
+                                  hipMemcpyHostToDevice, c->st));
+        const float2 *rays = nullptr;
+        TRY(get_rays(c, ref, &rays));
+        RefJob &job = hj[(size_t)i];
+        memset(&job, 0, sizeof job);
+        job.gray = ref.gray.as<float>();
+        job.rays = rays;
+        job.hyp = wk.hyp.as<float4>();
+        job.cost = wk.cost.as<float>();
+        job.textured = wk.textured.as<uint8_t>();
+        job.list = wk.list.as<int32_t>();
+        job.counts = wk.counts.as<int32_t>();
+        job.srcs = wk.srcs.as<SrcCam>();
+        job.w = ref.w;
+        job.h = ref.h;
+        job.nsrc = K;
+        job.list_cap = (int32_t)cap;
+        // dense.rs:362-363
+        job.dmin = (float)(js.range0 > 0.5 ? js.range0 : 0.5);
+        job.dmax = (float)js.range1;
+        job.imin = 1.0f / job.dmax;
+        job.imax = 1.0f / job.dmin;
+        // dense.rs:367-370: R_ref * (0, 0, 1)
+        job.up0 = (float)ref.R[2];
+        job.up1 = (float)ref.R[5];
+        job.up2 = (float)ref.R[8];
+        job.texture_min = prm->texture_min;
+        job.cost_max = prm->cost_max;
+        job.stride = prm->stride;
+        job.half = half;
+        job.pinhole = pinhole ? 1 : 0;
+        srcl[(size_t)i] = lanes_for(K);
+        pin[(size_t)i] = pinhole ? 1 : 0;
+    }
+
+    // group jobs by kernel class (source lanes, projection)
+    std::vector<KClass> classes;
+    std::vector<RefJob> table;
+    for (int32_t sl : {4, 8, 16})
+        for (int32_t ph : {1, 0}) {
+            KClass k;
+            k.srcl = sl;
+            k.pinhole = ph;
+            k.first = (int32_t)table.size();
+            for (int32_t i = 0; i < nj; ++i)
+                if (srcl[(size_t)i] == sl && pin[(size_t)i] == ph) {
+                    table.push_back(hj[(size_t)i]);
+                    const int64_t n = (int64_t)hj[(size_t)i].w * hj[(size_t)i].h;
+                    k.cap_sweep += (n + 1) / 2;
+                    k.cap_eval += n;
+                    if (k.count == 0) k.size = size_bucket(n);
+                    ++k.count;
+                }
+            if (k.count) classes.push_back(k);
+        }
+    const size_t tbl0 = c->host_jobs.size();
+    c->host_jobs.insert(c->host_jobs.end(), table.begin(), table.end());
+    HIP_TRY(c, c->jobs_dev.ensure(sizeof(RefJob) * table.size()));
+    HIP_TRY(c, hipMemcpyAsync(c->jobs_dev.p, c->host_jobs.data() + tbl0,
+                              sizeof(RefJob) * table.size(), hipMemcpyHostToDevice, c->st));
+
+    unsigned long long *evals = c->evals.as<unsigned long long>();
+    // initialisation: hypotheses, texture mask, parity lists, then the cost of
+    // every textured pixel's initial hypothesis
+    {
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (c->timing) {
+            ea = take_event(c);
+            eb = take_event(c);
+            HIP_TRY(c, hipEventRecord(ea, c->st));
+        }
+        for (int32_t i = 0; i < nj; ++i) {
+            launch_init(hj[(size_t)i], taps, first_step < 0 ? 1 : 0, c->st);
+            HIP_TRY(c, hipGetLastError());
+        }
+        if (first_step < 0) {
+            for (KClass &k : classes) {
+                k.sch = candidates(taps, k.srcl)[0];
+                k.sch.gpc = 8;
+                PmArgs a = base_args(c, k);
+                a.evals = evals;
+                TRY(pm_launch(c, PM_EVAL_CURRENT, taps, k, a, k.cap_eval));
+            }
+        }
+        if (c->timing) {
+            HIP_TRY(c, hipEventRecord(eb, c->st));
+            c->pending.push_back({ea, eb, 1});
+        }
+    }
+    // half-sweeps (dense.rs:413-467); the schedule of each class is forced,
+    // cached, or tuned per iteration on the class's first job
+    const int32_t s0 = first_step < 0 ? 0 : first_step;
+    for (int32_t s = s0; s < s0 + n_steps; ++s) {
+        if (cancel && *cancel) return fail(c, MVS_ERR_CANCELLED, "cancelled");
+        const int32_t it = s / 2, parity = s % 2;
+        for (KClass &k : classes) {
+            if (c->forced.split > 0) {
+                k.sch = forced_schedule(c, taps, k.srcl);
+            } else {
+                TuneKey key(taps, k.srcl, k.pinhole, k.size, it);
+                auto f = c->tuned.find(key);
+                if (f != c->tuned.end()) {
+                    k.sch = f->second;
+                } else if (c->autotune) {
+                    TRY(autotune_class(c, taps, k, table[(size_t)k.first], it, parity));
+                    c->tuned[key] = k.sch;
+                } else {
+                    k.sch = candidates(taps, k.srcl)[0];
+                }
+            }
+            hipEvent_t ea = nullptr, eb = nullptr;
+            if (c->timing) {
+                ea = take_event(c);
+                eb = take_event(c);
+                HIP_TRY(c, hipEventRecord(ea, c->st));
+            }
+            PmArgs a = base_args(c, k);
+            a.it = it;
+            a.parity = parity;
+            a.range = ldexpf(1.0f, -it);  // 0.5f32.powi(it), dense.rs:414
+            a.evals = evals + 1;
+            TRY(pm_launch(c, PM_SWEEP, taps, k, a, k.cap_sweep));
+            if (c->timing) {
+                HIP_TRY(c, hipEventRecord(eb, c->st));
+                c->pending.push_back({ea, eb, 0});
+            }
+        }
+    }
+    for (const JobSpec &js : jobs) c->stats.pixels += (int64_t)js.ref->w * js.ref->h;
+    // thresholded outputs (dense.rs:468-477)
+    for (int32_t i = 0; i < nj; ++i) {
+        MapDev *m = jobs[(size_t)i].out;
+        if (!m) continue;
+        TRY(ensure_map(c, *m, hj[(size_t)i].w, hj[(size_t)i].h));
+        launch_threshold(hj[(size_t)i], m->depth.as<float>(), m->normal.as<float>(), c->st);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(m->cost.p, hj[(size_t)i].cost,
+                                  sizeof(float) * (size_t)hj[(size_t)i].w * hj[(size_t)i].h,
+                                  hipMemcpyDeviceToDevice, c->st));
+        m->valid = true;
+    }
+    if (!classes.empty()) {
+        c->stats.split = classes[0].sch.split;
+        c->stats.gpc = classes[0].sch.gpc;
+    }
+    if (classes_out) *classes_out = classes;
     return MVS_OK;
 }
 
 int download_map(mvs_ctx *c, const MapDev &m, float *depth, float *normal, float *cost) {
     const size_t n = (size_t)m.w * m.h;
-    if (depth) HIP_TRY(c, hipMemcpyAsync(depth, m.depth.p, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    if (depth)
+        HIP_TRY(c, hipMemcpyAsync(depth, m.depth.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->st));
     if (normal)
-        HIP_TRY(c, hipMemcpyAsync(normal, m.normal.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c->st));
-    if (cost) HIP_TRY(c, hipMemcpyAsync(cost, m.cost.p, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(c, hipMemcpyAsync(normal, m.normal.p, 3 * n * sizeof(float),
+                                  hipMemcpyDeviceToHost, c->st));
+    if (cost)
+        HIP_TRY(c, hipMemcpyAsync(cost, m.cost.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->st));
     return sync_ctx(c);
 }
 
@@ -420,15 +662,27 @@ int upload_host_views(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int
     if (nsrc < 0) return fail(c, MVS_ERR_ARG, "negative source count");
     if (nsrc > 0 && !srcs) return fail(c, MVS_ERR_ARG, "null source views");
     if ((int32_t)c->tmp_views.size() < nsrc + 1) c->tmp_views.resize((size_t)nsrc + 1);
-    int rc = upload_view(c, c->tmp_views[0], ref);
-    if (rc) return rc;
+    TRY(upload_view(c, c->tmp_views[0], ref));
     out.clear();
     for (int32_t i = 0; i < nsrc; ++i) {
-        rc = upload_view(c, c->tmp_views[(size_t)i + 1], &srcs[i]);
-        if (rc) return rc;
+        TRY(upload_view(c, c->tmp_views[(size_t)i + 1], &srcs[i]));
         out.push_back(&c->tmp_views[(size_t)i + 1]);
     }
     return MVS_OK;
+}
+
+int abort_call(mvs_ctx *c, int rc) {
+    (void)hipStreamSynchronize(c->st);
+    collect_timing(c);
+    c->host_jobs.clear();
+    c->host_tables.clear();
+    return rc;
+}
+
+// Work-buffer bytes of one job slot (hyp, cost, mask, lists, sources).
+int64_t slot_bytes(const ViewDev &v) {
+    const int64_t n = (int64_t)v.w * v.h;
+    return n * (16 + 4 + 1 + 4) + 16 * (int64_t)sizeof(SrcCam) + 64;
 }
 
 }  // namespace
@@ -487,11 +741,13 @@ int mvs_create(int32_t device, mvs_ctx **out, char *err, size_t errlen) {
     e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) return report(MVS_ERR_DEVICE, hipGetErrorString(e));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return report(MVS_ERR_DEVICE, std::string("built for gfx950, device is ") + prop.gcnArchName);
+        return report(MVS_ERR_DEVICE,
+                      std::string("built for gfx950, device is ") + prop.gcnArchName);
     e = hipSetDevice(device);
     if (e != hipSuccess) return report(MVS_ERR_DEVICE, hipGetErrorString(e));
     mvs_ctx *c = new mvs_ctx();
     c->device = device;
+    c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -514,10 +770,14 @@ void mvs_destroy(mvs_ctx *c) {
         kv.second->release();
         delete kv.second;
     }
-    DevBuf *bufs[] = {&c->hyp, &c->cost, &c->textured, &c->list, &c->counts, &c->srcs,
-                      &c->evals, &c->ok_count, &c->up_u8, &c->px, &c->py, &c->hyp_in,
-                      &c->cost_out, &c->nbr_cams, &c->agree, &c->tmp_map.depth,
-                      &c->tmp_map.normal, &c->tmp_map.cost};
+    for (auto &wk : c->work) {
+        DevBuf *b[] = {&wk.hyp, &wk.cost, &wk.textured, &wk.list, &wk.counts, &wk.srcs};
+        for (DevBuf *x : b) x->release();
+    }
+    DevBuf *bufs[] = {&c->jobs_dev, &c->prefix, &c->counters, &c->evals, &c->ok_count,
+                      &c->up_u8,    &c->px,     &c->py,       &c->hyp_in, &c->cost_out,
+                      &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost,
+                      &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost};
     for (DevBuf *b : bufs) b->release();
     for (auto &v : c->tmp_views) v.gray.release();
     (void)hipStreamDestroy(c->st);
@@ -531,28 +791,31 @@ int mvs_synchronize(mvs_ctx *c) {
     return sync_ctx(c);
 }
 
+int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t autotune) {
+    if (!c || split < 0 || gpc < 0 || gpc > 1024) return MVS_ERR_ARG;
+    c->forced = Schedule{split, gpc};
+    c->autotune = autotune != 0;
+    return MVS_OK;
+}
+
 int mvs_depth_map(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
                   double range0, double range1, const mvs_params *prm, float *out_depth,
                   float *out_normal, float *out_cost, const volatile int32_t *cancel) {
     if (!c) return MVS_ERR_ARG;
-    int rc = check_params(c, prm);
-    if (rc) return rc;
+    TRY(check_params(c, prm));
     if (nsrc < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
     if (!out_depth) return fail(c, MVS_ERR_ARG, "null depth output");
     HIP_TRY(c, hipSetDevice(c->device));
     std::vector<const ViewDev *> sv;
-    rc = upload_host_views(c, ref, srcs, nsrc, sv);
-    if (rc) return rc;
-    std::vector<SrcCam> host_table;
-    RefJob job;
-    rc = run_map(c, c->tmp_views[0], sv, range0, range1, prm, -1, 2 * prm->iterations, c->srcs,
-                 host_table, cancel, &job);
-    if (rc) {
-        (void)hipStreamSynchronize(c->st);
-        return rc;
-    }
-    rc = finish_map(c, job, c->tmp_map);
-    if (rc) return rc;
+    TRY(upload_host_views(c, ref, srcs, nsrc, sv));
+    std::vector<JobSpec> jobs(1);
+    jobs[0].ref = &c->tmp_views[0];
+    jobs[0].srcs = sv;
+    jobs[0].range0 = range0;
+    jobs[0].range1 = range1;
+    jobs[0].out = &c->tmp_map;
+    int rc = run_batch(c, jobs, prm, -1, 2 * prm->iterations, cancel, nullptr);
+    if (rc) return abort_call(c, rc);
     return download_map(c, c->tmp_map, out_depth, out_normal, out_cost);
 }
 
@@ -560,15 +823,13 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
                  double range0, double range1, const mvs_params *prm, int64_t n,
                  const int32_t *px, const int32_t *py, const float *hyp4, float *out_cost) {
     if (!c) return MVS_ERR_ARG;
-    int rc = check_params(c, prm);
-    if (rc) return rc;
+    TRY(check_params(c, prm));
     if (nsrc < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
-    if (n < 0 || (n > 0 && (!px || !py || !hyp4 || !out_cost)))
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && (!px || !py || !hyp4 || !out_cost)))
         return fail(c, MVS_ERR_ARG, "bad cost list");
     HIP_TRY(c, hipSetDevice(c->device));
     std::vector<const ViewDev *> sv;
-    rc = upload_host_views(c, ref, srcs, nsrc, sv);
-    if (rc) return rc;
+    TRY(upload_host_views(c, ref, srcs, nsrc, sv));
     const ViewDev &rf = c->tmp_views[0];
     for (int64_t i = 0; i < n; ++i)
         if (px[i] < 0 || py[i] < 0 || px[i] >= rf.w || py[i] >= rf.h)
@@ -581,16 +842,28 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     HIP_TRY(c, hipMemcpyAsync(c->px.p, px, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
     HIP_TRY(c, hipMemcpyAsync(c->py.p, py, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
     HIP_TRY(c, hipMemcpyAsync(c->hyp_in.p, hyp4, (size_t)n * 16, hipMemcpyHostToDevice, c->st));
-    // Build the job (rays, sources) without running PatchMatch.
-    std::vector<SrcCam> host_table;
-    RefJob job;
-    rc = run_map(c, rf, sv, range0, range1, prm, 0, 0, c->srcs, host_table, nullptr, &job);
-    if (rc) return rc;
-    if (launch_cost_list(job, prm->taps, n, c->px.as<int32_t>(), c->py.as<int32_t>(),
-                         c->hyp_in.as<float4>(), c->cost_out.as<float>(), c->st))
-        return fail(c, MVS_ERR_ARG, "unsupported window/sources combination");
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->st));
+    // Build the job (rays, sources, mask) without running PatchMatch.
+    std::vector<JobSpec> jobs(1);
+    jobs[0].ref = &rf;
+    jobs[0].srcs = sv;
+    jobs[0].range0 = range0;
+    jobs[0].range1 = range1;
+    std::vector<KClass> classes;
+    int rc = run_batch(c, jobs, prm, 0, 0, nullptr, &classes);
+    if (rc) return abort_call(c, rc);
+    KClass k = classes[0];
+    k.sch = c->forced.split > 0 ? forced_schedule(c, prm->taps, k.srcl)
+                                : candidates(prm->taps, k.srcl)[0];
+    PmArgs a = base_args(c, k);
+    a.px = c->px.as<int32_t>();
+    a.py = c->py.as<int32_t>();
+    a.hyp_in = c->hyp_in.as<float4>();
+    a.out = c->cost_out.as<float>();
+    a.n_items = n;
+    rc = pm_launch(c, PM_COST_LIST, prm->taps, k, a, n);
+    if (rc) return abort_call(c, rc);
+    HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              c->st));
     return sync_ctx(c);
 }
 
@@ -599,48 +872,42 @@ int mvs_patchmatch_steps(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, 
                          int32_t first_step, int32_t n_steps, float *state, uint8_t *textured,
                          int64_t *evals) {
     if (!c) return MVS_ERR_ARG;
-    int rc = check_params(c, prm);
-    if (rc) return rc;
+    TRY(check_params(c, prm));
     if (nsrc < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
-    if (!state || n_steps < 0 || first_step < -1) return fail(c, MVS_ERR_ARG, "bad state or steps");
+    if (!state || n_steps < 0 || first_step < -1)
+        return fail(c, MVS_ERR_ARG, "bad state or steps");
     HIP_TRY(c, hipSetDevice(c->device));
     std::vector<const ViewDev *> sv;
-    rc = upload_host_views(c, ref, srcs, nsrc, sv);
-    if (rc) return rc;
+    TRY(upload_host_views(c, ref, srcs, nsrc, sv));
     const ViewDev &rf = c->tmp_views[0];
     const size_t n = (size_t)rf.w * rf.h;
-    rc = sync_ctx(c);  // flush earlier evals into stats
-    if (rc) return rc;
+    TRY(sync_ctx(c));  // flush earlier evaluations into the stats
     const int64_t evals0 = c->stats.evals;
-    std::vector<float> hyp4, cst;
+    std::vector<float> hyp4(4 * n), cst(n);
+    if (c->work.empty()) c->work.resize(1);
+    JobWork &wk = c->work[0];
     if (first_step >= 0) {
-        HIP_TRY(c, c->hyp.ensure(n * sizeof(float4)));
-        HIP_TRY(c, c->cost.ensure(n * sizeof(float)));
-        hyp4.resize(4 * n);
-        cst.resize(n);
+        HIP_TRY(c, wk.hyp.ensure(n * sizeof(float4)));
+        HIP_TRY(c, wk.cost.ensure(n * sizeof(float)));
         for (size_t i = 0; i < n; ++i) {
             for (int k = 0; k < 4; ++k) hyp4[4 * i + k] = state[5 * i + k];
             cst[i] = state[5 * i + 4];
         }
-        HIP_TRY(c, hipMemcpyAsync(c->hyp.p, hyp4.data(), n * 16, hipMemcpyHostToDevice, c->st));
-        HIP_TRY(c, hipMemcpyAsync(c->cost.p, cst.data(), n * 4, hipMemcpyHostToDevice, c->st));
+        HIP_TRY(c, hipMemcpyAsync(wk.hyp.p, hyp4.data(), n * 16, hipMemcpyHostToDevice, c->st));
+        HIP_TRY(c, hipMemcpyAsync(wk.cost.p, cst.data(), n * 4, hipMemcpyHostToDevice, c->st));
     }
-    std::vector<SrcCam> host_table;
-    RefJob job;
-    rc = run_map(c, rf, sv, range0, range1, prm, first_step, n_steps, c->srcs, host_table,
-                 nullptr, &job);
-    if (rc) {
-        (void)hipStreamSynchronize(c->st);
-        return rc;
-    }
-    hyp4.resize(4 * n);
-    cst.resize(n);
-    HIP_TRY(c, hipMemcpyAsync(hyp4.data(), c->hyp.p, n * 16, hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(c, hipMemcpyAsync(cst.data(), c->cost.p, n * 4, hipMemcpyDeviceToHost, c->st));
+    std::vector<JobSpec> jobs(1);
+    jobs[0].ref = &rf;
+    jobs[0].srcs = sv;
+    jobs[0].range0 = range0;
+    jobs[0].range1 = range1;
+    int rc = run_batch(c, jobs, prm, first_step, n_steps, nullptr, nullptr);
+    if (rc) return abort_call(c, rc);
+    HIP_TRY(c, hipMemcpyAsync(hyp4.data(), wk.hyp.p, n * 16, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(c, hipMemcpyAsync(cst.data(), wk.cost.p, n * 4, hipMemcpyDeviceToHost, c->st));
     if (textured)
-        HIP_TRY(c, hipMemcpyAsync(textured, c->textured.p, n, hipMemcpyDeviceToHost, c->st));
-    rc = sync_ctx(c);
-    if (rc) return rc;
+        HIP_TRY(c, hipMemcpyAsync(textured, wk.textured.p, n, hipMemcpyDeviceToHost, c->st));
+    TRY(sync_ctx(c));
     for (size_t i = 0; i < n; ++i) {
         for (int k = 0; k < 4; ++k) state[5 * i + k] = hyp4[4 * i + k];
         state[5 * i + 4] = cst[i];
@@ -655,7 +922,6 @@ int mvs_survey_create(mvs_ctx *c, int32_t nviews, mvs_survey **out) {
     s->ctx = c;
     s->views.resize((size_t)nviews);
     s->maps.resize((size_t)nviews);
-    s->src_tables.resize((size_t)nviews);
     *out = s;
     return MVS_OK;
 }
@@ -670,54 +936,16 @@ void mvs_survey_destroy(mvs_survey *s) {
         m.normal.release();
         m.cost.release();
     }
-    for (auto &t : s->src_tables) t.release();
     delete s;
 }
 
 int mvs_survey_set_view(mvs_survey *s, int32_t i, const mvs_view *v) {
     if (!s) return MVS_ERR_ARG;
     mvs_ctx *c = s->ctx;
-    if (i < 0 || i >= (int32_t)s->views.size()) return fail(c, MVS_ERR_ARG, "view index out of range");
+    if (i < 0 || i >= (int32_t)s->views.size())
+        return fail(c, MVS_ERR_ARG, "view index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
     return upload_view(c, s->views[(size_t)i], v);
-}
-
-static int survey_one(mvs_survey *s, int32_t ref, const int32_t *srcs, int32_t nsrc, double r0,
-                      double r1, const mvs_params *prm, const volatile int32_t *cancel,
-                      std::vector<SrcCam> &host_table) {
-    mvs_ctx *c = s->ctx;
-    const int32_t nv = (int32_t)s->views.size();
-    if (ref < 0 || ref >= nv || !s->views[(size_t)ref].set)
-        return fail(c, MVS_ERR_ARG, "reference view missing");
-    if (nsrc < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
-    std::vector<const ViewDev *> sv;
-    for (int32_t k = 0; k < nsrc; ++k) {
-        const int32_t j = srcs[k];
-        if (j < 0 || j >= nv || !s->views[(size_t)j].set || j == ref)
-            return fail(c, MVS_ERR_ARG, "bad source view index");
-        sv.push_back(&s->views[(size_t)j]);
-    }
-    RefJob job;
-    int rc = run_map(c, s->views[(size_t)ref], sv, r0, r1, prm, -1, 2 * prm->iterations,
-                     s->src_tables[(size_t)ref], host_table, cancel, &job);
-    if (rc) return rc;
-    return finish_map(c, job, s->maps[(size_t)ref]);
-}
-
-int mvs_survey_depth_map(mvs_survey *s, int32_t ref, const int32_t *srcs, int32_t nsrc,
-                         double range0, double range1, const mvs_params *prm) {
-    if (!s) return MVS_ERR_ARG;
-    mvs_ctx *c = s->ctx;
-    int rc = check_params(c, prm);
-    if (rc) return rc;
-    HIP_TRY(c, hipSetDevice(c->device));
-    std::vector<SrcCam> host_table;
-    rc = survey_one(s, ref, srcs, nsrc, range0, range1, prm, nullptr, host_table);
-    if (rc) {
-        (void)hipStreamSynchronize(c->st);
-        return rc;
-    }
-    return sync_ctx(c);
 }
 
 int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
@@ -726,34 +954,69 @@ int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
                           mvs_progress_fn progress, void *user) {
     if (!s) return MVS_ERR_ARG;
     mvs_ctx *c = s->ctx;
-    int rc = check_params(c, prm);
-    if (rc) return rc;
+    TRY(check_params(c, prm));
     if (njobs < 0 || (njobs > 0 && (!refs || !src_off || !ranges)))
         return fail(c, MVS_ERR_ARG, "bad job arrays");
     HIP_TRY(c, hipSetDevice(c->device));
-    std::vector<std::vector<SrcCam>> tables((size_t)njobs);
+    const int32_t nv = (int32_t)s->views.size();
+    // validate; views without neighbours get no map (the reference's Ok(None))
+    std::vector<JobSpec> all;
     for (int32_t j = 0; j < njobs; ++j) {
-        if (cancel && *cancel) {
-            (void)hipStreamSynchronize(c->st);
-            return fail(c, MVS_ERR_CANCELLED, "cancelled");
+        const int32_t ref = refs[j];
+        if (ref < 0 || ref >= nv || !s->views[(size_t)ref].set)
+            return fail(c, MVS_ERR_ARG, "reference view missing");
+        JobSpec js;
+        js.ref = &s->views[(size_t)ref];
+        for (int32_t k = src_off[j]; k < src_off[j + 1]; ++k) {
+            const int32_t q = srcs[k];
+            if (q < 0 || q >= nv || !s->views[(size_t)q].set || q == ref)
+                return fail(c, MVS_ERR_ARG, "bad source view index");
+            js.srcs.push_back(&s->views[(size_t)q]);
         }
-        const int32_t a = src_off[j], b = src_off[j + 1];
-        rc = survey_one(s, refs[j], srcs + a, b - a, ranges[2 * j], ranges[2 * j + 1], prm, cancel,
-                        tables[(size_t)j]);
-        if (rc == MVS_NO_NEIGHBOURS) {
-            s->maps[(size_t)refs[j]].valid = false;
-            rc = MVS_OK;
+        js.range0 = ranges[2 * j];
+        js.range1 = ranges[2 * j + 1];
+        js.out = &s->maps[(size_t)ref];
+        if (js.srcs.empty()) {
+            s->maps[(size_t)ref].valid = false;
+            continue;
         }
-        if (rc) {
-            (void)hipStreamSynchronize(c->st);
-            return rc;
+        all.push_back(js);
+    }
+    // batches bounded by a work-buffer budget (a quarter of free memory)
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = std::max<int64_t>((int64_t)(free_b / 4), 1ll << 30);
+    size_t i = 0;
+    int32_t done = 0;
+    while (i < all.size()) {
+        std::vector<JobSpec> batch;
+        int64_t bytes = 0;
+        while (i < all.size() && (batch.empty() || bytes + slot_bytes(*all[i].ref) <= budget)) {
+            bytes += slot_bytes(*all[i].ref);
+            batch.push_back(all[i++]);
         }
-        if (progress && progress(user, "depth", j + 1, njobs)) {
-            (void)hipStreamSynchronize(c->st);
-            return fail(c, MVS_ERR_CANCELLED, "cancelled");
+        int rc = run_batch(c, batch, prm, -1, 2 * prm->iterations, cancel, nullptr);
+        if (rc) return abort_call(c, rc);
+        if (progress) {
+            TRY(sync_ctx(c));
+            // Ctx::report("depth", k, n) per finished view (dense.rs:539)
+            for (size_t b = 0; b < batch.size(); ++b)
+                if (progress(user, "depth", ++done, (int32_t)all.size()))
+                    return fail(c, MVS_ERR_CANCELLED, "cancelled");
+        } else {
+            done += (int32_t)batch.size();
         }
     }
     return sync_ctx(c);
+}
+
+int mvs_survey_depth_map(mvs_survey *s, int32_t ref, const int32_t *srcs, int32_t nsrc,
+                         double range0, double range1, const mvs_params *prm) {
+    if (!s) return MVS_ERR_ARG;
+    if (nsrc < 1) return fail(s->ctx, MVS_NO_NEIGHBOURS, "no neighbours");
+    const int32_t off[2] = {0, nsrc};
+    const double rng[2] = {range0, range1};
+    return mvs_survey_depth_maps(s, 1, &ref, off, srcs, rng, prm, nullptr, nullptr, nullptr);
 }
 
 int mvs_survey_download(mvs_survey *s, int32_t view, float *depth, float *normal, float *cost) {
@@ -819,10 +1082,10 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
     HIP_TRY(c, hipGetLastError());
     unsigned long long ok = 0;
     if (agree)
-        HIP_TRY(c, hipMemcpyAsync(agree, c->agree.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(c, hipMemcpyAsync(agree, c->agree.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  c->st));
     HIP_TRY(c, hipMemcpyAsync(&ok, c->ok_count.p, sizeof ok, hipMemcpyDeviceToHost, c->st));
-    int rc = sync_ctx(c);
-    if (rc) return rc;
+    TRY(sync_ctx(c));
     if (n_consistent) *n_consistent = (int64_t)ok;
     return MVS_OK;
 }
@@ -842,8 +1105,7 @@ int mvs_selftest(mvs_ctx *c, int32_t which, int64_t n, uint64_t seed, int64_t *m
     }
     unsigned long long bad = 0;
     HIP_TRY(c, hipMemcpyAsync(&bad, c->ok_count.p, sizeof bad, hipMemcpyDeviceToHost, c->st));
-    int rc = sync_ctx(c);
-    if (rc) return rc;
+    TRY(sync_ctx(c));
     *mismatches = (int64_t)bad;
     return MVS_OK;
 }
@@ -856,16 +1118,14 @@ int mvs_set_timing(mvs_ctx *c, int32_t enable) {
 
 int mvs_get_stats(mvs_ctx *c, mvs_stats *out) {
     if (!c || !out) return MVS_ERR_ARG;
-    int rc = sync_ctx(c);
-    if (rc) return rc;
+    TRY(sync_ctx(c));
     *out = c->stats;
     return MVS_OK;
 }
 
 int mvs_reset_stats(mvs_ctx *c) {
     if (!c) return MVS_ERR_ARG;
-    int rc = sync_ctx(c);
-    if (rc) return rc;
+    TRY(sync_ctx(c));
     memset(&c->stats, 0, sizeof c->stats);
     return MVS_OK;
 }

@@ -43,6 +43,22 @@ struct RefJob {
     int32_t pinhole;      // every source has pinhole_exact set
 };
 
+// Arguments of one batched PatchMatch launch (k_pm in mvs_pm_impl.h).
+struct PmArgs {
+    const RefJob *jobs;   // device array of njobs jobs of one kernel class
+    int32_t njobs;
+    int32_t *prefix;      // 9 (njobs + 1) chunk and band offsets, by the prep kernel
+    int32_t *counters;    // 8 per-XCD work counters, zeroed by the prep kernel
+    int32_t it, parity;   // iteration and colour of a half-sweep
+    float range;          // 0.5^it
+    int32_t gpc;          // pixel groups per work chunk
+    unsigned long long *evals;
+    const int32_t *px, *py;  // explicit (pixel, hypothesis) list
+    const float4 *hyp_in;
+    float *out;
+    int64_t n_items;
+};
+
 // Kernel-side neighbour description for the consistency filter (f64, as the
 // reference's fusion, dense.rs:586-612).
 struct NbrCam {
@@ -65,20 +81,29 @@ void launch_gray_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t strid
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                  float2 *rays, hipStream_t st);
 // write_hyp = 0 keeps a caller-provided state and only rebuilds mask + lists.
+// Also zeroes the job's list counters first.
 void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t st);
-// Evaluate the current hypothesis of every textured pixel (both lists).
-int launch_eval_current(const RefJob &job, int32_t taps, unsigned long long *evals,
-                        hipStream_t st);
-int launch_sweep(const RefJob &job, int32_t taps, int32_t it, int32_t parity,
-                 unsigned long long *evals, hipStream_t st);
-int launch_cost_list(const RefJob &job, int32_t taps, int64_t n, const int32_t *px,
-                     const int32_t *py, const float4 *hyp, float *out, hipStream_t st);
+// One batched PatchMatch launch (prep + persistent kernel). mode: 0 evaluate
+// the current hypotheses, 1 half-sweep, 2 explicit cost list. srcl: source
+// lanes per pixel (4, 8, 16); split: lanes per source. Returns -1 for a class
+// that is not instantiated.
+enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
+int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole,
+              const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
+                 int32_t grid, hipStream_t st);
+int launch_pm_t7(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
+                 int32_t grid, hipStream_t st);
+int launch_pm_t11(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
+                  int32_t grid, hipStream_t st);
+// Tap splits instantiated for a window size (ascending), count returned.
+int splits_for(int32_t taps, int32_t *out);
 void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_t st);
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                         const NbrCam *nbrs, int32_t nn, int32_t need, int32_t *agree,
                         unsigned long long *n_ok, hipStream_t st);
 void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st);
-// Lanes per reference pixel for nsrc sources (one lane per source).
+// Source lanes per reference pixel for nsrc sources (one lane per source).
 int lanes_for(int32_t nsrc);
 bool taps_supported(int32_t taps);
 

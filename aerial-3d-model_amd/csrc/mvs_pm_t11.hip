@@ -1,0 +1,11 @@
+// PatchMatch kernels for the 11x11 window (see mvs_pm_impl.h).
+#include "mvs_pm_impl.h"
+
+namespace amvs {
+
+int launch_pm_t11(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
+                 int32_t grid, hipStream_t st) {
+    return PmDispatch<11, 2, 4>::run(mode, srcl, split, pinhole, a, grid, st);
+}
+
+}  // namespace amvs

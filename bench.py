@@ -4,9 +4,10 @@ Workload (the survey the metric is quoted on): 77 nadir views at 2000x1500 u8
 grey, seeded synthetic heightfield survey; each view is a reference view with
 its 8 nearest-GPS neighbours, 7x7 NCC window, 6 PatchMatch iterations, then the
 geometric-consistency filter. A "step" is one reference view's depth map
-(initialisation + 12 half-sweeps + threshold); the timed region covers K steps
-enqueued back to back plus the consistency filter of those K views, with all
-images already resident in HBM.
+(initialisation + 12 half-sweeps + threshold). The timed region runs K steps
+as one batch - every half-sweep of all K views is one persistent launch - plus
+the consistency filter of those K views, with all images already resident in
+HBM. The plain path (one view per call) is timed on a subset beside it.
 
 Prints one JSON line (rank 0 of one GPU).
 """
@@ -88,6 +89,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", default="640x480")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--plain-views", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this depth engine runs on a single MI355X "
@@ -122,6 +124,22 @@ def main():
     timed = jobs[:steps]
     done = set(j[0] for j in timed)
 
+    # plain path: one reference view per call
+    plain_n = max(0, min(args.plain_views, len(jobs)))
+    plain = None
+    if plain_n:
+        ctx.synchronize()
+        tp = time.perf_counter()
+        for (i, nb, rng) in jobs[:plain_n]:
+            sv.depth_map(i, nb, rng[0], rng[1], prm)
+        ctx.synchronize()
+        dtp = time.perf_counter() - tp
+        plain = dict(value=sum(surv.views[j[0]].w * surv.views[j[0]].h
+                               for j in jobs[:plain_n]) / dtp / 1e6,
+                     unit="depth Mpix/s", views=plain_n)
+        log(f"plain GPU path: {plain['value']:.2f} depth Mpix/s over {plain_n} views")
+    ctx.reset_stats()
+
     ctx.synchronize()
     t0 = time.perf_counter()
     sv.depth_maps(timed, prm)
@@ -146,7 +164,7 @@ def main():
     achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
     roofline = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
-                    kernel="k_patchmatch<7,8,pinhole,sweep>", avg_launch_ms=avg_ms,
+                    kernel=f"k_pm<7,8,{st['split']},pinhole,sweep>", avg_launch_ms=avg_ms,
                     evals_per_launch=evals_per_launch, flops_per_eval=fpe,
                     launches=st["sweep_launches"])
 
@@ -176,8 +194,10 @@ def main():
                               "geometric-consistency filter"),
                     views=len(surv.views), width=spec.width, height=spec.height,
                     neighbours=spec.neighbours, window=f"{spec.taps}x{spec.taps}/stride{spec.stride}",
-                    iterations=spec.iterations, timed_views=steps, path="plain GPU"),
-        roofline=roofline, cpu_baseline=cpu, depth_error=err,
+                    iterations=spec.iterations, timed_views=steps,
+                    path="batched GPU (one persistent launch per half-sweep for all views)",
+                    schedule=dict(split=st["split"], gpc=st["gpc"])),
+        roofline=roofline, cpu_baseline=cpu, depth_error=err, plain_gpu=plain,
         gpu_vs_cpu=(value / cpu["value"]) if cpu else None,
         evals_per_pixel=st["evals"] / max(1, pixels),
     )

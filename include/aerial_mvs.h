@@ -174,9 +174,17 @@ typedef struct {
     int64_t pixels;           /* reference pixels processed */
     int64_t init_evals;       /* of which in the initialisation */
     int64_t sweep_evals;      /* of which in the half-sweeps */
+    int32_t split;            /* schedule of the last batch: lanes per source */
+    int32_t gpc;              /* and pixel groups per work chunk */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
+
+/* Kernel schedule (no reference counterpart; results do not depend on it):
+ * split = lanes sharing one source's window (0 = choose automatically), gpc =
+ * pixel groups per work chunk (0 = default). autotune = 1 times the candidate
+ * schedules once per kernel class on the first batch and keeps the fastest. */
+int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t autotune);
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
 int mvs_reset_stats(mvs_ctx *ctx);
 

@@ -240,3 +240,62 @@ def test_shared_reciprocal_division_is_ieee(gpu_ctx):
     bad = gpu_ctx.selftest_division(1 << 30, seed=12345)
     _say(f"division mismatches: {bad} of {2 << 30}")
     assert bad == 0
+
+
+@pytest.mark.parametrize("taps,stride,k", [(7, 1, 4), (7, 1, 8), (11, 1, 4), (11, 1, 8)])
+def test_ncc_cost_parity_every_schedule(gpu_ctx, small_survey, taps, stride, k):
+    """Each tap split (lanes per source) and chunk size gives the oracle's
+    costs bit for bit: the slice-to-slice sums keep the reference's order."""
+    s = small_survey
+    ref = s.reference_index()
+    nb = s.neighbours[ref][:k]
+    rng = np.random.default_rng(200 + taps + k)
+    xs, ys, hy = _random_hyps(s, ref, 3000, rng)
+    prm = capi.params(taps=taps, stride=stride)
+    want = orc.OracleDepth(s.views[ref], [s.views[j] for j in nb], 40, 80, taps=taps,
+                           stride=stride).cost(xs, ys, hy)
+    splits = {5: [1], 7: [1, 2], 11: [2, 4]}[taps]
+    try:
+        for split in splits:
+            for gpc in (1, 8):
+                _say(f"case ncc_cost schedule taps={taps} K={k} split={split} gpc={gpc}")
+                gpu_ctx.set_schedule(split=split, gpc=gpc, autotune=False)
+                got = gpu_ctx.ncc_cost(s.views[ref], [s.views[j] for j in nb], 40, 80, prm,
+                                       xs, ys, hy)
+                assert _compare_costs(got, want, f"{taps}x{stride}/K{k}/split{split}") == 1.0
+    finally:
+        gpu_ctx.set_schedule(0, 0, True)
+
+
+def test_depth_map_identical_across_schedules(gpu_ctx, small_survey):
+    s = small_survey
+    ref = s.reference_index()
+    rv, sv = s.views[ref], [s.views[j] for j in s.neighbours[ref][:8]]
+    prm = capi.params(taps=7, stride=1, iterations=3)
+    maps = []
+    try:
+        for split, gpc in ((1, 4), (2, 16), (2, 1)):
+            _say(f"case depth map schedule split={split} gpc={gpc}")
+            gpu_ctx.set_schedule(split=split, gpc=gpc, autotune=False)
+            maps.append(gpu_ctx.depth_map(rv, sv, 40, 80, prm))
+    finally:
+        gpu_ctx.set_schedule(0, 0, True)
+    for m in maps[1:]:
+        for a, b in zip(maps[0], m):
+            assert np.array_equal(a, b)
+    od, _, _, _ = orc.OracleDepth(rv, sv, 40, 80, taps=7, stride=1).depth_map(iterations=3)
+    assert np.array_equal(maps[0][0], od)
+
+
+def test_full_depth_map_11x11(gpu_ctx, small_survey):
+    s = small_survey
+    ref = s.reference_index()
+    rv, sv = s.views[ref], [s.views[j] for j in s.neighbours[ref][:4]]
+    prm = capi.params(taps=11, stride=1, iterations=2)
+    _say("case full depth map 11x11")
+    gd, _, _ = gpu_ctx.depth_map(rv, sv, 40, 80, prm)
+    od, _, _, _ = orc.OracleDepth(rv, sv, 40, 80, taps=11, stride=1).depth_map(iterations=2)
+    both = (gd > 0) & (od > 0)
+    within = float(np.mean(np.abs(gd[both] - od[both]) / od[both] <= 0.005))
+    _say(f"11x11: within0.5%={within:.6f} identical={np.mean(gd == od):.6f}")
+    assert within >= 0.99
