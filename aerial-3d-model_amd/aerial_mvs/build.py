@@ -44,7 +44,8 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-UNITS = ["mvs_api.hip", "mvs_kernels.hip", "mvs_pm_t5.hip", "mvs_pm_t7.hip", "mvs_pm_t11.hip"]
+UNITS = ["mvs_api.hip", "mvs_kernels.hip", "mvs_pm_t5.hip", "mvs_pm_t7.hip", "mvs_pm_t11.hip",
+         "mvs_refine.hip"]
 
 
 def build_mvs(extra=None):

@@ -18,6 +18,7 @@ MVS_NO_NEIGHBOURS = 1
 MVS_GRAY_U8 = 0
 MVS_GRAY_F32 = 1
 MVS_CONS_REFERENCE = 0
+MVS_CONS_REPROJ = 1
 
 EXPORTS = [
     "mvs_default_params", "mvs_dense_params_for_profile", "mvs_select_neighbours",
@@ -26,7 +27,7 @@ EXPORTS = [
     "mvs_survey_set_view", "mvs_survey_depth_map", "mvs_survey_depth_maps",
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
-    "mvs_set_schedule",
+    "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair",
 ]
 
 
@@ -41,7 +42,7 @@ class MvsView(C.Structure):
 class MvsParams(C.Structure):
     _fields_ = [("taps", C.c_int32), ("stride", C.c_int32), ("iterations", C.c_int32),
                 ("texture_min", C.c_float), ("cost_max", C.c_float),
-                ("reserved", C.c_int32 * 3)]
+                ("refine_steps", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class MvsStats(C.Structure):
@@ -49,7 +50,9 @@ class MvsStats(C.Structure):
                 ("init_launches", C.c_int64), ("init_ms", C.c_double),
                 ("evals", C.c_int64), ("pixels", C.c_int64),
                 ("init_evals", C.c_int64), ("sweep_evals", C.c_int64),
-                ("split", C.c_int32), ("gpc", C.c_int32)]
+                ("split", C.c_int32), ("gpc", C.c_int32),
+                ("refine_launches", C.c_int64), ("refine_ms", C.c_double),
+                ("refine_pixels", C.c_int64), ("refine_accepted", C.c_int64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)
@@ -87,6 +90,8 @@ def lib():
                                     vp, vp, vp, vp]
         L.mvs_ncc_cost.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
                                    vp, vp, vp, vp]
+        L.mvs_ncc_pair.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
+                                   vp, vp, vp, vp, vp, vp, vp]
         L.mvs_patchmatch_steps.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64,
                                            P(MvsParams), i32, i32, vp, vp, P(C.c_int64)]
         L.mvs_survey_create.argtypes = [vp, i32, P(vp)]
@@ -101,7 +106,9 @@ def lib():
         L.mvs_set_timing.argtypes = [vp, i32]
         L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
         L.mvs_reset_stats.argtypes = [vp]
-        L.mvs_set_schedule.argtypes = [vp, i32, i32, i32]
+        L.mvs_set_schedule.argtypes = [vp, i32, i32, i32, i32]
+        L.mvs_tuned_schedules.argtypes = [vp, vp, i32]
+        L.mvs_tuned_schedules.restype = i32
         L.mvs_selftest.argtypes = [vp, i32, i64, C.c_uint64, P(C.c_int64)]
         L.mvs_save_depth.argtypes = [C.c_char_p, i32, i32, vp]
         L.mvs_load_depth.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, i64]
@@ -109,10 +116,11 @@ def lib():
     return _lib
 
 
-def params(taps=5, stride=2, iterations=4, texture_min=0.008, cost_max=0.45):
+def params(taps=5, stride=2, iterations=4, texture_min=0.008, cost_max=0.45, refine_steps=0):
     p = MvsParams()
     lib().mvs_default_params(C.byref(p))
     p.taps, p.stride, p.iterations = taps, stride, iterations
+    p.refine_steps = refine_steps
     p.texture_min, p.cost_max = texture_min, cost_max
     return p
 
@@ -249,6 +257,28 @@ class Context:
                                       hy.ctypes.data, out.ctypes.data))
         return out
 
+    def ncc_pair(self, ref, srcs, range0, range1, prm, xs, ys, hyps, pairs=None, grad_out=None,
+                 backward=True):
+        """Fixed-pair cost 1 - (NCC_s0 + NCC_s1)/2 of (pixel, plane) items and,
+        with backward, its gradient w.r.t. (d, n0, n1, n2) times grad_out.
+        pairs: (n, 2) source indices; None or rows of -1 select the best pair.
+        Returns (cost, pairs, grad or None)."""
+        r, arr, keep = self._views(ref, srcs)
+        xs = np.ascontiguousarray(xs, np.int32)
+        ys = np.ascontiguousarray(ys, np.int32)
+        hy = np.ascontiguousarray(hyps, np.float32).reshape(-1, 4)
+        n = len(xs)
+        pr = (np.full((n, 2), -1, np.int32) if pairs is None
+              else np.array(pairs, np.int32).reshape(n, 2).copy())
+        go = None if grad_out is None else np.ascontiguousarray(grad_out, np.float32)
+        out = np.zeros(n, np.float32)
+        grad = np.zeros((n, 4), np.float32) if backward else None
+        self.check(lib().mvs_ncc_pair(self._h, C.byref(r), arr, len(srcs), range0, range1,
+                                      C.byref(prm), n, xs.ctypes.data, ys.ctypes.data,
+                                      hy.ctypes.data, pr.ctypes.data, _ptr(go), out.ctypes.data,
+                                      _ptr(grad)))
+        return out, pr, grad
+
     def patchmatch_steps(self, ref, srcs, range0, range1, prm, first_step, n_steps, state=None):
         """Returns (state (h,w,5), textured, evals). first_step = -1 initialises."""
         r, arr, keep = self._views(ref, srcs)
@@ -273,8 +303,15 @@ class Context:
     def reset_stats(self):
         self.check(lib().mvs_reset_stats(self._h))
 
-    def set_schedule(self, split=0, gpc=0, autotune=True):
-        self.check(lib().mvs_set_schedule(self._h, split, gpc, 1 if autotune else 0))
+    def set_schedule(self, split=0, gpc=0, per_cu=16, autotune=True):
+        self.check(lib().mvs_set_schedule(self._h, split, gpc, per_cu, 1 if autotune else 0))
+
+    def tuned_schedules(self):
+        n = lib().mvs_tuned_schedules(self._h, None, 0)
+        out = np.zeros((max(n, 1), 7), np.int32)
+        lib().mvs_tuned_schedules(self._h, out.ctypes.data, n)
+        keys = ("taps", "srcl", "pinhole", "log2px", "iteration", "split", "gpc")
+        return [dict(zip(keys, map(int, r))) for r in out[:n]]
 
     def selftest_division(self, n, seed=1):
         bad = C.c_int64()

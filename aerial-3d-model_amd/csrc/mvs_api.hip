@@ -83,11 +83,12 @@ typedef std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t> TuneKey;
 struct Schedule {
     int32_t split = 1;
     int32_t gpc = 8;
+    int32_t per_cu = 16;  // workgroups per CU in the grid; 0: one per chunk
 };
 
 struct TimedLaunch {
     hipEvent_t a, b;
-    int kind;  // 0 sweep, 1 init+eval
+    int kind;  // 0 sweep, 1 init+eval, 2 refinement
 };
 
 }  // namespace
@@ -103,12 +104,13 @@ struct mvs_ctx {
     std::vector<ViewDev> tmp_views;  // host-view calls
     MapDev tmp_map;
     DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
+    DevBuf pairs, grad_io, gout;  // pair-cost lists
     std::map<RayKey, DevBuf *> rays;
     // host copies that must outlive the asynchronous uploads of a call
     std::vector<RefJob> host_jobs;
     std::vector<std::vector<SrcCam>> host_tables;
     // schedule: forced (split > 0) or tuned per class
-    Schedule forced{0, 0};
+    Schedule forced{0, 0, 0};
     bool autotune = true;
     std::map<TuneKey, Schedule> tuned;
     // timing
@@ -167,6 +169,9 @@ void collect_timing(mvs_ctx *c) {
             if (t.kind == 0) {
                 c->stats.sweep_ms += ms;
                 c->stats.sweep_launches += 1;
+            } else if (t.kind == 2) {
+                c->stats.refine_ms += ms;
+                c->stats.refine_launches += 1;
             } else {
                 c->stats.init_ms += ms;
                 c->stats.init_launches += 1;
@@ -181,7 +186,7 @@ void collect_timing(mvs_ctx *c) {
 int sync_ctx(mvs_ctx *c) {
     HIP_TRY(c, hipStreamSynchronize(c->st));
     collect_timing(c);
-    unsigned long long ev[2] = {0, 0};
+    unsigned long long ev[4] = {0, 0, 0, 0};
     if (c->evals.p) {
         HIP_TRY(c, hipMemcpy(ev, c->evals.p, sizeof ev, hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemset(c->evals.p, 0, sizeof ev));
@@ -189,6 +194,8 @@ int sync_ctx(mvs_ctx *c) {
     c->stats.init_evals += (int64_t)ev[0];
     c->stats.sweep_evals += (int64_t)ev[1];
     c->stats.evals += (int64_t)(ev[0] + ev[1]);
+    c->stats.refine_accepted += (int64_t)ev[2];
+    c->stats.refine_pixels += (int64_t)ev[3];
     c->host_jobs.clear();
     c->host_tables.clear();
     return MVS_OK;
@@ -214,6 +221,8 @@ int check_params(mvs_ctx *c, const mvs_params *p) {
     if (p->stride < 1 || p->stride > 8) return fail(c, MVS_ERR_ARG, "tap stride must be 1..8");
     if (p->iterations < 0 || p->iterations > 64)
         return fail(c, MVS_ERR_ARG, "iterations out of range");
+    if (p->refine_steps < 0 || p->refine_steps > 64)
+        return fail(c, MVS_ERR_ARG, "refine steps out of range (0..64)");
     return MVS_OK;
 }
 
@@ -328,6 +337,7 @@ int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
 struct KClass {
     int32_t srcl = 0, pinhole = 1, first = 0, count = 0;
     int64_t cap_sweep = 0, cap_eval = 0;  // upper bounds of the list lengths
+    std::vector<int64_t> caps;            // pixels of each job
     int32_t size = 0;                     // log2 pixels of the first job
     Schedule sch;
 };
@@ -350,17 +360,35 @@ PmArgs base_args(mvs_ctx *c, const KClass &k) {
     return a;
 }
 
-int32_t grid_for(mvs_ctx *c, int64_t items, int32_t srcl, const Schedule &s) {
-    const int64_t pix = 64 / (srcl * s.split);
-    const int64_t chunks = (items + pix * s.gpc - 1) / (pix * s.gpc);
-    const int64_t resident = (int64_t)c->num_cu * 16;
-    return (int32_t)std::max<int64_t>(1, std::min(chunks, resident));
+// Grid of a launch. Persistent: resident workgroups, at most one per chunk.
+// Static: one workgroup per position of the longest possible XCD sequence
+// (each job's chunks split into 8 bands), times 8. items_of(j) bounds the
+// items of job j in this launch.
+template <class F>
+int32_t grid_for(mvs_ctx *c, const KClass &k, bool persist, F items_of) {
+    const int64_t pix = 64 / (k.srcl * k.sch.split), chunk = pix * k.sch.gpc;
+    int64_t chunks = 0, seq = 0;
+    for (int32_t j = 0; j < k.count; ++j) {
+        const int64_t n = (items_of(j) + chunk - 1) / chunk;
+        chunks += n;
+        seq += (n + 7) / 8;
+    }
+    const int64_t g = persist ? std::min<int64_t>(chunks, (int64_t)c->num_cu * k.sch.per_cu)
+                              : 8 * seq;
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(g, (1ll << 31) - 1));
 }
 
 int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs &a,
-              int64_t items) {
-    const int32_t grid = grid_for(c, items, k.srcl, k.sch);
-    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, a, grid, c->st))
+              int64_t n_items = 0) {
+    const bool persist = mode == PM_SWEEP && k.sch.per_cu > 0;
+    int32_t grid;
+    if (mode == PM_COST_LIST)
+        grid = grid_for(c, k, false, [&](int32_t) { return n_items; });
+    else if (mode == PM_SWEEP)
+        grid = grid_for(c, k, persist, [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
+    else
+        grid = grid_for(c, k, persist, [&](int32_t j) { return k.caps[(size_t)j]; });
+    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, persist, a, grid, c->st))
         return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
     HIP_TRY(c, hipGetLastError());
     return MVS_OK;
@@ -373,7 +401,9 @@ std::vector<Schedule> candidates(int32_t taps, int32_t srcl) {
     std::vector<Schedule> cand;
     for (int i = 0; i < ns; ++i) {
         if (srcl * sp[i] > 64) continue;
-        for (int32_t gpc : {4, 16}) cand.push_back(Schedule{sp[i], gpc});
+        cand.push_back(Schedule{sp[i], 1, 0});   // static grid
+        cand.push_back(Schedule{sp[i], 1, 16});  // persistent
+        cand.push_back(Schedule{sp[i], 4, 16});
     }
     return cand;
 }
@@ -403,7 +433,7 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
     Schedule best = cand[0];
     KClass one = k;
     one.count = 1;
-    one.cap_sweep = ((int64_t)j0.w * j0.h + 1) / 2;
+    one.caps.assign(1, (int64_t)j0.w * j0.h);
     for (const Schedule &s : cand) {
         one.sch = s;
         PmArgs a = base_args(c, one);
@@ -415,7 +445,7 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
         HIP_TRY(c, hipMemcpyAsync(j0.cost, c->save_cost.p, n * sizeof(float),
                                   hipMemcpyDeviceToDevice, c->st));
         HIP_TRY(c, hipEventRecord(ea, c->st));
-        TRY(pm_launch(c, PM_SWEEP, taps, one, a, one.cap_sweep));
+        TRY(pm_launch(c, PM_SWEEP, taps, one, a));
         HIP_TRY(c, hipEventRecord(eb, c->st));
         HIP_TRY(c, hipEventSynchronize(eb));
         float ms = 0.0f;
@@ -465,8 +495,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
     }
     if ((int32_t)c->work.size() < nj) c->work.resize((size_t)nj);
     if (!c->evals.p) {
-        HIP_TRY(c, c->evals.ensure(2 * sizeof(unsigned long long)));
-        HIP_TRY(c, hipMemset(c->evals.p, 0, 2 * sizeof(unsigned long long)));
+        HIP_TRY(c, c->evals.ensure(4 * sizeof(unsigned long long)));
+        HIP_TRY(c, hipMemset(c->evals.p, 0, 4 * sizeof(unsigned long long)));
     }
     HIP_TRY(c, c->prefix.ensure(sizeof(int32_t) * 9 * ((size_t)nj + 1)));
     HIP_TRY(c, c->counters.ensure(sizeof(int32_t) * 8));
@@ -544,6 +574,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
                     const int64_t n = (int64_t)hj[(size_t)i].w * hj[(size_t)i].h;
                     k.cap_sweep += (n + 1) / 2;
                     k.cap_eval += n;
+                    k.caps.push_back(n);
                     if (k.count == 0) k.size = size_bucket(n);
                     ++k.count;
                 }
@@ -572,10 +603,9 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         if (first_step < 0) {
             for (KClass &k : classes) {
                 k.sch = candidates(taps, k.srcl)[0];
-                k.sch.gpc = 8;
                 PmArgs a = base_args(c, k);
                 a.evals = evals;
-                TRY(pm_launch(c, PM_EVAL_CURRENT, taps, k, a, k.cap_eval));
+                TRY(pm_launch(c, PM_EVAL_CURRENT, taps, k, a));
             }
         }
         if (c->timing) {
@@ -615,7 +645,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
             a.parity = parity;
             a.range = ldexpf(1.0f, -it);  // 0.5f32.powi(it), dense.rs:414
             a.evals = evals + 1;
-            TRY(pm_launch(c, PM_SWEEP, taps, k, a, k.cap_sweep));
+            TRY(pm_launch(c, PM_SWEEP, taps, k, a));
             if (c->timing) {
                 HIP_TRY(c, hipEventRecord(eb, c->st));
                 c->pending.push_back({ea, eb, 0});
@@ -623,6 +653,26 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         }
     }
     for (const JobSpec &js : jobs) c->stats.pixels += (int64_t)js.ref->w * js.ref->h;
+    // Gauss-Newton refinement of the accepted pixels (no reference
+    // counterpart), before the outputs are thresholded
+    if (prm->refine_steps > 0 && first_step < 0) {
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (c->timing) {
+            ea = take_event(c);
+            eb = take_event(c);
+            HIP_TRY(c, hipEventRecord(ea, c->st));
+        }
+        for (int32_t i = 0; i < nj; ++i) {
+            if (!jobs[(size_t)i].out) continue;
+            if (launch_refine(taps, hj[(size_t)i], prm->refine_steps, evals + 2, c->st))
+                return fail(c, MVS_ERR_ARG, "no refinement kernel for this window");
+            HIP_TRY(c, hipGetLastError());
+        }
+        if (c->timing) {
+            HIP_TRY(c, hipEventRecord(eb, c->st));
+            c->pending.push_back({ea, eb, 2});
+        }
+    }
     // thresholded outputs (dense.rs:468-477)
     for (int32_t i = 0; i < nj; ++i) {
         MapDev *m = jobs[(size_t)i].out;
@@ -777,6 +827,7 @@ void mvs_destroy(mvs_ctx *c) {
     DevBuf *bufs[] = {&c->jobs_dev, &c->prefix, &c->counters, &c->evals, &c->ok_count,
                       &c->up_u8,    &c->px,     &c->py,       &c->hyp_in, &c->cost_out,
                       &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost,
+                      &c->pairs,    &c->grad_io, &c->gout,
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost};
     for (DevBuf *b : bufs) b->release();
     for (auto &v : c->tmp_views) v.gray.release();
@@ -791,11 +842,32 @@ int mvs_synchronize(mvs_ctx *c) {
     return sync_ctx(c);
 }
 
-int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t autotune) {
-    if (!c || split < 0 || gpc < 0 || gpc > 1024) return MVS_ERR_ARG;
-    c->forced = Schedule{split, gpc};
+int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
+                     int32_t autotune) {
+    if (!c || split < 0 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
+        return MVS_ERR_ARG;
+    c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;
     return MVS_OK;
+}
+
+int32_t mvs_tuned_schedules(mvs_ctx *c, int32_t *out, int32_t cap) {
+    if (!c) return 0;
+    int32_t k = 0;
+    for (const auto &kv : c->tuned) {
+        if (out && k < cap) {
+            int32_t *r = out + 7 * k;
+            r[0] = std::get<0>(kv.first);
+            r[1] = std::get<1>(kv.first);
+            r[2] = std::get<2>(kv.first);
+            r[3] = std::get<3>(kv.first);
+            r[4] = std::get<4>(kv.first);
+            r[5] = kv.second.split;
+            r[6] = kv.second.gpc;
+        }
+        ++k;
+    }
+    return k;
 }
 
 int mvs_depth_map(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
@@ -864,6 +936,65 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     if (rc) return abort_call(c, rc);
     HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost,
                               c->st));
+    return sync_ctx(c);
+}
+
+int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                 double range0, double range1, const mvs_params *prm, int64_t n,
+                 const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                 const float *grad_out, float *out_cost, float *grad4) {
+    if (!c) return MVS_ERR_ARG;
+    TRY(check_params(c, prm));
+    if (nsrc < 2) return fail(c, MVS_NO_NEIGHBOURS, "a source pair needs two neighbours");
+    if (n < 0 || n >= (1ll << 31) || (n > 0 && (!px || !py || !hyp4 || !pairs || !out_cost)))
+        return fail(c, MVS_ERR_ARG, "bad pair list");
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<const ViewDev *> sv;
+    TRY(upload_host_views(c, ref, srcs, nsrc, sv));
+    const ViewDev &rf = c->tmp_views[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (px[i] < 0 || py[i] < 0 || px[i] >= rf.w || py[i] >= rf.h)
+            return fail(c, MVS_ERR_ARG, "pixel outside the reference view");
+        const int32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+        const bool choose = a == -1 && b == -1;
+        if (!choose && (a < 0 || b < 0 || a >= nsrc || b >= nsrc || a == b))
+            return fail(c, MVS_ERR_ARG, "source pair must be two distinct sources or (-1, -1)");
+    }
+    if (n == 0) return MVS_OK;
+    HIP_TRY(c, c->px.ensure((size_t)n * 4));
+    HIP_TRY(c, c->py.ensure((size_t)n * 4));
+    HIP_TRY(c, c->hyp_in.ensure((size_t)n * 16));
+    HIP_TRY(c, c->cost_out.ensure((size_t)n * 4));
+    HIP_TRY(c, c->pairs.ensure((size_t)n * 8));
+    HIP_TRY(c, c->grad_io.ensure((size_t)n * 16));
+    HIP_TRY(c, c->gout.ensure((size_t)n * 4));
+    HIP_TRY(c, hipMemcpyAsync(c->px.p, px, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(c, hipMemcpyAsync(c->py.p, py, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(c, hipMemcpyAsync(c->hyp_in.p, hyp4, (size_t)n * 16, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(c, hipMemcpyAsync(c->pairs.p, pairs, (size_t)n * 8, hipMemcpyHostToDevice, c->st));
+    if (grad_out)
+        HIP_TRY(c, hipMemcpyAsync(c->gout.p, grad_out, (size_t)n * 4, hipMemcpyHostToDevice,
+                                  c->st));
+    // Build the job (rays, sources, mask) without running PatchMatch.
+    std::vector<JobSpec> jobs(1);
+    jobs[0].ref = &rf;
+    jobs[0].srcs = sv;
+    jobs[0].range0 = range0;
+    jobs[0].range1 = range1;
+    int rc = run_batch(c, jobs, prm, 0, 0, nullptr, nullptr);
+    if (rc) return abort_call(c, rc);
+    rc = launch_pair_list(prm->taps, c->host_jobs.back(), n, c->px.as<int32_t>(),
+                          c->py.as<int32_t>(), c->hyp_in.as<float4>(), c->pairs.as<int32_t>(),
+                          grad_out ? c->gout.as<float>() : nullptr, c->cost_out.as<float>(),
+                          grad4 ? c->grad_io.as<float4>() : nullptr, c->st);
+    if (rc) return abort_call(c, fail(c, MVS_ERR_ARG, "no pair kernel for this window"));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost,
+                              c->st));
+    HIP_TRY(c, hipMemcpyAsync(pairs, c->pairs.p, (size_t)n * 8, hipMemcpyDeviceToHost, c->st));
+    if (grad4)
+        HIP_TRY(c, hipMemcpyAsync(grad4, c->grad_io.p, (size_t)n * 16, hipMemcpyDeviceToHost,
+                                  c->st));
     return sync_ctx(c);
 }
 
@@ -1032,7 +1163,8 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
                            int32_t rule, int32_t *agree, int64_t *n_consistent) {
     if (!s) return MVS_ERR_ARG;
     mvs_ctx *c = s->ctx;
-    if (rule != MVS_CONS_REFERENCE) return fail(c, MVS_ERR_ARG, "unknown consistency rule");
+    if (rule != MVS_CONS_REFERENCE && rule != MVS_CONS_REPROJ)
+        return fail(c, MVS_ERR_ARG, "unknown consistency rule");
     const int32_t nv = (int32_t)s->views.size();
     if (ref < 0 || ref >= nv || !s->maps[(size_t)ref].valid)
         return fail(c, MVS_ERR_ARG, "reference view has no depth map");
@@ -1074,10 +1206,10 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
         HIP_TRY(c, hipMemcpyAsync(c->nbr_cams.p, cams.data(), sizeof(NbrCam) * cams.size(),
                                   hipMemcpyHostToDevice, c->st));
     HIP_TRY(c, hipMemsetAsync(c->ok_count.p, 0, sizeof(unsigned long long), c->st));
-    // dense.rs:585
-    const int32_t need = nn >= 3 ? 2 : 1;
+    // dense.rs:585 for the reference rule; three views for the reprojection rule
+    const int32_t need = rule == MVS_CONS_REPROJ ? 3 : (nn >= 3 ? 2 : 1);
     launch_consistency(rc64, s->maps[(size_t)ref].depth.as<float>(), rv.w, rv.h,
-                       c->nbr_cams.as<NbrCam>(), nn, need, c->agree.as<int32_t>(),
+                       c->nbr_cams.as<NbrCam>(), nn, need, rule, c->agree.as<int32_t>(),
                        c->ok_count.as<unsigned long long>(), c->st);
     HIP_TRY(c, hipGetLastError());
     unsigned long long ok = 0;

@@ -88,21 +88,31 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 // lanes per pixel (4, 8, 16); split: lanes per source. Returns -1 for a class
 // that is not instantiated.
 enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
-int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole,
+// persist: resident workgroups with atomic work fetch (half-sweeps only);
+// otherwise one workgroup per band position.
+int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
               const PmArgs &a, int32_t grid, hipStream_t st);
-int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
-                 int32_t grid, hipStream_t st);
-int launch_pm_t7(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
-                 int32_t grid, hipStream_t st);
-int launch_pm_t11(int mode, int32_t srcl, int32_t split, bool pinhole, const PmArgs &a,
-                  int32_t grid, hipStream_t st);
+int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                 const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t7(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                 const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t11(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                  const PmArgs &a, int32_t grid, hipStream_t st);
 // Tap splits instantiated for a window size (ascending), count returned.
 int splits_for(int32_t taps, int32_t *out);
 void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_t st);
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
-                        const NbrCam *nbrs, int32_t nn, int32_t need, int32_t *agree,
-                        unsigned long long *n_ok, hipStream_t st);
+                        const NbrCam *nbrs, int32_t nn, int32_t need, int32_t rule,
+                        int32_t *agree, unsigned long long *n_ok, hipStream_t st);
 void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st);
+// Gauss-Newton refinement of a job's accepted pixels (steps per pixel);
+// counters (nullable) [0] += accepted steps, [1] += pixels refined.
+int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long long *counters,
+                  hipStream_t st);
+// Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
+int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
+                     const int32_t *py, const float4 *hyp, int32_t *pairs, const float *gout,
+                     float *cost, float4 *grad, hipStream_t st);
 // Source lanes per reference pixel for nsrc sources (one lane per source).
 int lanes_for(int32_t nsrc);
 bool taps_supported(int32_t taps);

@@ -165,9 +165,13 @@ __global__ void k_threshold(RefJob job, float *__restrict__ depth, float *__rest
 }
 
 // Geometric consistency (dense.rs:585-615) in f64 like the reference's fusion.
+// rule 0: dense.rs:585-615 (relative depth difference < 1%); rule 1: the
+// neighbour's own point at the pixel the ref point lands on, back-projected
+// from that pixel's centre, reprojects within 1 px of the ref pixel centre.
 __global__ void k_consistency(RefCam64 rf, const float *__restrict__ depth, int32_t w, int32_t h,
                               const NbrCam *__restrict__ nbrs, int32_t nn, int32_t need,
-                              int32_t *__restrict__ agree, unsigned long long *n_ok) {
+                              int32_t rule, int32_t *__restrict__ agree,
+                              unsigned long long *n_ok) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool good = false;
     if (i < (int64_t)w * h) {
@@ -219,7 +223,47 @@ __global__ void k_consistency(RefCam64 rf, const float *__restrict__ depth, int3
                 if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)v.w) || !(fy < (double)v.h))
                     continue;
                 const double dn = (double)v.depth[(int64_t)fy * v.w + (int64_t)fx];
-                if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) ++a;
+                if (rule == 0) {
+                    if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) ++a;
+                    continue;
+                }
+                if (!(dn > 0.0)) continue;
+                // back-project the neighbour pixel centre at its own depth
+                const double nqx = (fx + 0.5 - v.cx) / v.f, nqy = (fy + 0.5 - v.cy) / v.f;
+                double npx = nqx, npy = nqy;
+                for (int k2 = 0; k2 < 8; ++k2) {
+                    const double r2n = npx * npx + npy * npy;
+                    double dd = 1.0 + v.k1 * r2n;
+                    dd = dd + (v.k2 * r2n) * r2n;
+                    npx = nqx / dd;
+                    npy = nqy / dd;
+                }
+                const double xn0 = npx * dn, xn1 = npy * dn, xn2 = dn;
+                double er[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    double acc = v.R[0 * 3 + r] * xn0;
+                    acc = acc + v.R[1 * 3 + r] * xn1;
+                    acc = acc + v.R[2 * 3 + r] * xn2;
+                    er[r] = acc + v.C[r] - rf.C[r];
+                }
+                double xr[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    double acc = rf.R[r * 3 + 0] * er[0];
+                    acc = acc + rf.R[r * 3 + 1] * er[1];
+                    acc = acc + rf.R[r * 3 + 2] * er[2];
+                    xr[r] = acc;
+                }
+                if (!(xr[2] > 0.0)) continue;
+                const double rx = xr[0] / xr[2], ry = xr[1] / xr[2];
+                const double r2r = rx * rx + ry * ry;
+                double dr = 1.0 + rf.k1 * r2r;
+                dr = dr + (rf.k2 * r2r) * r2r;
+                const double fr = rf.f * dr;
+                const double ex = (fr * rx + rf.cx) - ((double)x + 0.5);
+                const double ey = (fr * ry + rf.cy) - ((double)y + 0.5);
+                if (ex * ex + ey * ey <= 1.0) ++a;
             }
             agree[i] = a;
             good = a >= need;
@@ -282,11 +326,11 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
                        write_hyp);
 }
 
-int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole,
+int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
               const PmArgs &a, int32_t grid, hipStream_t st) {
-    if (taps == 5) return launch_pm_t5(mode, srcl, split, pinhole, a, grid, st);
-    if (taps == 7) return launch_pm_t7(mode, srcl, split, pinhole, a, grid, st);
-    if (taps == 11) return launch_pm_t11(mode, srcl, split, pinhole, a, grid, st);
+    if (taps == 5) return launch_pm_t5(mode, srcl, split, pinhole, persist, a, grid, st);
+    if (taps == 7) return launch_pm_t7(mode, srcl, split, pinhole, persist, a, grid, st);
+    if (taps == 11) return launch_pm_t11(mode, srcl, split, pinhole, persist, a, grid, st);
     return -1;
 }
 
@@ -321,11 +365,11 @@ void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_
 }
 
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
-                        const NbrCam *nbrs, int32_t nn, int32_t need, int32_t *agree,
-                        unsigned long long *n_ok, hipStream_t st) {
+                        const NbrCam *nbrs, int32_t nn, int32_t need, int32_t rule,
+                        int32_t *agree, unsigned long long *n_ok, hipStream_t st) {
     const int64_t n = (int64_t)w * h;
     hipLaunchKernelGGL(k_consistency, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ref,
-                       depth, w, h, nbrs, nn, need, agree, n_ok);
+                       depth, w, h, nbrs, nn, need, rule, agree, n_ok);
 }
 
 }  // namespace amvs

@@ -66,7 +66,11 @@ __global__ void k_pm_prep(PmArgs a, int32_t mode, int32_t chunk) {
     }
 }
 
-template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE>
+// PERSIST: a grid of resident workgroups that fetch band positions with
+// atomics and steal across XCDs; otherwise a static grid where workgroup b
+// takes position b / 8 of XCD (b mod 8)'s sequence (the grid is sized for
+// the longest possible sequences and surplus workgroups exit at once).
+template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST>
 __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
@@ -90,14 +94,21 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
     const int home = blockIdx.x % kNumXcd;
     unsigned long long n_evals = 0;
 
-    for (int attempt = 0; attempt < kNumXcd; ++attempt) {
+    for (int attempt = 0; attempt < (PERSIST ? kNumXcd : 1); ++attempt) {
         const int xcd = (home + attempt) % kNumXcd;
         const int32_t *bp = a.prefix + (int64_t)(1 + xcd) * (nj + 1);
         const int32_t total = bp[nj];
+        bool first = true;
         while (true) {
             int32_t v = 0;
-            if (lane == 0) v = atomicAdd(&a.counters[xcd], 1);
-            v = __builtin_amdgcn_readfirstlane(__shfl(v, 0));
+            if (PERSIST) {
+                if (lane == 0) v = atomicAdd(&a.counters[xcd], 1);
+                v = __builtin_amdgcn_readfirstlane(__shfl(v, 0));
+            } else {
+                if (!first) break;
+                first = false;
+                v = (int32_t)(blockIdx.x / kNumXcd);
+            }
             if (v >= total) break;
             // job of sequence position v: the last j with bp[j] <= v
             int32_t jl = 0, jh = nj - 1;
@@ -412,40 +423,49 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
     }
 }
 
-// Prep + persistent launch of one kernel class.
-template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE>
+// Prep + launch of one kernel class.
+template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST>
 void launch_pm_class(const PmArgs &a, int32_t grid, hipStream_t st) {
     constexpr int PIX = 64 / (SRCL * SPLIT);
     hipLaunchKernelGGL(k_pm_prep, dim3(1), dim3(64), 0, st, a, (int32_t)MODE, PIX * a.gpc);
-    hipLaunchKernelGGL((k_pm<TAPS, SRCL, SPLIT, PINHOLE, MODE>), dim3((unsigned)grid), dim3(64),
-                       0, st, a);
+    hipLaunchKernelGGL((k_pm<TAPS, SRCL, SPLIT, PINHOLE, MODE, PERSIST>), dim3((unsigned)grid),
+                       dim3(64), 0, st, a);
 }
 
 // Dispatch over the classes instantiated for one window size.
 template <int TAPS, int... SPLITS>
 struct PmDispatch {
     template <int SRCL, int SPLIT, bool PH>
-    static void go(int mode, const PmArgs &a, int32_t grid, hipStream_t st) {
+    static void go(int mode, bool persist, const PmArgs &a, int32_t grid, hipStream_t st) {
         if (mode == MODE_EVAL_CURRENT)
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_EVAL_CURRENT>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_EVAL_CURRENT, false>(a, grid, st);
+        else if (mode == MODE_SWEEP && persist)
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, true>(a, grid, st);
         else if (mode == MODE_SWEEP)
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, false>(a, grid, st);
         else
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_COST_LIST>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_COST_LIST, false>(a, grid, st);
     }
     template <int SPLIT>
-    static bool try_split(int mode, int srcl, int split, bool ph, const PmArgs &a, int32_t grid,
-                          hipStream_t st) {
+    static bool try_split(int mode, int srcl, int split, bool ph, bool persist, const PmArgs &a,
+                          int32_t grid, hipStream_t st) {
         if (split != SPLIT) return false;
-        if (srcl == 4) ph ? go<4, SPLIT, true>(mode, a, grid, st) : go<4, SPLIT, false>(mode, a, grid, st);
-        else if (srcl == 8) ph ? go<8, SPLIT, true>(mode, a, grid, st) : go<8, SPLIT, false>(mode, a, grid, st);
-        else if (srcl == 16) ph ? go<16, SPLIT, true>(mode, a, grid, st) : go<16, SPLIT, false>(mode, a, grid, st);
-        else return false;
+        if (srcl == 4)
+            ph ? go<4, SPLIT, true>(mode, persist, a, grid, st)
+               : go<4, SPLIT, false>(mode, persist, a, grid, st);
+        else if (srcl == 8)
+            ph ? go<8, SPLIT, true>(mode, persist, a, grid, st)
+               : go<8, SPLIT, false>(mode, persist, a, grid, st);
+        else if (srcl == 16)
+            ph ? go<16, SPLIT, true>(mode, persist, a, grid, st)
+               : go<16, SPLIT, false>(mode, persist, a, grid, st);
+        else
+            return false;
         return true;
     }
-    static int run(int mode, int srcl, int split, bool ph, const PmArgs &a, int32_t grid,
-                   hipStream_t st) {
-        const bool done = (try_split<SPLITS>(mode, srcl, split, ph, a, grid, st) || ...);
+    static int run(int mode, int srcl, int split, bool ph, bool persist, const PmArgs &a,
+                   int32_t grid, hipStream_t st) {
+        const bool done = (try_split<SPLITS>(mode, srcl, split, ph, persist, a, grid, st) || ...);
         return done ? 0 : -1;
     }
 };

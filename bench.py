@@ -4,10 +4,12 @@ Workload (the survey the metric is quoted on): 77 nadir views at 2000x1500 u8
 grey, seeded synthetic heightfield survey; each view is a reference view with
 its 8 nearest-GPS neighbours, 7x7 NCC window, 6 PatchMatch iterations, then the
 geometric-consistency filter. A "step" is one reference view's depth map
-(initialisation + 12 half-sweeps + threshold). The timed region runs K steps
-as one batch - every half-sweep of all K views is one persistent launch - plus
-the consistency filter of those K views, with all images already resident in
-HBM. The plain path (one view per call) is timed on a subset beside it.
+(initialisation + 12 half-sweeps + Gauss-Newton refinement + threshold). The
+timed region runs K steps as one batch - every half-sweep of all K views is one
+persistent launch - plus the consistency filter of those K views, with all
+images already resident in HBM. The plain path (one view per call) is timed on
+a subset beside it, and the same subset is run without refinement (the
+reference's algorithm) to report the depth error refinement removes.
 
 Prints one JSON line (rank 0 of one GPU).
 """
@@ -90,6 +92,7 @@ def main():
     ap.add_argument("--cpu-crop", default="640x480")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--plain-views", type=int, default=8)
+    ap.add_argument("--refine-steps", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this depth engine runs on a single MI355X "
@@ -111,7 +114,9 @@ def main():
                                     spec.neighbours)
         jobs.append((i, nb, spec.depth_range))
     steps = max(1, min(args.steps, len(jobs)))
-    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations,
+                      refine_steps=args.refine_steps)
+    prm_plain = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
 
     ctx = capi.Context(0)
     sv = capi.Survey(ctx, surv.views)
@@ -138,6 +143,21 @@ def main():
                                for j in jobs[:plain_n]) / dtp / 1e6,
                      unit="depth Mpix/s", views=plain_n)
         log(f"plain GPU path: {plain['value']:.2f} depth Mpix/s over {plain_n} views")
+        # the same views without refinement (the reference's algorithm), untimed
+        errs_u, errs_r = [], []
+        for (i, nb, rng) in jobs[:plain_n]:
+            d_r, _, _ = sv.download(i)
+            errs_r.append(depth_errors(d_r, surv.views[i].depth_gt))
+            sv.depth_map(i, nb, rng[0], rng[1], prm_plain)
+            d_u, _, _ = sv.download(i)
+            errs_u.append(depth_errors(d_u, surv.views[i].depth_gt))
+        ctx.synchronize()
+        plain["depth_error_unrefined"] = {k: float(np.mean([e[k] for e in errs_u]))
+                                          for k in errs_u[0]}
+        plain["depth_error_refined"] = {k: float(np.mean([e[k] for e in errs_r]))
+                                        for k in errs_r[0]}
+        log(f"refinement on {plain_n} views: {plain['depth_error_unrefined']} -> "
+            f"{plain['depth_error_refined']}")
     ctx.reset_stats()
 
     ctx.synchronize()
@@ -181,7 +201,7 @@ def main():
     if not args.no_cpu:
         cw, ch = (int(a) for a in args.cpu_crop.split("x"))
         log("CPU baseline (oracle) ...")
-        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm)
+        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm_plain)
         log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
 
     line = dict(
@@ -195,11 +215,16 @@ def main():
                     views=len(surv.views), width=spec.width, height=spec.height,
                     neighbours=spec.neighbours, window=f"{spec.taps}x{spec.taps}/stride{spec.stride}",
                     iterations=spec.iterations, timed_views=steps,
-                    path="batched GPU (one persistent launch per half-sweep for all views)",
-                    schedule=dict(split=st["split"], gpc=st["gpc"])),
+                    refine_steps=args.refine_steps,
+                    path=("batched GPU (one persistent launch per half-sweep for all views) + "
+                          f"{args.refine_steps} Gauss-Newton refinement steps"),
+                    schedule=[dict(iteration=t["iteration"], split=t["split"], gpc=t["gpc"])
+                              for t in ctx.tuned_schedules()]),
         roofline=roofline, cpu_baseline=cpu, depth_error=err, plain_gpu=plain,
         gpu_vs_cpu=(value / cpu["value"]) if cpu else None,
         evals_per_pixel=st["evals"] / max(1, pixels),
+        refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
+                    pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
     )
     print(json.dumps(line), flush=True)
     sv.close()

@@ -73,7 +73,10 @@ typedef struct {
     int32_t iterations;      /* PatchMatch iterations */
     float texture_min;
     float cost_max;
-    int32_t reserved[3];     /* zero */
+    int32_t refine_steps;    /* Gauss-Newton refinement steps after PatchMatch
+                              * (0 = none, the reference's behaviour; no
+                              * reference counterpart) */
+    int32_t reserved[2];     /* zero */
 } mvs_params;
 
 void mvs_default_params(mvs_params *p);
@@ -120,6 +123,21 @@ int mvs_ncc_cost(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_
                  double range0, double range1, const mvs_params *prm, int64_t n,
                  const int32_t *px, const int32_t *py, const float *hyp4, float *out_cost);
 
+/* Fixed-pair NCC cost and its gradient: the forward and backward pass of the
+ * refinement (no reference counterpart; the cost is Ctx2::cost's best-two
+ * average, dense.rs:255-262, with the two sources given).
+ * cost_i = 1 - (NCC_s0 + NCC_s1) / 2 for item i's plane hyp4[i] = {d, n0, n1,
+ * n2} at pixel (px[i], py[i]) with sources pairs[2i], pairs[2i+1]; an item
+ * whose pair is (-1, -1) gets its two best sources at that plane (ties to the
+ * lower index), written back to pairs (-1, -1 stays when fewer than two
+ * sources are defined). grad4 (nullable): n x 4 floats, grad_out[i] (nullable
+ * = 1) times d cost_i / d(d, n0, n1, n2) with n free (not renormalised).
+ * Undefined items get cost 2 and gradient 0. */
+int mvs_ncc_pair(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                 double range0, double range1, const mvs_params *prm, int64_t n,
+                 const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                 const float *grad_out, float *out_cost, float *grad4);
+
 /* Run PatchMatch in steps on a host state of w*h x {d, n0, n1, n2, cost}
  * (Hyp, dense.rs:156-161). first_step = -1 runs the initialisation
  * (dense.rs:365-402) before the sweeps; half-sweep s covers iteration s/2,
@@ -158,9 +176,12 @@ int mvs_survey_download(mvs_survey *sv, int32_t view, float *depth, float *norma
  * (fusion filter, dense.rs:575-615): neighbours = the first prm.neighbours+1 of
  * cam.neighbours that have a depth map (the caller passes that list). rule
  * MVS_CONS_REFERENCE: relative depth difference < 1% in >= 2 maps (>= 1 when
- * fewer than 3 neighbours). Writes per-pixel agree counts (-1 where depth <= 0)
- * and returns the count of consistent pixels via n_consistent. */
-enum { MVS_CONS_REFERENCE = 0 };
+ * fewer than 3 neighbours). rule MVS_CONS_REPROJ (no reference counterpart):
+ * the neighbour's own depth at the pixel the ref point lands on, back-projected
+ * from that pixel's centre, reprojects within 1 px of the ref pixel centre, in
+ * >= 3 maps. Writes per-pixel agree counts (-1 where depth <= 0) and returns the
+ * count of consistent pixels via n_consistent. */
+enum { MVS_CONS_REFERENCE = 0, MVS_CONS_REPROJ = 1 };
 int mvs_survey_consistency(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int32_t nn,
                            int32_t rule, int32_t *agree, int64_t *n_consistent);
 
@@ -176,15 +197,26 @@ typedef struct {
     int64_t sweep_evals;      /* of which in the half-sweeps */
     int32_t split;            /* schedule of the last batch: lanes per source */
     int32_t gpc;              /* and pixel groups per work chunk */
+    int64_t refine_launches;  /* Gauss-Newton refinement kernel */
+    double refine_ms;
+    int64_t refine_pixels;    /* pixels refined (PatchMatch cost below cost_max) */
+    int64_t refine_accepted;  /* accepted refinement steps */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
 
 /* Kernel schedule (no reference counterpart; results do not depend on it):
  * split = lanes sharing one source's window (0 = choose automatically), gpc =
- * pixel groups per work chunk (0 = default). autotune = 1 times the candidate
- * schedules once per kernel class on the first batch and keeps the fastest. */
-int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t autotune);
+ * pixel groups per work chunk (0 = default), per_cu = persistent workgroups per
+ * compute unit (0 = one workgroup per chunk). autotune = 1 times the candidate
+ * schedules per kernel class and iteration on the first batch and keeps the
+ * fastest. */
+int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t per_cu,
+                     int32_t autotune);
+/* The autotuner's choices so far, 7 int32 per entry: taps, source lanes,
+ * pinhole, log2 pixels, iteration, split, gpc. Returns the entry count (writes
+ * at most cap entries). */
+int32_t mvs_tuned_schedules(mvs_ctx *ctx, int32_t *out, int32_t cap);
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
 int mvs_reset_stats(mvs_ctx *ctx);
 

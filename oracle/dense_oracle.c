@@ -558,6 +558,7 @@ typedef struct {
     int32_t nnbr;
     int32_t *agree;
     int32_t need;
+    int32_t rule; /* 0: dense.rs:585-615; 1: reprojection <= 1 px */
 } cons_arg;
 
 /* dense.rs:585-615 */
@@ -604,7 +605,36 @@ static void cons_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
                     continue;
                 int64_t ux = (int64_t)fx, uy = (int64_t)fy;
                 double dn = (double)a->nbrs[k].depth[(size_t)uy * (size_t)v->w + (size_t)ux];
-                if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) agree += 1;
+                if (a->rule == 0) {
+                    if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) agree += 1;
+                    continue;
+                }
+                /* reprojection: the neighbour's own point at that pixel, back in
+                 * the reference image, within 1 px of the pixel centre */
+                if (!(dn > 0.0)) continue;
+                double pn[2];
+                orc_undistort(v->f, v->cx, v->cy, v->k1, v->k2, (double)ux + 0.5,
+                              (double)uy + 0.5, pn);
+                double xn[3] = {pn[0] * dn, pn[1] * dn, dn}, xwn[3];
+                for (int r = 0; r < 3; ++r) {
+                    double acc = v->R[0 * 3 + r] * xn[0];
+                    acc = acc + v->R[1 * 3 + r] * xn[1];
+                    acc = acc + v->R[2 * 3 + r] * xn[2];
+                    xwn[r] = acc + v->C[r];
+                }
+                double er[3] = {xwn[0] - rf->C[0], xwn[1] - rf->C[1], xwn[2] - rf->C[2]}, xr[3];
+                for (int r = 0; r < 3; ++r) {
+                    double acc = rf->R[r * 3 + 0] * er[0];
+                    acc = acc + rf->R[r * 3 + 1] * er[1];
+                    acc = acc + rf->R[r * 3 + 2] * er[2];
+                    xr[r] = acc;
+                }
+                if (!(xr[2] > 0.0)) continue;
+                double ur[2];
+                orc_distort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, xr[0] / xr[2], xr[1] / xr[2],
+                            ur);
+                double ex = ur[0] - ((double)x + 0.5), ey = ur[1] - ((double)y + 0.5);
+                if (ex * ex + ey * ey <= 1.0) agree += 1;
             }
             a->agree[i] = agree;
             if (agree >= a->need) *cnt += 1;
@@ -613,6 +643,136 @@ static void cons_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
 
 int64_t orc_consistency(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
                         int32_t nnbr, int32_t *agree, int32_t nthreads) {
-    cons_arg a = {ref, depth, nbrs, nnbr, agree, nnbr >= 3 ? 2 : 1};
+    cons_arg a = {ref, depth, nbrs, nnbr, agree, nnbr >= 3 ? 2 : 1, 0};
     return run_rows(cons_rows, &a, ref->h, nthreads);
+}
+
+int64_t orc_consistency_reproj(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                               int32_t nnbr, int32_t need, int32_t *agree, int32_t nthreads) {
+    cons_arg a = {ref, depth, nbrs, nnbr, agree, need, 1};
+    return run_rows(cons_rows, &a, ref->h, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fixed-pair f64 cost and gradient (checks of the GPU refinement)          */
+/* ------------------------------------------------------------------------ */
+
+/* Bilinear grey at pixel-centre coordinates in f64 with its derivatives
+ * (the same cell convention as gray_at, dense.rs:69-85). */
+static int gray_at64(const float *g, int32_t w, int32_t h, double x, double y, double *val,
+                     double *gu, double *gv, uint64_t *cell) {
+    double fx = x - 0.5, fy = y - 0.5;
+    if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)(w - 1)) || !(fy < (double)(h - 1)))
+        return 0;
+    int64_t x0 = (int64_t)fx, y0 = (int64_t)fy;
+    double ax = fx - (double)x0, ay = fy - (double)y0;
+    size_t i = (size_t)y0 * (size_t)w + (size_t)x0;
+    double g00 = g[i], g10 = g[i + 1], g01 = g[i + (size_t)w], g11 = g[i + (size_t)w + 1];
+    double top = g00 * (1.0 - ax) + g10 * ax, bot = g01 * (1.0 - ax) + g11 * ax;
+    *val = top * (1.0 - ay) + bot * ay;
+    *gu = (1.0 - ay) * (g10 - g00) + ay * (g11 - g01);
+    *gv = (1.0 - ax) * (g01 - g00) + ax * (g11 - g10);
+    *cell = *cell * 1000003ull + (uint64_t)(y0 * 65536 + x0);
+    return 1;
+}
+
+int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
+                    int32_t s1, double *cost, double grad[4], uint64_t *cells) {
+    const int32_t w = c->rf.w, h = c->rf.h, N = c->n;
+    if (s0 < 0 || s1 < 0 || s0 >= c->nsrc || s1 >= c->nsrc) return 0;
+    const double d = hyp[0], n[3] = {hyp[1], hyp[2], hyp[3]};
+    const float *rc = c->rays + 2 * ((size_t)y * (size_t)w + (size_t)x);
+    const double ch[3] = {rc[0], rc[1], 1.0};
+    const double nc = n[0] * ch[0] + n[1] * ch[1] + n[2] * ch[2];
+    const double cc = d * nc;
+    double refv[ORC_MAX_TAPS], dq[ORC_MAX_TAPS], rk[ORC_MAX_TAPS][3];
+    double ddq[ORC_MAX_TAPS][4]; /* d dq / d(d, n0, n1, n2) */
+    for (int32_t k = 0; k < N; ++k) {
+        int32_t qx = x + c->wdx[k], qy = y + c->wdy[k];
+        if (qx < 0 || qy < 0 || qx >= w || qy >= h) return 0;
+        size_t qi = (size_t)qy * (size_t)w + (size_t)qx;
+        refv[k] = c->rf.gray[qi];
+        rk[k][0] = c->rays[2 * qi];
+        rk[k][1] = c->rays[2 * qi + 1];
+        rk[k][2] = 1.0;
+        double den = n[0] * rk[k][0] + n[1] * rk[k][1] + n[2];
+        if (fabs(den) < 1e-6) return 0;
+        dq[k] = cc / den;
+        if (dq[k] <= 0.0) return 0;
+        ddq[k][0] = nc / den;
+        for (int a = 0; a < 3; ++a) ddq[k][1 + a] = (d * ch[a] - dq[k] * rk[k][a]) / den;
+    }
+    double mr = 0.0;
+    for (int32_t k = 0; k < N; ++k) mr += refv[k];
+    mr /= N;
+    double na = 0.0;
+    for (int32_t k = 0; k < N; ++k) na += (refv[k] - mr) * (refv[k] - mr);
+    na = sqrt(na);
+    if (na < 1e-12) return 0;
+    uint64_t cell = 0;
+    double total = 0.0, gsum[4] = {0, 0, 0, 0};
+    const int32_t pair[2] = {s0, s1};
+    for (int ps = 0; ps < 2; ++ps) {
+        const int32_t si = pair[ps];
+        const orc_view *v = &c->srcs[si];
+        double Rr[9], tt[3];
+        /* relative pose in f64 (the f32 cast is the kernels' business) */
+        {
+            const double *Rr_ = c->rf.R, *Cr = c->rf.C, *Rs = v->R, *Cs = v->C;
+            double dc[3] = {Cr[0] - Cs[0], Cr[1] - Cs[1], Cr[2] - Cs[2]};
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j)
+                    Rr[i * 3 + j] = Rs[i * 3] * Rr_[j * 3] + Rs[i * 3 + 1] * Rr_[j * 3 + 1] +
+                                    Rs[i * 3 + 2] * Rr_[j * 3 + 2];
+                tt[i] = Rs[i * 3] * dc[0] + Rs[i * 3 + 1] * dc[1] + Rs[i * 3 + 2] * dc[2];
+            }
+        }
+        double sv[ORC_MAX_TAPS], dsv[ORC_MAX_TAPS][4];
+        for (int32_t k = 0; k < N; ++k) {
+            double p[3] = {dq[k] * rk[k][0], dq[k] * rk[k][1], dq[k]};
+            double xs[3], wk[3];
+            for (int i = 0; i < 3; ++i) {
+                xs[i] = Rr[i * 3] * p[0] + Rr[i * 3 + 1] * p[1] + Rr[i * 3 + 2] * p[2] + tt[i];
+                wk[i] = Rr[i * 3] * rk[k][0] + Rr[i * 3 + 1] * rk[k][1] + Rr[i * 3 + 2];
+            }
+            if (xs[2] <= 0.0) return 0;
+            const double qx = xs[0] / xs[2], qy = xs[1] / xs[2];
+            const double r2 = qx * qx + qy * qy;
+            const double D = 1.0 + v->k1 * r2 + v->k2 * r2 * r2;
+            const double dD = 2.0 * v->k1 + 4.0 * v->k2 * r2; /* dD/dx = dD * x */
+            const double u = v->f * D * qx + v->cx, vv = v->f * D * qy + v->cy;
+            double val, gu, gv;
+            if (!gray_at64(v->gray, v->w, v->h, u, vv, &val, &gu, &gv, &cell)) return 0;
+            sv[k] = val;
+            /* d(u, v)/d(qx, qy) */
+            const double ux = v->f * (D + qx * dD * qx), uy = v->f * qx * dD * qy;
+            const double vx = v->f * qy * dD * qx, vy = v->f * (D + qy * dD * qy);
+            /* d(qx, qy)/d dq = (w0 - qx w2, w1 - qy w2) / xs2 */
+            const double dqx = (wk[0] - qx * wk[2]) / xs[2], dqy = (wk[1] - qy * wk[2]) / xs[2];
+            const double dI = gu * (ux * dqx + uy * dqy) + gv * (vx * dqx + vy * dqy);
+            for (int a = 0; a < 4; ++a) dsv[k][a] = dI * ddq[k][a];
+        }
+        double ms = 0.0;
+        for (int32_t k = 0; k < N; ++k) ms += sv[k];
+        ms /= N;
+        double nb = 0.0, ab = 0.0;
+        for (int32_t k = 0; k < N; ++k) {
+            nb += (sv[k] - ms) * (sv[k] - ms);
+            ab += (refv[k] - mr) * (sv[k] - ms);
+        }
+        nb = sqrt(nb);
+        if (nb < 1e-12) return 0;
+        const double ncc = ab / (na * nb);
+        total += ncc;
+        /* d ncc / d sv_k = (a_k / |a| - ncc b_k / |b|) / |b| */
+        for (int32_t k = 0; k < N; ++k) {
+            const double g = ((refv[k] - mr) / na - ncc * (sv[k] - ms) / nb) / nb;
+            for (int a = 0; a < 4; ++a) gsum[a] += g * dsv[k][a];
+        }
+    }
+    *cost = 1.0 - 0.5 * total;
+    if (grad)
+        for (int a = 0; a < 4; ++a) grad[a] = -0.5 * gsum[a];
+    if (cells) *cells = cell;
+    return 1;
 }

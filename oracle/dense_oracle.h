@@ -87,6 +87,17 @@ void orc_threshold(const orc_ctx *c, const float *state, float *depth);
 int64_t orc_depth_map(const orc_ctx *c, int32_t iterations, int32_t nthreads, float *state,
                       uint8_t *textured, float *depth);
 
+/* Fixed-pair NCC cost in f64 and its analytic gradient — the checks of the
+ * GPU's backward pass and refinement (no reference counterpart: the reference
+ * has no refinement). cost = 1 - (NCC_s0 + NCC_s1) / 2 for the plane
+ * (d, n) with the given two sources; returns 0 when the cost is undefined
+ * (window off the image, degenerate plane, failed or flat source). grad (may be
+ * NULL) receives d cost / d(d, n0, n1, n2); cells (may be NULL) receives a
+ * signature of the bilinear cells every sample fell in, so finite differences
+ * can skip steps that cross a cell edge. */
+int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
+                    int32_t s1, double *cost, double grad[4], uint64_t *cells);
+
 /* Geometric consistency of one reference depth map against neighbour maps
  * (dense.rs:585-615). Writes the agree count per pixel (-1 where depth <= 0)
  * and returns the number of pixels with agree >= need. */
@@ -96,6 +107,13 @@ typedef struct {
 } orc_nbr_depth;
 int64_t orc_consistency(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
                         int32_t nnbr, int32_t *agree, int32_t nthreads);
+/* Reprojection rule (no reference counterpart; the rule the 77-view survey
+ * workload states): a neighbour agrees when the ref pixel's point lands on a
+ * neighbour pixel (floor of the projection, as above) whose own depth, back-
+ * projected from that pixel's centre and projected into the reference,
+ * falls within 1 px of the ref pixel's centre; consistent when agree >= need. */
+int64_t orc_consistency_reproj(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                               int32_t nnbr, int32_t need, int32_t *agree, int32_t nthreads);
 
 #ifdef __cplusplus
 }

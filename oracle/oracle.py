@@ -85,6 +85,13 @@ def lib():
         L.orc_consistency.argtypes = [C.POINTER(OrcView), _f32p, C.POINTER(OrcNbrDepth),
                                       C.c_int32, _i32p, C.c_int32]
         L.orc_consistency.restype = C.c_int64
+        L.orc_consistency_reproj.argtypes = [C.POINTER(OrcView), _f32p, C.POINTER(OrcNbrDepth),
+                                             C.c_int32, C.c_int32, _i32p, C.c_int32]
+        L.orc_consistency_reproj.restype = C.c_int64
+        L.orc_pair_cost64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int32,
+                                      C.c_int32, C.POINTER(C.c_double), _f64p,
+                                      C.POINTER(C.c_uint64)]
+        L.orc_pair_cost64.restype = C.c_int
         _lib = L
     return _lib
 
@@ -239,6 +246,31 @@ class OracleDepth:
         lib().orc_cost_batch(self.ctx, len(xs), xs, ys, hy.ravel(), out, nthreads)
         return out
 
+    def pair_cost64(self, x, y, hyp, s0, s1):
+        """f64 fixed-pair cost 1 - (NCC_s0 + NCC_s1)/2, its analytic gradient
+        w.r.t. (d, n0, n1, n2) and the bilinear-cell signature of every sample;
+        None where the cost is undefined. (s, s) gives 1 - NCC_s."""
+        h = np.ascontiguousarray(hyp, np.float64).reshape(4)
+        c = C.c_double(0.0)
+        g = np.zeros(4, np.float64)
+        cell = C.c_uint64(0)
+        ok = lib().orc_pair_cost64(self.ctx, int(x), int(y), h, int(s0), int(s1), C.byref(c), g,
+                                   C.byref(cell))
+        if not ok:
+            return None
+        return c.value, g, cell.value
+
+    def pair_costs64(self, xs, ys, hyps, pairs):
+        """Vector form of pair_cost64: (cost (nan if undefined), grad (n, 4))."""
+        n = len(xs)
+        cost = np.full(n, np.nan)
+        grad = np.full((n, 4), np.nan)
+        for i in range(n):
+            r = self.pair_cost64(xs[i], ys[i], hyps[i], pairs[i][0], pairs[i][1])
+            if r is not None:
+                cost[i], grad[i] = r[0], r[1]
+        return cost, grad
+
     def init(self, nthreads=8):
         st = np.zeros((self.h, self.w, 5), np.float32)
         tex = np.zeros((self.h, self.w), np.uint8)
@@ -264,9 +296,10 @@ class OracleDepth:
         return d, st, tex, evals
 
 
-def consistency(ref, depth, nbrs, nthreads=8):
-    """Agree counts of the fusion filter (dense.rs:585-615). nbrs: list of
-    (view, depth map). Returns (agree int32 map, number consistent)."""
+def consistency(ref, depth, nbrs, nthreads=8, rule=0, need=3):
+    """Agree counts of the fusion filter (dense.rs:585-615; rule 0) or of the
+    reprojection rule (rule 1: <= 1 px, consistent when agree >= need). nbrs:
+    list of (view, depth map). Returns (agree int32 map, number consistent)."""
     rv = _orc_view(ref, None)
     keep = []
     arr = (OrcNbrDepth * max(len(nbrs), 1))()
@@ -279,5 +312,10 @@ def consistency(ref, depth, nbrs, nthreads=8):
         arr[i].depth = d.ctypes.data
     depth = np.ascontiguousarray(depth, np.float32)
     agree = np.zeros(depth.shape, np.int32)
-    n = lib().orc_consistency(C.byref(rv), depth.ravel(), arr, len(nbrs), agree.ravel(), nthreads)
+    if rule == 0:
+        n = lib().orc_consistency(C.byref(rv), depth.ravel(), arr, len(nbrs), agree.ravel(),
+                                  nthreads)
+    else:
+        n = lib().orc_consistency_reproj(C.byref(rv), depth.ravel(), arr, len(nbrs), need,
+                                         agree.ravel(), nthreads)
     return agree, n

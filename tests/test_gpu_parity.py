@@ -257,14 +257,14 @@ def test_ncc_cost_parity_every_schedule(gpu_ctx, small_survey, taps, stride, k):
     splits = {5: [1], 7: [1, 2], 11: [2, 4]}[taps]
     try:
         for split in splits:
-            for gpc in (1, 8):
+            for gpc, per_cu in ((1, 16), (8, 16), (1, 0)):
                 _say(f"case ncc_cost schedule taps={taps} K={k} split={split} gpc={gpc}")
-                gpu_ctx.set_schedule(split=split, gpc=gpc, autotune=False)
+                gpu_ctx.set_schedule(split=split, gpc=gpc, per_cu=per_cu, autotune=False)
                 got = gpu_ctx.ncc_cost(s.views[ref], [s.views[j] for j in nb], 40, 80, prm,
                                        xs, ys, hy)
                 assert _compare_costs(got, want, f"{taps}x{stride}/K{k}/split{split}") == 1.0
     finally:
-        gpu_ctx.set_schedule(0, 0, True)
+        gpu_ctx.set_schedule(0, 0, 16, True)
 
 
 def test_depth_map_identical_across_schedules(gpu_ctx, small_survey):
@@ -274,12 +274,12 @@ def test_depth_map_identical_across_schedules(gpu_ctx, small_survey):
     prm = capi.params(taps=7, stride=1, iterations=3)
     maps = []
     try:
-        for split, gpc in ((1, 4), (2, 16), (2, 1)):
-            _say(f"case depth map schedule split={split} gpc={gpc}")
-            gpu_ctx.set_schedule(split=split, gpc=gpc, autotune=False)
+        for split, gpc, per_cu in ((1, 4, 16), (2, 16, 16), (2, 1, 0), (4, 1, 16)):
+            _say(f"case depth map schedule split={split} gpc={gpc} per_cu={per_cu}")
+            gpu_ctx.set_schedule(split=split, gpc=gpc, per_cu=per_cu, autotune=False)
             maps.append(gpu_ctx.depth_map(rv, sv, 40, 80, prm))
     finally:
-        gpu_ctx.set_schedule(0, 0, True)
+        gpu_ctx.set_schedule(0, 0, 16, True)
     for m in maps[1:]:
         for a, b in zip(maps[0], m):
             assert np.array_equal(a, b)

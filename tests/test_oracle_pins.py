@@ -480,3 +480,71 @@ def test_consistency_brute_force(tiny_survey):
     # a neighbour identical to the reference always agrees
     a2, _ = orc.consistency(views[ref], dref, [(views[ref], dref)])
     assert np.all(a2[dref > 0] == 1) and np.all(a2[dref <= 0] == -1)
+
+
+def _reproj64(ref, depth, nbrs):
+    """float64 numpy statement of the reprojection rule (pinhole cameras):
+    ref pixel -> world -> neighbour pixel (floor) -> the neighbour's depth
+    there, back-projected from that pixel's centre -> reference image; agree
+    when within 1 px of the ref pixel centre."""
+    h, w = depth.shape
+    agree = np.full((h, w), -1, np.int64)
+    ys, xs = np.nonzero(depth > 0)
+    d = depth[ys, xs].astype(np.float64)
+    Rr = np.asarray(ref.R)
+    Xc = np.stack([(xs + 0.5 - ref.cx) / ref.f * d, (ys + 0.5 - ref.cy) / ref.f * d, d], 1)
+    Xw = Xc @ Rr + ref.C
+    cnt = np.zeros(len(d), np.int64)
+    for v, dn in nbrs:
+        Rn = np.asarray(v.R)
+        xs_ = (Xw - v.C) @ Rn.T
+        z = xs_[:, 2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u = np.floor(v.f * xs_[:, 0] / z + v.cx)
+            vv = np.floor(v.f * xs_[:, 1] / z + v.cy)
+        ok = (z > 0) & (u >= 0) & (vv >= 0) & (u < v.w) & (vv < v.h)
+        dd = np.zeros(len(d))
+        dd[ok] = dn[vv[ok].astype(int), u[ok].astype(int)]
+        ok &= dd > 0
+        Xn = np.stack([(u + 0.5 - v.cx) / v.f * dd, (vv + 0.5 - v.cy) / v.f * dd, dd], 1) @ Rn + v.C
+        xr = (Xn - ref.C) @ Rr.T
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ur = ref.f * xr[:, 0] / xr[:, 2] + ref.cx
+            vr = ref.f * xr[:, 1] / xr[:, 2] + ref.cy
+        err2 = (ur - (xs + 0.5)) ** 2 + (vr - (ys + 0.5)) ** 2
+        cnt += ok & (xr[:, 2] > 0) & (err2 <= 1.0)
+    agree[ys, xs] = cnt
+    return agree
+
+
+def test_consistency_reprojection_rule(tiny_survey):
+    s = tiny_survey
+    views = s.views
+    ref = s.reference_index()
+    rng = np.random.default_rng(12)
+    dref = views[ref].depth_gt.astype(np.float32) * rng.uniform(
+        0.99, 1.01, views[ref].depth_gt.shape).astype(np.float32)
+    dref[rng.uniform(size=dref.shape) < 0.1] = 0.0
+    nbrs = []
+    for j in s.neighbours[ref][:5]:
+        dj = views[j].depth_gt.astype(np.float32) * rng.uniform(
+            0.99, 1.01, views[j].depth_gt.shape).astype(np.float32)
+        nbrs.append((views[j], dj))
+    agree, n = orc.consistency(views[ref], dref, nbrs, rule=1, need=3)
+    want = _reproj64(views[ref], dref, nbrs)
+    # identical except within f64 rounding of a pixel edge or the 1 px circle
+    assert np.mean(agree == want) > 0.999
+    assert n == int(((agree >= 3) & (dref > 0)).sum())
+    # depth noise of 1% moves reprojections: some but not all agree
+    frac = np.mean(agree[dref > 0] >= 3)
+    assert 0.05 < frac < 0.999
+    # a neighbour identical to the reference reprojects exactly
+    a2, _ = orc.consistency(views[ref], dref, [(views[ref], dref)], rule=1, need=1)
+    assert np.all(a2[dref > 0] == 1) and np.all(a2[dref <= 0] == -1)
+    # exact ground-truth maps agree almost everywhere they overlap
+    gt = [(views[j], views[j].depth_gt.astype(np.float32)) for j in s.neighbours[ref][:5]]
+    a3, _ = orc.consistency(views[ref], views[ref].depth_gt.astype(np.float32), gt, rule=1,
+                            need=1)
+    w3 = _reproj64(views[ref], views[ref].depth_gt.astype(np.float32), gt)
+    assert np.mean(a3 == w3) > 0.999
+    assert np.mean(a3 >= 1) > 0.9
