@@ -27,7 +27,7 @@ EXPORTS = [
     "mvs_survey_set_view", "mvs_survey_depth_map", "mvs_survey_depth_maps",
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
-    "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair",
+    "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair", "mvs_set_graphs",
 ]
 
 
@@ -52,7 +52,8 @@ class MvsStats(C.Structure):
                 ("init_evals", C.c_int64), ("sweep_evals", C.c_int64),
                 ("split", C.c_int32), ("gpc", C.c_int32),
                 ("refine_launches", C.c_int64), ("refine_ms", C.c_double),
-                ("refine_pixels", C.c_int64), ("refine_accepted", C.c_int64)]
+                ("refine_pixels", C.c_int64), ("refine_accepted", C.c_int64),
+                ("graph_captures", C.c_int64), ("graph_replays", C.c_int64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)
@@ -107,6 +108,7 @@ def lib():
         L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
         L.mvs_reset_stats.argtypes = [vp]
         L.mvs_set_schedule.argtypes = [vp, i32, i32, i32, i32]
+        L.mvs_set_graphs.argtypes = [vp, i32]
         L.mvs_tuned_schedules.argtypes = [vp, vp, i32]
         L.mvs_tuned_schedules.restype = i32
         L.mvs_selftest.argtypes = [vp, i32, i64, C.c_uint64, P(C.c_int64)]
@@ -306,6 +308,9 @@ class Context:
     def set_schedule(self, split=0, gpc=0, per_cu=16, autotune=True):
         self.check(lib().mvs_set_schedule(self._h, split, gpc, per_cu, 1 if autotune else 0))
 
+    def set_graphs(self, enable=True):
+        self.check(lib().mvs_set_graphs(self._h, 1 if enable else 0))
+
     def tuned_schedules(self):
         n = lib().mvs_tuned_schedules(self._h, None, 0)
         out = np.zeros((max(n, 1), 7), np.int32)
@@ -371,11 +376,12 @@ class Survey:
                                                  c.ctypes.data))
         return d, n, c
 
-    def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE):
+    def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE, download=True):
+        """(agree counts or None, number of consistent pixels)."""
         h, w = self.shapes[ref]
         nb = np.ascontiguousarray(nbrs, np.int32)
-        agree = np.zeros((h, w), np.int32)
+        agree = np.zeros((h, w), np.int32) if download else None
         n = C.c_int64()
         self.ctx.check(lib().mvs_survey_consistency(self._h, ref, nb.ctypes.data, len(nb), rule,
-                                                    agree.ctypes.data, C.byref(n)))
+                                                    _ptr(agree), C.byref(n)))
         return agree, n.value

@@ -113,6 +113,9 @@ struct mvs_ctx {
     Schedule forced{0, 0, 0};
     bool autotune = true;
     std::map<TuneKey, Schedule> tuned;
+    // half-sweep graphs, keyed by the bytes of their launches
+    bool graphs = true;
+    std::map<std::string, hipGraphExec_t> graph_cache;
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -479,6 +482,92 @@ Schedule forced_schedule(mvs_ctx *c, int32_t taps, int32_t srcl) {
 // initialisation (dense.rs:365-402) then n_steps half-sweeps from 0;
 // first_step >= 0 keeps the state already in the job slots and runs half-sweeps
 // first_step .. first_step + n_steps - 1.
+// One half-sweep of every class of a batch, launched directly or as a cached
+// hipGraph. A graph is keyed by the bytes of its launches (arguments, grids,
+// kernel selectors), so a replay repeats exactly what a capture recorded; the
+// job tables it reads are device memory rewritten by each call. The kernels
+// clear their own counters (no memset nodes).
+struct SweepLaunch {
+    PmArgs a;
+    int32_t grid, srcl, split, pinhole, persist, taps;
+};
+
+int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes, int32_t it,
+                   int32_t parity, unsigned long long *evals, std::vector<SweepLaunch> &out) {
+    out.clear();
+    for (const KClass &k : classes) {
+        SweepLaunch L;
+        memset(&L, 0, sizeof L);
+        L.a = base_args(c, k);
+        L.a.it = it;
+        L.a.parity = parity;
+        L.a.range = ldexpf(1.0f, -it);  // 0.5f32.powi(it), dense.rs:414
+        L.a.evals = evals;
+        L.persist = k.sch.per_cu > 0;
+        L.grid = grid_for(c, k, L.persist != 0,
+                          [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
+        L.srcl = k.srcl;
+        L.split = k.sch.split;
+        L.pinhole = k.pinhole;
+        L.taps = taps;
+        out.push_back(L);
+    }
+    return MVS_OK;
+}
+
+int enqueue_sweeps(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
+    for (const SweepLaunch &L : ls) {
+        if (launch_pm(PM_SWEEP, L.taps, L.srcl, L.split, L.pinhole != 0, L.persist != 0, L.a,
+                      L.grid, c->st))
+            return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
+        HIP_TRY(c, hipGetLastError());
+    }
+    return MVS_OK;
+}
+
+int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
+    std::string key;
+    auto put = [&key](const void *p, size_t n) { key.append((const char *)p, n); };
+    for (const SweepLaunch &L : ls) {
+        put(&L.a.jobs, sizeof L.a.jobs);
+        put(&L.a.njobs, sizeof L.a.njobs);
+        put(&L.a.prefix, sizeof L.a.prefix);
+        put(&L.a.counters, sizeof L.a.counters);
+        put(&L.a.it, sizeof L.a.it);
+        put(&L.a.parity, sizeof L.a.parity);
+        put(&L.a.range, sizeof L.a.range);
+        put(&L.a.gpc, sizeof L.a.gpc);
+        put(&L.a.evals, sizeof L.a.evals);
+        const int32_t sel[6] = {L.grid, L.srcl, L.split, L.pinhole, L.persist, L.taps};
+        put(sel, sizeof sel);
+    }
+    auto f = c->graph_cache.find(key);
+    if (f == c->graph_cache.end()) {
+        if (c->graph_cache.size() >= 512) {
+            for (auto &kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
+            c->graph_cache.clear();
+        }
+        HIP_TRY(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_sweeps(c, ls);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIP_TRY(c, e);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(c, ei);
+        f = c->graph_cache.emplace(key, x).first;
+        c->stats.graph_captures += 1;
+    }
+    HIP_TRY(c, hipGraphLaunch(f->second, c->st));
+    c->stats.graph_replays += 1;
+    return MVS_OK;
+}
+
 int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *prm,
               int32_t first_step, int32_t n_steps, const volatile int32_t *cancel,
               std::vector<KClass> *classes_out) {
@@ -614,11 +703,14 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         }
     }
     // half-sweeps (dense.rs:413-467); the schedule of each class is forced,
-    // cached, or tuned per iteration on the class's first job
+    // cached, or tuned per iteration on the class's first job. A half-sweep
+    // whose schedules are all known runs as a cached graph.
     const int32_t s0 = first_step < 0 ? 0 : first_step;
+    std::vector<SweepLaunch> ls;
     for (int32_t s = s0; s < s0 + n_steps; ++s) {
         if (cancel && *cancel) return fail(c, MVS_ERR_CANCELLED, "cancelled");
         const int32_t it = s / 2, parity = s % 2;
+        bool known = true;
         for (KClass &k : classes) {
             if (c->forced.split > 0) {
                 k.sch = forced_schedule(c, taps, k.srcl);
@@ -630,26 +722,26 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
                 } else if (c->autotune) {
                     TRY(autotune_class(c, taps, k, table[(size_t)k.first], it, parity));
                     c->tuned[key] = k.sch;
+                    known = false;
                 } else {
                     k.sch = candidates(taps, k.srcl)[0];
                 }
             }
-            hipEvent_t ea = nullptr, eb = nullptr;
-            if (c->timing) {
-                ea = take_event(c);
-                eb = take_event(c);
-                HIP_TRY(c, hipEventRecord(ea, c->st));
-            }
-            PmArgs a = base_args(c, k);
-            a.it = it;
-            a.parity = parity;
-            a.range = ldexpf(1.0f, -it);  // 0.5f32.powi(it), dense.rs:414
-            a.evals = evals + 1;
-            TRY(pm_launch(c, PM_SWEEP, taps, k, a));
-            if (c->timing) {
-                HIP_TRY(c, hipEventRecord(eb, c->st));
-                c->pending.push_back({ea, eb, 0});
-            }
+        }
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (c->timing) {
+            ea = take_event(c);
+            eb = take_event(c);
+            HIP_TRY(c, hipEventRecord(ea, c->st));
+        }
+        TRY(sweep_launches(c, taps, classes, it, parity, evals + 1, ls));
+        if (c->graphs && known)
+            TRY(run_sweeps_graph(c, ls));
+        else
+            TRY(enqueue_sweeps(c, ls));
+        if (c->timing) {
+            HIP_TRY(c, hipEventRecord(eb, c->st));
+            c->pending.push_back({ea, eb, 0});
         }
     }
     for (const JobSpec &js : jobs) c->stats.pixels += (int64_t)js.ref->w * js.ref->h;
@@ -816,6 +908,7 @@ void mvs_destroy(mvs_ctx *c) {
         (void)hipEventDestroy(t.b);
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
+    for (auto &kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
     for (auto &kv : c->rays) {
         kv.second->release();
         delete kv.second;
@@ -848,6 +941,17 @@ int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
         return MVS_ERR_ARG;
     c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;
+    return MVS_OK;
+}
+
+int mvs_set_graphs(mvs_ctx *c, int32_t enable) {
+    if (!c) return MVS_ERR_ARG;
+    if (!enable) {
+        (void)hipStreamSynchronize(c->st);
+        for (auto &kv : c->graph_cache) (void)hipGraphExecDestroy(kv.second);
+        c->graph_cache.clear();
+    }
+    c->graphs = enable != 0;
     return MVS_OK;
 }
 

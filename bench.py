@@ -27,6 +27,12 @@ sys.path[:0] = [os.path.join(REPO, "aerial-3d-model_amd")]
 METRIC = ("depth Mpix/s (7x7 NCC, 8 nbrs) batched+refined GPU vs plain GPU vs CPU; depth err")
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE over the 12 half-sweeps of an 8-view batch of this
+# workload, scaled by a 1 GiB device copy (read counter x2.000, write x1.000);
+# profiles/k_pm_sweep_8x2000x1500_k8_7x7_counters.md holds the per-launch figures.
+READ_BYTES_PER_EVAL = 718.9
+WRITE_BYTES_PER_EVAL = 7.42
 
 
 def log(*a):
@@ -89,16 +95,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-crop", default="640x480")
+    ap.add_argument("--cpu-crop", default="1280x960")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--plain-views", type=int, default=8)
-    ap.add_argument("--refine-steps", type=int, default=4)
+    ap.add_argument("--refine-steps", type=int, default=5)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB, 5 times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this depth engine runs on a single MI355X "
               "(multi-GPU sharding by reference view is designed but not built); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe << 18  # float32 elements
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe} MiB 5 times")
+        return
 
     from aerial_mvs import capi, survey
 
@@ -164,10 +184,12 @@ def main():
     t0 = time.perf_counter()
     sv.depth_maps(timed, prm)
     n_cons = 0
+    cons_nbrs = {}
     for (i, _, _) in timed:
         nbrs = [j for j in surv.neighbours[i][:spec.neighbours + 1] if j in done]
+        cons_nbrs[i] = nbrs
         if nbrs:
-            n_cons += sv.consistency(i, nbrs)[1]
+            n_cons += sv.consistency(i, nbrs, rule=capi.MVS_CONS_REPROJ, download=False)[1]
     ctx.synchronize()
     t1 = time.perf_counter()
 
@@ -182,19 +204,31 @@ def main():
     avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
     evals_per_launch = st["sweep_evals"] / max(1, st["sweep_launches"])
     achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
+    traffic = (READ_BYTES_PER_EVAL + WRITE_BYTES_PER_EVAL) * evals_per_launch
     roofline = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None,
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
+                    traffic_note=("HBM-side bytes per launch: rocprofv3 FETCH_SIZE/WRITE_SIZE per "
+                                  "evaluation (copy-calibrated) x evaluations per launch"),
+                    hbm_gbs=traffic / (avg_ms * 1e-3) / 1e9,
                     kernel=f"k_pm<7,8,{st['split']},pinhole,sweep>", avg_launch_ms=avg_ms,
                     evals_per_launch=evals_per_launch, flops_per_eval=fpe,
                     launches=st["sweep_launches"])
 
-    # depth error against the FP64 ground truth, all timed views
-    errs = []
+    # depth error against the FP64 ground truth, all timed views: every valid
+    # pixel, and the pixels the consistency filter keeps
+    errs, errs_c = [], []
     for (i, _, _) in timed:
         d, _, _ = sv.download(i)
         errs.append(depth_errors(d, surv.views[i].depth_gt))
+        if cons_nbrs[i]:
+            agree, _ = sv.consistency(i, cons_nbrs[i], rule=capi.MVS_CONS_REPROJ)
+            errs_c.append(depth_errors(np.where(agree >= 3, d, 0.0), surv.views[i].depth_gt))
     err = {k: float(np.mean([e.get(k, 0.0) for e in errs])) for k in errs[0]}
+    if errs_c:
+        err["consistent"] = {k: float(np.mean([e.get(k, 0.0) for e in errs_c]))
+                             for k in errs_c[0]}
     err["consistent_pixels"] = int(n_cons)
+    err["consistency_rule"] = "reprojection <= 1 px in >= 3 of the K+1 nearest views"
     err["reference"] = "FP64 ray-cast ground truth of the synthetic scene"
 
     cpu = None

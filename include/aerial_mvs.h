@@ -187,8 +187,8 @@ int mvs_survey_consistency(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int
 
 /* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
 typedef struct {
-    int64_t sweep_launches;
-    double sweep_ms;          /* summed device time of the half-sweep kernel */
+    int64_t sweep_launches;   /* half-sweeps (every kernel class of a batch) */
+    double sweep_ms;          /* summed device time of the half-sweep kernels */
     int64_t init_launches;
     double init_ms;
     int64_t evals;            /* cost evaluations counted on the device */
@@ -201,6 +201,8 @@ typedef struct {
     double refine_ms;
     int64_t refine_pixels;    /* pixels refined (PatchMatch cost below cost_max) */
     int64_t refine_accepted;  /* accepted refinement steps */
+    int64_t graph_captures;   /* half-sweep graphs captured */
+    int64_t graph_replays;    /* half-sweeps run as a graph launch */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
@@ -213,6 +215,12 @@ int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
  * fastest. */
 int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune);
+/* Half-sweep graphs (no reference counterpart; results do not depend on it):
+ * with enable = 1 (the default) every half-sweep whose schedules are known
+ * runs as a hipGraph of its launches, captured once and replayed by later
+ * half-sweeps and calls with the same launch arguments. 0 launches directly
+ * and frees the cached graphs. */
+int mvs_set_graphs(mvs_ctx *ctx, int32_t enable);
 /* The autotuner's choices so far, 7 int32 per entry: taps, source lanes,
  * pinhole, log2 pixels, iteration, split, gpc. Returns the entry count (writes
  * at most cap entries). */

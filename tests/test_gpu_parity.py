@@ -203,6 +203,16 @@ def test_survey_batch_and_consistency(gpu_ctx, tiny_survey):
         oag, on = orc.consistency(s.views[ref], maps[ref], [(s.views[j], maps[j]) for j in nbrs])
         _say(f"ref {ref}: consistent gpu={n_ok} oracle={on}")
         assert np.array_equal(agree, oag) and n_ok == on
+        # reprojection rule (<= 1 px in >= 3 views): both sides compute in f64
+        # with the same operation order, so the counts are identical
+        agree, n_ok = sv.consistency(ref, nbrs, rule=capi.MVS_CONS_REPROJ)
+        oag, on = orc.consistency(s.views[ref], maps[ref], [(s.views[j], maps[j]) for j in nbrs],
+                                  rule=1, need=3)
+        _say(f"ref {ref}: reprojection-consistent gpu={n_ok} oracle={on}")
+        assert np.array_equal(agree, oag) and n_ok == on
+        assert 0 < n_ok
+    with pytest.raises(capi.MvsError):
+        sv.consistency(i0, s.neighbours[i0][:2], rule=7)
     sv.close()
 
 
@@ -299,3 +309,32 @@ def test_full_depth_map_11x11(gpu_ctx, small_survey):
     within = float(np.mean(np.abs(gd[both] - od[both]) / od[both] <= 0.005))
     _say(f"11x11: within0.5%={within:.6f} identical={np.mean(gd == od):.6f}")
     assert within >= 0.99
+
+
+def test_half_sweep_graph_replays_match_direct_launches(gpu_ctx, small_survey):
+    """Half-sweeps run as cached hipGraphs give the same maps as direct
+    launches, on every replay, with other work (a cost list, a depth map of
+    another window) and host synchronisations between the replays."""
+    s = small_survey
+    ref = s.reference_index()
+    srcs = [s.views[j] for j in s.neighbours[ref][:4]]
+    prm = capi.params(taps=7, stride=1, iterations=3)
+    other = capi.params(taps=5, stride=2, iterations=2)
+    rng = np.random.default_rng(7)
+    xs, ys, hy = _random_hyps(s, ref, 500, rng)
+    _say("case graphs off")
+    gpu_ctx.set_graphs(False)
+    want, wn, wc = gpu_ctx.depth_map(s.views[ref], srcs, *s.spec.depth_range, prm)
+    gpu_ctx.set_graphs(True)
+    gpu_ctx.reset_stats()
+    for rep in range(3):
+        _say(f"case graph replay {rep}")
+        d, n, c = gpu_ctx.depth_map(s.views[ref], srcs, *s.spec.depth_range, prm)
+        assert np.array_equal(d, want) and np.array_equal(n, wn) and np.array_equal(c, wc)
+        gpu_ctx.ncc_cost(s.views[ref], srcs, *s.spec.depth_range, prm, xs, ys, hy)
+        gpu_ctx.depth_map(s.views[ref], srcs[:3], *s.spec.depth_range, other)
+        gpu_ctx.synchronize()
+    st = gpu_ctx.stats()
+    _say(f"graph captures {st['graph_captures']}, replays {st['graph_replays']}")
+    assert st["graph_replays"] >= 3 * 6
+    assert st["graph_captures"] < st["graph_replays"]
