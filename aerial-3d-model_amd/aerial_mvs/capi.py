@@ -19,6 +19,8 @@ MVS_GRAY_U8 = 0
 MVS_GRAY_F32 = 1
 MVS_CONS_REFERENCE = 0
 MVS_CONS_REPROJ = 1
+MVS_MODE_EXACT = 0
+MVS_MODE_FAST = 1
 
 EXPORTS = [
     "mvs_default_params", "mvs_dense_params_for_profile", "mvs_select_neighbours",
@@ -42,7 +44,7 @@ class MvsView(C.Structure):
 class MvsParams(C.Structure):
     _fields_ = [("taps", C.c_int32), ("stride", C.c_int32), ("iterations", C.c_int32),
                 ("texture_min", C.c_float), ("cost_max", C.c_float),
-                ("refine_steps", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("refine_steps", C.c_int32), ("mode", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class MvsStats(C.Structure):
@@ -118,11 +120,13 @@ def lib():
     return _lib
 
 
-def params(taps=5, stride=2, iterations=4, texture_min=0.008, cost_max=0.45, refine_steps=0):
+def params(taps=5, stride=2, iterations=4, texture_min=0.008, cost_max=0.45, refine_steps=0,
+           fast=False):
     p = MvsParams()
     lib().mvs_default_params(C.byref(p))
     p.taps, p.stride, p.iterations = taps, stride, iterations
     p.refine_steps = refine_steps
+    p.mode = MVS_MODE_FAST if fast else MVS_MODE_EXACT
     p.texture_min, p.cost_max = texture_min, cost_max
     return p
 
@@ -313,9 +317,10 @@ class Context:
 
     def tuned_schedules(self):
         n = lib().mvs_tuned_schedules(self._h, None, 0)
-        out = np.zeros((max(n, 1), 7), np.int32)
+        out = np.zeros((max(n, 1), 9), np.int32)
         lib().mvs_tuned_schedules(self._h, out.ctypes.data, n)
-        keys = ("taps", "srcl", "pinhole", "log2px", "iteration", "split", "gpc")
+        keys = ("taps", "srcl", "pinhole", "log2px", "iteration", "mode", "split", "gpc",
+                "per_cu")
         return [dict(zip(keys, map(int, r))) for r in out[:n]]
 
     def selftest_division(self, n, seed=1):

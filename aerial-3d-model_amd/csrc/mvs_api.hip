@@ -51,6 +51,9 @@ struct DevBuf {
 struct ViewDev {
     int32_t w = 0, h = 0;
     DevBuf gray;
+    DevBuf quad;   // corner-packed 8-bit plane (8-bit inputs)
+    DevBuf quadf;  // corner-packed grey plane
+    bool has_quad = false;
     double f = 0, cx = 0, cy = 0, k1 = 0, k2 = 0;
     double R[9] = {0}, C[3] = {0};
     bool set = false;
@@ -77,8 +80,8 @@ struct JobSpec {
 };
 
 typedef std::tuple<double, double, double, double, double, int32_t, int32_t> RayKey;
-// taps, source lanes, pinhole, log2 pixels, iteration
-typedef std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t> TuneKey;
+// taps, source lanes, pinhole, log2 pixels, iteration, fast arithmetic
+typedef std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t, int32_t> TuneKey;
 
 struct Schedule {
     int32_t split = 1;
@@ -226,6 +229,8 @@ int check_params(mvs_ctx *c, const mvs_params *p) {
         return fail(c, MVS_ERR_ARG, "iterations out of range");
     if (p->refine_steps < 0 || p->refine_steps > 64)
         return fail(c, MVS_ERR_ARG, "refine steps out of range (0..64)");
+    if (p->mode != MVS_MODE_EXACT && p->mode != MVS_MODE_FAST)
+        return fail(c, MVS_ERR_ARG, "unknown arithmetic mode");
     return MVS_OK;
 }
 
@@ -241,12 +246,21 @@ int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v) {
         launch_gray_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride, dv.gray.as<float>(),
                             c->st);
         HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, dv.quad.ensure(n * sizeof(uint32_t)));
+        launch_quad_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride,
+                            dv.quad.as<uint32_t>(), c->st);
+        HIP_TRY(c, hipGetLastError());
+        dv.has_quad = true;
     } else {
+        dv.has_quad = false;
         HIP_TRY(c, hipMemcpy2DAsync(dv.gray.p, (size_t)v->w * sizeof(float), v->pixels,
                                     (size_t)v->stride * sizeof(float),
                                     (size_t)v->w * sizeof(float), v->h, hipMemcpyHostToDevice,
                                     c->st));
     }
+    HIP_TRY(c, dv.quadf.ensure(n * sizeof(float4)));
+    launch_quadf(dv.gray.as<float>(), v->w, v->h, dv.quadf.as<float4>(), c->st);
+    HIP_TRY(c, hipGetLastError());
     // the staging buffer and the caller's pixels are reused after return
     HIP_TRY(c, hipStreamSynchronize(c->st));
     dv.w = v->w;
@@ -316,6 +330,8 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
     split_f64(s.k1, c.k1, c.k1_lo);
     split_f64(s.k2, c.k2, c.k2_lo);
     c.gray = s.gray.as<float>();
+    c.quad = s.has_quad ? s.quad.as<uint32_t>() : nullptr;
+    c.quadf = s.quadf.as<float4>();
     c.w = s.w;
     c.h = s.h;
     c.pinhole_exact = (s.k1 == 0.0 && s.k2 == 0.0 && c.f_lo == 0.0f && c.cx_lo == 0.0f &&
@@ -339,6 +355,7 @@ int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
 // in the device job table.
 struct KClass {
     int32_t srcl = 0, pinhole = 1, first = 0, count = 0;
+    int32_t fast = 0;                     // MVS_MODE_FAST kernels
     int64_t cap_sweep = 0, cap_eval = 0;  // upper bounds of the list lengths
     std::vector<int64_t> caps;            // pixels of each job
     int32_t size = 0;                     // log2 pixels of the first job
@@ -391,7 +408,8 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
         grid = grid_for(c, k, persist, [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
     else
         grid = grid_for(c, k, persist, [&](int32_t j) { return k.caps[(size_t)j]; });
-    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, persist, a, grid, c->st))
+    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, persist, k.fast != 0, a, grid,
+                  c->st))
         return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
     HIP_TRY(c, hipGetLastError());
     return MVS_OK;
@@ -489,7 +507,7 @@ Schedule forced_schedule(mvs_ctx *c, int32_t taps, int32_t srcl) {
 // clear their own counters (no memset nodes).
 struct SweepLaunch {
     PmArgs a;
-    int32_t grid, srcl, split, pinhole, persist, taps;
+    int32_t grid, srcl, split, pinhole, persist, taps, fast;
 };
 
 int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes, int32_t it,
@@ -510,6 +528,7 @@ int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes,
         L.split = k.sch.split;
         L.pinhole = k.pinhole;
         L.taps = taps;
+        L.fast = k.fast;
         out.push_back(L);
     }
     return MVS_OK;
@@ -517,7 +536,8 @@ int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes,
 
 int enqueue_sweeps(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
     for (const SweepLaunch &L : ls) {
-        if (launch_pm(PM_SWEEP, L.taps, L.srcl, L.split, L.pinhole != 0, L.persist != 0, L.a,
+        if (launch_pm(PM_SWEEP, L.taps, L.srcl, L.split, L.pinhole != 0, L.persist != 0,
+                      L.fast != 0, L.a,
                       L.grid, c->st))
             return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
         HIP_TRY(c, hipGetLastError());
@@ -538,7 +558,7 @@ int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
         put(&L.a.range, sizeof L.a.range);
         put(&L.a.gpc, sizeof L.a.gpc);
         put(&L.a.evals, sizeof L.a.evals);
-        const int32_t sel[6] = {L.grid, L.srcl, L.split, L.pinhole, L.persist, L.taps};
+        const int32_t sel[7] = {L.grid, L.srcl, L.split, L.pinhole, L.persist, L.taps, L.fast};
         put(sel, sizeof sel);
     }
     auto f = c->graph_cache.find(key);
@@ -592,7 +612,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
 
     // host job descriptions
     std::vector<RefJob> hj((size_t)nj);
-    std::vector<int32_t> srcl((size_t)nj), pin((size_t)nj);
+    std::vector<int32_t> srcl((size_t)nj), pin((size_t)nj), fst((size_t)nj);
     for (int32_t i = 0; i < nj; ++i) {
         const JobSpec &js = jobs[(size_t)i];
         JobWork &wk = c->work[(size_t)i];
@@ -607,10 +627,11 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
         c->host_tables.emplace_back();
         std::vector<SrcCam> &tab = c->host_tables.back();
-        bool pinhole = true;
+        bool pinhole = true, quads = true;
         for (const ViewDev *s : js.srcs) {
             tab.push_back(make_src(ref, *s));
             pinhole = pinhole && tab.back().pinhole_exact;
+            quads = quads && s->has_quad;
         }
         HIP_TRY(c, hipMemcpyAsync(wk.srcs.p, tab.data(), sizeof(SrcCam) * (size_t)K,
                                   hipMemcpyHostToDevice, c->st));
@@ -646,19 +667,24 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         job.pinhole = pinhole ? 1 : 0;
         srcl[(size_t)i] = lanes_for(K);
         pin[(size_t)i] = pinhole ? 1 : 0;
+        // the fast kernels read the corner-packed 8-bit planes; views given
+        // as f32 grey run the exact kernels
+        fst[(size_t)i] = (prm->mode == MVS_MODE_FAST && quads) ? 1 : 0;
     }
 
     // group jobs by kernel class (source lanes, projection)
     std::vector<KClass> classes;
     std::vector<RefJob> table;
     for (int32_t sl : {4, 8, 16})
-        for (int32_t ph : {1, 0}) {
+        for (int32_t ph : {1, 0})
+            for (int32_t fa : {1, 0}) {
             KClass k;
             k.srcl = sl;
             k.pinhole = ph;
+            k.fast = fa;
             k.first = (int32_t)table.size();
             for (int32_t i = 0; i < nj; ++i)
-                if (srcl[(size_t)i] == sl && pin[(size_t)i] == ph) {
+                if (srcl[(size_t)i] == sl && pin[(size_t)i] == ph && fst[(size_t)i] == fa) {
                     table.push_back(hj[(size_t)i]);
                     const int64_t n = (int64_t)hj[(size_t)i].w * hj[(size_t)i].h;
                     k.cap_sweep += (n + 1) / 2;
@@ -715,7 +741,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
             if (c->forced.split > 0) {
                 k.sch = forced_schedule(c, taps, k.srcl);
             } else {
-                TuneKey key(taps, k.srcl, k.pinhole, k.size, it);
+                TuneKey key(taps, k.srcl, k.pinhole, k.size, it, k.fast);
                 auto f = c->tuned.find(key);
                 if (f != c->tuned.end()) {
                     k.sch = f->second;
@@ -923,7 +949,11 @@ void mvs_destroy(mvs_ctx *c) {
                       &c->pairs,    &c->grad_io, &c->gout,
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost};
     for (DevBuf *b : bufs) b->release();
-    for (auto &v : c->tmp_views) v.gray.release();
+    for (auto &v : c->tmp_views) {
+        v.gray.release();
+        v.quadf.release();
+        v.quad.release();
+    }
     (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -960,14 +990,16 @@ int32_t mvs_tuned_schedules(mvs_ctx *c, int32_t *out, int32_t cap) {
     int32_t k = 0;
     for (const auto &kv : c->tuned) {
         if (out && k < cap) {
-            int32_t *r = out + 7 * k;
+            int32_t *r = out + 9 * k;
             r[0] = std::get<0>(kv.first);
             r[1] = std::get<1>(kv.first);
             r[2] = std::get<2>(kv.first);
             r[3] = std::get<3>(kv.first);
             r[4] = std::get<4>(kv.first);
-            r[5] = kv.second.split;
-            r[6] = kv.second.gpc;
+            r[5] = std::get<5>(kv.first);
+            r[6] = kv.second.split;
+            r[7] = kv.second.gpc;
+            r[8] = kv.second.per_cu;
         }
         ++k;
     }
@@ -1165,7 +1197,11 @@ void mvs_survey_destroy(mvs_survey *s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->st);
-    for (auto &v : s->views) v.gray.release();
+    for (auto &v : s->views) {
+        v.gray.release();
+        v.quadf.release();
+        v.quad.release();
+    }
     for (auto &m : s->maps) {
         m.depth.release();
         m.normal.release();

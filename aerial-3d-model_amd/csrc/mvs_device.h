@@ -91,6 +91,11 @@ __device__ __forceinline__ void div2_shared(const float a0, const float a1, cons
 // flat_load).
 typedef const __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ gfloat *as_global(const float *p) { return (gfloat *)p; }
+typedef const __attribute__((address_space(1))) uint32_t gu32;
+__device__ __forceinline__ gu32 *as_global(const uint32_t *p) { return (gu32 *)p; }
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) f32x4 gf32x4;
+__device__ __forceinline__ gf32x4 *as_global(const float4 *p) { return (gf32x4 *)p; }
 
 // float-float helpers for the general (distorted / non-f32 intrinsics) path
 struct ff {

@@ -19,6 +19,11 @@ struct SrcCam {
     float f_lo, cx_lo, cy_lo;
     float k1, k1_lo, k2, k2_lo;
     const float *gray;  // device grey plane, row pitch = w
+    // 8-bit inputs only (else null): per pixel the bytes of (x, y), (x+1, y),
+    // (x, y+1), (x+1, y+1), low to high — a bilinear sample's four corners
+    const uint32_t *quad;
+    // per pixel the grey of the same four corners (every view)
+    const float4 *quadf;
     int32_t w, h;
     int32_t pinhole_exact;
     int32_t pad;
@@ -78,6 +83,9 @@ struct RefCam64 {
 // ---- launchers (mvs_kernels.hip) ----
 void launch_gray_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride, float *dst,
                          hipStream_t st);
+void launch_quad_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride,
+                         uint32_t *dst, hipStream_t st);
+void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st);
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                  float2 *rays, hipStream_t st);
 // write_hyp = 0 keeps a caller-provided state and only rebuilds mask + lists.
@@ -89,15 +97,22 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 // that is not instantiated.
 enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // persist: resident workgroups with atomic work fetch (half-sweeps only);
-// otherwise one workgroup per band position.
+// otherwise one workgroup per band position. fast: the contracted-arithmetic
+// kernels (MVS_MODE_FAST) instead of the reference-rounding ones.
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
-              const PmArgs &a, int32_t grid, hipStream_t st);
+              bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st);
 int launch_pm_t7(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st);
 int launch_pm_t11(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                   const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t5f(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                  const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t7f(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                  const PmArgs &a, int32_t grid, hipStream_t st);
+int launch_pm_t11f(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
+                   const PmArgs &a, int32_t grid, hipStream_t st);
 // Tap splits instantiated for a window size (ascending), count returned.
 int splits_for(int32_t taps, int32_t *out);
 void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_t st);

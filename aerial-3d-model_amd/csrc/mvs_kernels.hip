@@ -41,6 +41,33 @@ __global__ void k_gray_from_u8(const uint8_t *__restrict__ src, int32_t w, int32
     dst[i] = c_gray_lut[src[(int64_t)y * stride + x]];
 }
 
+// Corner-packed 8-bit plane for the fast kernels: pixel (x, y) holds the bytes
+// of (x, y), (x+1, y), (x, y+1), (x+1, y+1) (clamped at the last column/row),
+// so one aligned 32-bit load fetches a bilinear sample's four corners.
+__global__ void k_quad_from_u8(const uint8_t *__restrict__ src, int32_t w, int32_t h,
+                               int32_t stride, uint32_t *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    const int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    const int32_t x1 = x + 1 < w ? x + 1 : x, y1 = y + 1 < h ? y + 1 : y;
+    const uint8_t *r0 = src + (int64_t)y * stride, *r1 = src + (int64_t)y1 * stride;
+    dst[i] = (uint32_t)r0[x] | ((uint32_t)r0[x1] << 8) | ((uint32_t)r1[x] << 16) |
+             ((uint32_t)r1[x1] << 24);
+}
+
+// Corner-packed grey plane: pixel (x, y) holds the grey of (x, y), (x+1, y),
+// (x, y+1), (x+1, y+1) (clamped at the last column/row), so one 16-byte load
+// fetches a bilinear sample's four corners with their exact values.
+__global__ void k_quadf(const float *__restrict__ g, int32_t w, int32_t h,
+                        float4 *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    const int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    const int32_t x1 = x + 1 < w ? x + 1 : x, y1 = y + 1 < h ? y + 1 : y;
+    const float *r0 = g + (int64_t)y * w, *r1 = g + (int64_t)y1 * w;
+    dst[i] = make_float4(r0[x], r0[x1], r1[x], r1[x1]);
+}
+
 // dense.rs:343-349 with geometry.rs:34-45: f64 fixed point, cast to f32.
 __global__ void k_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                        float2 *__restrict__ rays) {
@@ -308,6 +335,19 @@ void launch_gray_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t strid
                        h, stride, dst);
 }
 
+void launch_quad_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride,
+                         uint32_t *dst, hipStream_t st) {
+    const int64_t n = (int64_t)w * h;
+    hipLaunchKernelGGL(k_quad_from_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src,
+                       w, h, stride, dst);
+}
+
+void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st) {
+    const int64_t n = (int64_t)w * h;
+    hipLaunchKernelGGL(k_quadf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gray, w, h,
+                       dst);
+}
+
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                  float2 *rays, hipStream_t st) {
     const int64_t n = (int64_t)w * h;
@@ -327,7 +367,13 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 }
 
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
-              const PmArgs &a, int32_t grid, hipStream_t st) {
+              bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
+    if (fast) {
+        if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole, persist, a, grid, st);
+        if (taps == 7) return launch_pm_t7f(mode, srcl, split, pinhole, persist, a, grid, st);
+        if (taps == 11) return launch_pm_t11f(mode, srcl, split, pinhole, persist, a, grid, st);
+        return -1;
+    }
     if (taps == 5) return launch_pm_t5(mode, srcl, split, pinhole, persist, a, grid, st);
     if (taps == 7) return launch_pm_t7(mode, srcl, split, pinhole, persist, a, grid, st);
     if (taps == 11) return launch_pm_t11(mode, srcl, split, pinhole, persist, a, grid, st);

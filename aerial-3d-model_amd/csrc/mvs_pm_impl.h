@@ -70,7 +70,12 @@ __global__ void k_pm_prep(PmArgs a, int32_t mode, int32_t chunk) {
 // atomics and steal across XCDs; otherwise a static grid where workgroup b
 // takes position b / 8 of XCD (b mod 8)'s sequence (the grid is sized for
 // the longest possible sequences and surplus workgroups exit at once).
-template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST>
+// FAST: the same algorithm with contracted multiply-adds, the hardware
+// reciprocal for the projection and plane divisions, and each tap slice's
+// NCC sums formed independently and added across the slice lanes; results
+// stay within the float tolerance of the exact kernels (FAST = false), which
+// round operation for operation like the reference.
+template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST, bool FAST>
 __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
@@ -137,7 +142,10 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
             const float F = S.f, CX = S.cx, CY = S.cy;
             const ff FF{S.f, S.f_lo}, FCX{S.cx, S.cx_lo}, FCY{S.cy, S.cy_lo},
                 FK1{S.k1, S.k1_lo}, FK2{S.k2, S.k2_lo};
+            const float K1 = S.k1, K2 = S.k2;
             const float *__restrict__ G = S.gray;
+            const uint32_t *__restrict__ Q = S.quad;  // fast kernels only
+            const float4 *__restrict__ QF = S.quadf;
             const int32_t sw = S.w;
             const float wm1 = (float)(S.w - 1), hm1 = (float)(S.h - 1);
 
@@ -207,6 +215,15 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                 }
                 const float2 rc = pixel_ok ? job.rays[pidx] : make_float2(0.0f, 0.0f);
                 __syncthreads();
+                // fast sums: this slice's sum of the centred reference grey
+                float asum = 0.0f;
+                if constexpr (FAST) {
+                    if (pixel_ok) {
+#pragma unroll
+                        for (int i = 0; i < SEG; ++i)
+                            if (SPLIT == 1 || part * SEG + i < N) asum += Ps[g][part * SEG + i].w;
+                    }
+                }
 
                 // Cost of hypothesis (d, n) at this group's pixel; `act` is
                 // uniform over the group. Every lane of the wave calls this.
@@ -227,7 +244,11 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                                 float den = n0 * ray.x + n1 * ray.y;
                                 den = den + n2;
                                 // IEEE quotient for |den| >= 1e-6; smaller fails
-                                const float dq = div1(cc, den, recip_refined(den));
+                                float dq;
+                                if constexpr (FAST)
+                                    dq = cc * __builtin_amdgcn_rcpf(den);
+                                else
+                                    dq = div1(cc, den, recip_refined(den));
                                 pf = pf || (fabsf(den) < 1e-6f) || !(dq > 0.0f);
                                 float *pp = &Ps[g][j].x;
                                 pp[0] = ray.x * dq;
@@ -249,29 +270,51 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                         // which gives the same cost while the gathers of
                         // successive taps overlap.
                         const float4 *P = Ps[g];
-                        float sv[SEG];
+                        // exact: the samples, for the reference's two-pass sums;
+                        // fast: single-pass sums shifted by the slice's first
+                        // sample (c0), merged across slices below
+                        float sv[FAST ? 1 : SEG];
+                        float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
                         bool ok = true;
 #pragma unroll
                         for (int i = 0; i < SEG; ++i) {
                             const int j = part * SEG + i;
                             const bool inw = SPLIT == 1 || j < N;
                             const float4 p = P[inw ? j : 0];
-                            float xs0 = R0 * p.x + R1 * p.y;
-                            xs0 = xs0 + R2 * p.z;
-                            xs0 = xs0 + T0;
-                            float xs1 = R3 * p.x + R4 * p.y;
-                            xs1 = xs1 + R5 * p.z;
-                            xs1 = xs1 + T1;
-                            float xs2 = R6 * p.x + R7 * p.y;
-                            xs2 = xs2 + R8 * p.z;
-                            xs2 = xs2 + T2;
-                            float q0, q1;
-                            div2_shared(xs0, xs1, xs2, q0, q1);
-                            float u, v;
-                            if (PINHOLE)
-                                project_pinhole(q0, q1, F, CX, CY, u, v);
-                            else
-                                project_general(q0, q1, FF, FCX, FCY, FK1, FK2, u, v);
+                            float xs2, u, v;
+                            if constexpr (FAST) {
+                                xs2 = fmaf(R6, p.x, fmaf(R7, p.y, fmaf(R8, p.z, T2)));
+                                const float xs0 = fmaf(R0, p.x, fmaf(R1, p.y, fmaf(R2, p.z, T0)));
+                                const float xs1 = fmaf(R3, p.x, fmaf(R4, p.y, fmaf(R5, p.z, T1)));
+                                const float inv = __builtin_amdgcn_rcpf(xs2);
+                                if (PINHOLE) {
+                                    const float fi = F * inv;
+                                    u = fmaf(fi, xs0, CX);
+                                    v = fmaf(fi, xs1, CY);
+                                } else {
+                                    const float q0 = xs0 * inv, q1 = xs1 * inv;
+                                    const float r2 = fmaf(q0, q0, q1 * q1);
+                                    const float fd = F * fmaf(fmaf(K2, r2, K1), r2, 1.0f);
+                                    u = fmaf(fd, q0, CX);
+                                    v = fmaf(fd, q1, CY);
+                                }
+                            } else {
+                                float xs0 = R0 * p.x + R1 * p.y;
+                                xs0 = xs0 + R2 * p.z;
+                                xs0 = xs0 + T0;
+                                float xs1 = R3 * p.x + R4 * p.y;
+                                xs1 = xs1 + R5 * p.z;
+                                xs1 = xs1 + T1;
+                                xs2 = R6 * p.x + R7 * p.y;
+                                xs2 = xs2 + R8 * p.z;
+                                xs2 = xs2 + T2;
+                                float q0, q1;
+                                div2_shared(xs0, xs1, xs2, q0, q1);
+                                if (PINHOLE)
+                                    project_pinhole(q0, q1, F, CX, CY, u, v);
+                                else
+                                    project_general(q0, q1, FF, FCX, FCY, FK1, FK2, u, v);
+                            }
                             // gray_at, dense.rs:69-85: None unless 0 <= fx < w-1
                             // and 0 <= fy < h-1 (NaN fails too)
                             const float fx = u - 0.5f, fy = v - 0.5f;
@@ -283,59 +326,112 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                             // fx >= 0: fract is fx - floor(fx) = fx - x0, exactly
                             const float ax = __builtin_amdgcn_fractf(sx),
                                         ay = __builtin_amdgcn_fractf(sy);
-                            gfloat *gp = as_global(G) + (y0 * sw + x0);
-                            const float g00 = gp[0], g10 = gp[1], g01 = gp[sw], g11 = gp[sw + 1];
-                            const float bx = 1.0f - ax, by = 1.0f - ay;
-                            float top = g00 * bx;
-                            top = top + g10 * ax;
-                            float bot = g01 * bx;
-                            bot = bot + g11 * ax;
-                            float val = top * by;
-                            val = val + bot * ay;
-                            sv[i] = val;
+                            float val;
+                            if constexpr (FAST) {
+                                // the four corners as bytes in one 32-bit load;
+                                // NCC is scale-invariant, so raw 0..255 values
+                                // serve (the grey table is b / 255 to 1 ulp)
+                                const uint32_t qd = as_global(Q)[y0 * sw + x0];
+                                const float g00 = (float)(qd & 0xffu);
+                                const float g10 = (float)((qd >> 8) & 0xffu);
+                                const float g01 = (float)((qd >> 16) & 0xffu);
+                                const float g11 = (float)(qd >> 24);
+                                const float top = fmaf(ax, g10 - g00, g00);
+                                const float bot = fmaf(ax, g11 - g01, g01);
+                                val = fmaf(ay, bot - top, top);
+                            } else {
+                                // the four corners' grey in one 16-byte load
+                                const f32x4 qv = as_global(QF)[y0 * sw + x0];
+                                const float g00 = qv.x, g10 = qv.y, g01 = qv.z, g11 = qv.w;
+                                const float bx = 1.0f - ax, by = 1.0f - ay;
+                                float top = g00 * bx;
+                                top = top + g10 * ax;
+                                float bot = g01 * bx;
+                                bot = bot + g11 * ax;
+                                val = top * by;
+                                val = val + bot * ay;
+                            }
+                            if constexpr (FAST) {
+                                if (i == 0) c0 = val;
+                                if (SPLIT == 1 || j < N) {
+                                    const float dd = val - c0;
+                                    s1 += dd;
+                                    s2 = fmaf(dd, dd, s2);
+                                    sa = fmaf(p.w, dd, sa);
+                                }
+                            } else {
+                                sv[i] = val;
+                            }
                         }
                         if (SPLIT > 1) {
                             // every slice of the source must have succeeded
                             const uint64_t bad = __ballot(!ok);
                             ok = ((bad >> src0) & SMASK) == 0ull;
                         }
-                        // mean: sum in tap order, continued slice to slice
-                        float sum = 0.0f;
+                        float ms, var = 0.0f, cov = 0.0f;
+                        if constexpr (FAST) {
+                            // merge the slices' shifted sums (pairwise
+                            // mean/variance update): slice count np, mean mp,
+                            // M2 = s2 - s1^2/np; cov uses sum(a) of the slice
+                            const int first = part * SEG;
+                            const float np =
+                                (float)(SPLIT == 1 ? N : (N - first < SEG ? N - first : SEG));
+                            float tot = fmaf(np, c0, s1);
 #pragma unroll
-                        for (int p = 0; p < SPLIT; ++p) {
-                            if (part == p) {
+                            for (int off = 1; off < SPLIT; off <<= 1) tot += __shfl_xor(tot, off);
+                            ms = tot * (1.0f / (float)N);
+                            const float mp = c0 + s1 / np;
+                            const float dm = mp - ms;
+                            var = fmaxf(s2 - s1 * (s1 / np), 0.0f) + np * dm * dm;
+                            cov = sa - (ms - c0) * asum;
 #pragma unroll
-                                for (int i = 0; i < SEG; ++i)
-                                    if (SPLIT == 1 || p * SEG + i < N) sum = sum + sv[i];
+                            for (int off = 1; off < SPLIT; off <<= 1) {
+                                var += __shfl_xor(var, off);
+                                cov += __shfl_xor(cov, off);
                             }
-                            if (SPLIT > 1) sum = __shfl(sum, src0 + p);
-                        }
-                        const float ms = sum / (float)N;
-                        // variance and covariance, same order
-                        float var = 0.0f, cov = 0.0f;
+                            var = var * (1.0f / (float)N);
+                            const float ss = __builtin_amdgcn_sqrtf(var);
+                            // dense.rs:246-248's 1e-3 grey units, in byte units
+                            if (ok && !(ss < 1e-3f * 255.0f))
+                                cs = 1.0f - cov * __builtin_amdgcn_rcpf((float)N * sr * ss);
+                        } else {
+                            // mean: sum in tap order, continued slice to slice
+                            float sum = 0.0f;
 #pragma unroll
-                        for (int p = 0; p < SPLIT; ++p) {
-                            if (part == p) {
+                            for (int p = 0; p < SPLIT; ++p) {
+                                if (part == p) {
 #pragma unroll
-                                for (int i = 0; i < SEG; ++i) {
-                                    if (SPLIT == 1 || p * SEG + i < N) {
-                                        const float dd = sv[i] - ms;
-                                        var = var + dd * dd;
-                                        cov = cov + P[p * SEG + i].w * dd;
+                                    for (int i = 0; i < SEG; ++i)
+                                        if (SPLIT == 1 || p * SEG + i < N) sum = sum + sv[i];
+                                }
+                                if (SPLIT > 1) sum = __shfl(sum, src0 + p);
+                            }
+                            ms = sum / (float)N;
+                            // variance and covariance, same order
+#pragma unroll
+                            for (int p = 0; p < SPLIT; ++p) {
+                                if (part == p) {
+#pragma unroll
+                                    for (int i = 0; i < SEG; ++i) {
+                                        if (SPLIT == 1 || p * SEG + i < N) {
+                                            const float dd = sv[i] - ms;
+                                            var = var + dd * dd;
+                                            cov = cov + P[p * SEG + i].w * dd;
+                                        }
                                     }
                                 }
+                                if (SPLIT > 1) {
+                                    var = __shfl(var, src0 + p);
+                                    cov = __shfl(cov, src0 + p);
+                                }
                             }
-                            if (SPLIT > 1) {
-                                var = __shfl(var, src0 + p);
-                                cov = __shfl(cov, src0 + p);
+                            var = var / (float)N;
+                            const float ss = sqrtf(var);
+                            if (ok && !(ss < 1e-3f)) {
+                                float den = (float)N * sr;
+                                den = den * ss;
+                                cs = 1.0f - cov / den;
                             }
-                        }
-                        var = var / (float)N;
-                        const float ss = sqrtf(var);
-                        if (ok && !(ss < 1e-3f)) {
-                            float den = (float)N * sr;
-                            den = den * ss;
-                            cs = 1.0f - cov / den;
                         }
                     }
                     // Best two of the K source costs (dense.rs:255-262): exact
@@ -424,27 +520,27 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
 }
 
 // Prep + launch of one kernel class.
-template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST>
+template <int TAPS, int SRCL, int SPLIT, bool PINHOLE, int MODE, bool PERSIST, bool FAST>
 void launch_pm_class(const PmArgs &a, int32_t grid, hipStream_t st) {
     constexpr int PIX = 64 / (SRCL * SPLIT);
     hipLaunchKernelGGL(k_pm_prep, dim3(1), dim3(64), 0, st, a, (int32_t)MODE, PIX * a.gpc);
-    hipLaunchKernelGGL((k_pm<TAPS, SRCL, SPLIT, PINHOLE, MODE, PERSIST>), dim3((unsigned)grid),
-                       dim3(64), 0, st, a);
+    hipLaunchKernelGGL((k_pm<TAPS, SRCL, SPLIT, PINHOLE, MODE, PERSIST, FAST>),
+                       dim3((unsigned)grid), dim3(64), 0, st, a);
 }
 
-// Dispatch over the classes instantiated for one window size.
-template <int TAPS, int... SPLITS>
+// Dispatch over the classes instantiated for one window size and arithmetic.
+template <int TAPS, bool FAST, int... SPLITS>
 struct PmDispatch {
     template <int SRCL, int SPLIT, bool PH>
     static void go(int mode, bool persist, const PmArgs &a, int32_t grid, hipStream_t st) {
         if (mode == MODE_EVAL_CURRENT)
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_EVAL_CURRENT, false>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_EVAL_CURRENT, false, FAST>(a, grid, st);
         else if (mode == MODE_SWEEP && persist)
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, true>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, true, FAST>(a, grid, st);
         else if (mode == MODE_SWEEP)
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, false>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_SWEEP, false, FAST>(a, grid, st);
         else
-            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_COST_LIST, false>(a, grid, st);
+            launch_pm_class<TAPS, SRCL, SPLIT, PH, MODE_COST_LIST, false, FAST>(a, grid, st);
     }
     template <int SPLIT>
     static bool try_split(int mode, int srcl, int split, bool ph, bool persist, const PmArgs &a,

@@ -5,7 +5,7 @@ namespace amvs {
 
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st) {
-    return PmDispatch<5, 1>::run(mode, srcl, split, pinhole, persist, a, grid, st);
+    return PmDispatch<5, false, 1>::run(mode, srcl, split, pinhole, persist, a, grid, st);
 }
 
 }  // namespace amvs

@@ -3,13 +3,19 @@
 Workload (the survey the metric is quoted on): 77 nadir views at 2000x1500 u8
 grey, seeded synthetic heightfield survey; each view is a reference view with
 its 8 nearest-GPS neighbours, 7x7 NCC window, 6 PatchMatch iterations, then the
-geometric-consistency filter. A "step" is one reference view's depth map
-(initialisation + 12 half-sweeps + Gauss-Newton refinement + threshold). The
-timed region runs K steps as one batch - every half-sweep of all K views is one
-persistent launch - plus the consistency filter of those K views, with all
-images already resident in HBM. The plain path (one view per call) is timed on
-a subset beside it, and the same subset is run without refinement (the
-reference's algorithm) to report the depth error refinement removes.
+geometric-consistency filter (reprojection <= 1 px in >= 3 views). A "step" is
+one reference view's depth map (initialisation + 12 half-sweeps + Gauss-Newton
+refinement + threshold).
+
+Three paths:
+  - batched+refined (the headline `value`): all K views as one batch - every
+    half-sweep of all views is one persistent launch, run as a cached graph -
+    with the fast-arithmetic kernels (costs within 1e-5 of the exact ones) and
+    5 Gauss-Newton steps, plus the consistency filter, images resident in HBM;
+  - plain: one view per call with the exact kernels and no refinement - the
+    bit-identical drop-in of the reference's algorithm - on a subset;
+  - CPU: the oracle restatement of recon/dense.rs on a bounded crop.
+Depth errors are against the FP64 ray-cast ground truth of the scene.
 
 Prints one JSON line (rank 0 of one GPU).
 """
@@ -30,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE over the 12 half-sweeps of an 8-view batch of this
 # workload, scaled by a 1 GiB device copy (read counter x2.000, write x1.000);
-# profiles/k_pm_sweep_8x2000x1500_k8_7x7_counters.md holds the per-launch figures.
-READ_BYTES_PER_EVAL = 718.9
-WRITE_BYTES_PER_EVAL = 7.42
+# profiles/k_pm_sweep_8x2000x1500_k8_7x7_counters.md holds the per-launch
+# figures, per arithmetic mode.
+BYTES_PER_EVAL = {"exact": (718.9, 7.42), "fast": (None, None)}
 
 
 def log(*a):
@@ -56,10 +62,20 @@ def depth_errors(depth, gt):
                 within_0p2=float(np.mean(rel <= 0.002)), median_rel=float(np.median(rel)))
 
 
-def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm):
+def mean_errors(errs):
+    return {k: float(np.mean([e.get(k, 0.0) for e in errs])) for k in errs[0]}
+
+
+def within(a, b, tol=0.005):
+    both = (a > 0) & (b > 0)
+    return float(np.mean(np.abs(a[both] - b[both]) / b[both] <= tol)) if both.any() else 0.0
+
+
+def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm_exact, prm_fast):
     """The CPU oracle on a bounded sample: a centred crop of the first
     reference view (its intrinsics shifted accordingly) against the same 8
-    full-size sources, I and window as the GPU run."""
+    full-size sources, I and window as the GPU run; the GPU's maps of the same
+    crop (exact and fast kernels, no refinement) are compared with it."""
     import copy
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
@@ -74,18 +90,18 @@ def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm):
     t = time.perf_counter()
     od, _, _, ev = o.depth_map(iterations=spec.iterations, nthreads=nthreads)
     dt = time.perf_counter() - t
-    # the GPU on the same crop: parity of the sample
-    gd, _, _ = gpu_ctx.depth_map(cv, sv, rng[0], rng[1], prm)
-    both = (gd > 0) & (od > 0)
-    within = float(np.mean(np.abs(gd[both] - od[both]) / od[both] <= 0.005)) if both.any() else 0.0
+    ge, _, _ = gpu_ctx.depth_map(cv, sv, rng[0], rng[1], prm_exact)
+    gf, _, _ = gpu_ctx.depth_map(cv, sv, rng[0], rng[1], prm_fast)
     mpix = crop_w * crop_h / 1e6
     return dict(value=mpix / dt, unit="depth Mpix/s", cores=nthreads, kind="port",
                 sample=(f"oracle C restatement of recon/dense.rs, {crop_w}x{crop_h} centre crop "
                         f"of view {ref} vs its {len(srcs)} full-size neighbours, "
                         f"{spec.taps}x{spec.taps} window, {spec.iterations} iterations: "
                         f"{dt:.1f} s, {ev} cost evaluations"),
-                seconds=dt, sample_gpu_within_0p5=within,
-                sample_identical=float(np.mean(gd == od)))
+                seconds=dt,
+                sample_exact_identical=float(np.mean(ge == od)),
+                sample_fast_within_0p5=within(gf, od),
+                sample_fast_valid_agree=float(np.mean((gf > 0) == (od > 0))))
 
 
 def main():
@@ -94,6 +110,8 @@ def main():
     ap.add_argument("--steps", type=int, default=77)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
+    ap.add_argument("--mode", choices=("fast", "exact"), default="fast",
+                    help="arithmetic of the batched path's PatchMatch kernels")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", default="1280x960")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -134,50 +152,44 @@ def main():
                                     spec.neighbours)
         jobs.append((i, nb, spec.depth_range))
     steps = max(1, min(args.steps, len(jobs)))
-    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations,
-                      refine_steps=args.refine_steps)
-    prm_plain = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
+    geom = dict(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
+    prm = capi.params(**geom, refine_steps=args.refine_steps, fast=args.mode == "fast")
+    prm_exact = capi.params(**geom)                    # the reference's algorithm
+    prm_fast0 = capi.params(**geom, fast=True)         # fast, no refinement
 
     ctx = capi.Context(0)
     sv = capi.Survey(ctx, surv.views)
     log("views resident in HBM")
     for w in range(args.warmup):
-        sv.depth_maps([jobs[w % len(jobs)]], prm)
+        j = jobs[w % len(jobs)]
+        sv.depth_maps([j], prm)
+        sv.depth_map(j[0], j[1], j[2][0], j[2][1], prm_exact)
     ctx.synchronize()
     ctx.reset_stats()
     ctx.set_timing(True)
     timed = jobs[:steps]
     done = set(j[0] for j in timed)
 
-    # plain path: one reference view per call
+    # plain path: one reference view per call, exact kernels, no refinement
     plain_n = max(0, min(args.plain_views, len(jobs)))
     plain = None
+    plain_maps = {}
     if plain_n:
         ctx.synchronize()
         tp = time.perf_counter()
         for (i, nb, rng) in jobs[:plain_n]:
-            sv.depth_map(i, nb, rng[0], rng[1], prm)
+            sv.depth_map(i, nb, rng[0], rng[1], prm_exact)
         ctx.synchronize()
         dtp = time.perf_counter() - tp
         plain = dict(value=sum(surv.views[j[0]].w * surv.views[j[0]].h
                                for j in jobs[:plain_n]) / dtp / 1e6,
-                     unit="depth Mpix/s", views=plain_n)
+                     unit="depth Mpix/s", views=plain_n,
+                     path="one view per call, exact kernels (bit-identical to the oracle), "
+                          "no refinement")
+        plain_maps = {i: sv.download(i)[0] for (i, _, _) in jobs[:plain_n]}
+        plain["depth_error"] = mean_errors([depth_errors(plain_maps[i], surv.views[i].depth_gt)
+                                            for i in plain_maps])
         log(f"plain GPU path: {plain['value']:.2f} depth Mpix/s over {plain_n} views")
-        # the same views without refinement (the reference's algorithm), untimed
-        errs_u, errs_r = [], []
-        for (i, nb, rng) in jobs[:plain_n]:
-            d_r, _, _ = sv.download(i)
-            errs_r.append(depth_errors(d_r, surv.views[i].depth_gt))
-            sv.depth_map(i, nb, rng[0], rng[1], prm_plain)
-            d_u, _, _ = sv.download(i)
-            errs_u.append(depth_errors(d_u, surv.views[i].depth_gt))
-        ctx.synchronize()
-        plain["depth_error_unrefined"] = {k: float(np.mean([e[k] for e in errs_u]))
-                                          for k in errs_u[0]}
-        plain["depth_error_refined"] = {k: float(np.mean([e[k] for e in errs_r]))
-                                        for k in errs_r[0]}
-        log(f"refinement on {plain_n} views: {plain['depth_error_unrefined']} -> "
-            f"{plain['depth_error_refined']}")
     ctx.reset_stats()
 
     ctx.synchronize()
@@ -204,38 +216,46 @@ def main():
     avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
     evals_per_launch = st["sweep_evals"] / max(1, st["sweep_launches"])
     achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
-    traffic = (READ_BYTES_PER_EVAL + WRITE_BYTES_PER_EVAL) * evals_per_launch
+    rd, wr = BYTES_PER_EVAL[args.mode]
+    traffic = (rd + wr) * evals_per_launch if rd is not None else None
     roofline = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
                     traffic_note=("HBM-side bytes per launch: rocprofv3 FETCH_SIZE/WRITE_SIZE per "
                                   "evaluation (copy-calibrated) x evaluations per launch"),
-                    hbm_gbs=traffic / (avg_ms * 1e-3) / 1e9,
-                    kernel=f"k_pm<7,8,{st['split']},pinhole,sweep>", avg_launch_ms=avg_ms,
-                    evals_per_launch=evals_per_launch, flops_per_eval=fpe,
+                    hbm_gbs=(traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                    kernel=(f"k_pm<{spec.taps},8,{st['split']},pinhole,sweep,persist,"
+                            f"{'fast' if args.mode == 'fast' else 'exact'}>"),
+                    avg_launch_ms=avg_ms, evals_per_launch=evals_per_launch, flops_per_eval=fpe,
                     launches=st["sweep_launches"])
 
     # depth error against the FP64 ground truth, all timed views: every valid
     # pixel, and the pixels the consistency filter keeps
     errs, errs_c = [], []
+    maps = {}
     for (i, _, _) in timed:
         d, _, _ = sv.download(i)
+        maps[i] = d
         errs.append(depth_errors(d, surv.views[i].depth_gt))
         if cons_nbrs[i]:
             agree, _ = sv.consistency(i, cons_nbrs[i], rule=capi.MVS_CONS_REPROJ)
             errs_c.append(depth_errors(np.where(agree >= 3, d, 0.0), surv.views[i].depth_gt))
-    err = {k: float(np.mean([e.get(k, 0.0) for e in errs])) for k in errs[0]}
+    err = mean_errors(errs)
     if errs_c:
-        err["consistent"] = {k: float(np.mean([e.get(k, 0.0) for e in errs_c]))
-                             for k in errs_c[0]}
+        err["consistent"] = mean_errors(errs_c)
     err["consistent_pixels"] = int(n_cons)
     err["consistency_rule"] = "reprojection <= 1 px in >= 3 of the K+1 nearest views"
     err["reference"] = "FP64 ray-cast ground truth of the synthetic scene"
+    common = [i for i in plain_maps if i in maps]
+    if common:
+        # the batched refined maps against the plain exact ones, same views
+        err["vs_plain_exact_within_0p5"] = float(np.mean([within(maps[i], plain_maps[i])
+                                                          for i in common]))
 
     cpu = None
     if not args.no_cpu:
         cw, ch = (int(a) for a in args.cpu_crop.split("x"))
         log("CPU baseline (oracle) ...")
-        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm_plain)
+        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm_exact, prm_fast0)
         log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
 
     line = dict(
@@ -249,16 +269,19 @@ def main():
                     views=len(surv.views), width=spec.width, height=spec.height,
                     neighbours=spec.neighbours, window=f"{spec.taps}x{spec.taps}/stride{spec.stride}",
                     iterations=spec.iterations, timed_views=steps,
-                    refine_steps=args.refine_steps,
-                    path=("batched GPU (one persistent launch per half-sweep for all views) + "
+                    refine_steps=args.refine_steps, mode=args.mode,
+                    path=("batched GPU (one persistent launch per half-sweep for all views, "
+                          f"cached graphs), {args.mode} arithmetic, "
                           f"{args.refine_steps} Gauss-Newton refinement steps"),
-                    schedule=[dict(iteration=t["iteration"], split=t["split"], gpc=t["gpc"])
+                    schedule=[dict(iteration=t["iteration"], mode=t["mode"], split=t["split"],
+                                   gpc=t["gpc"], per_cu=t["per_cu"])
                               for t in ctx.tuned_schedules()]),
         roofline=roofline, cpu_baseline=cpu, depth_error=err, plain_gpu=plain,
         gpu_vs_cpu=(value / cpu["value"]) if cpu else None,
         evals_per_pixel=st["evals"] / max(1, pixels),
         refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
+        graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
     )
     print(json.dumps(line), flush=True)
     sv.close()

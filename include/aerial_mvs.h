@@ -76,8 +76,14 @@ typedef struct {
     int32_t refine_steps;    /* Gauss-Newton refinement steps after PatchMatch
                               * (0 = none, the reference's behaviour; no
                               * reference counterpart) */
-    int32_t reserved[2];     /* zero */
+    int32_t mode;            /* MVS_MODE_EXACT (0): PatchMatch rounds operation for
+                              * operation like the reference (bit-identical maps);
+                              * MVS_MODE_FAST (1): contracted multiply-adds and
+                              * hardware reciprocals, costs within 1e-5 of exact */
+    int32_t reserved[1];     /* zero */
 } mvs_params;
+
+enum { MVS_MODE_EXACT = 0, MVS_MODE_FAST = 1 };
 
 void mvs_default_params(mvs_params *p);
 
@@ -221,9 +227,9 @@ int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t per_cu,
  * half-sweeps and calls with the same launch arguments. 0 launches directly
  * and frees the cached graphs. */
 int mvs_set_graphs(mvs_ctx *ctx, int32_t enable);
-/* The autotuner's choices so far, 7 int32 per entry: taps, source lanes,
- * pinhole, log2 pixels, iteration, split, gpc. Returns the entry count (writes
- * at most cap entries). */
+/* The autotuner's choices so far, 9 int32 per entry: taps, source lanes,
+ * pinhole, log2 pixels, iteration, mode, split, gpc, per_cu. Returns the entry
+ * count (writes at most cap entries). */
 int32_t mvs_tuned_schedules(mvs_ctx *ctx, int32_t *out, int32_t cap);
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
 int mvs_reset_stats(mvs_ctx *ctx);

@@ -1,0 +1,133 @@
+"""The fast-arithmetic PatchMatch kernels (MVS_MODE_FAST) against the oracle.
+
+Bars (the float tolerance of the exact path's contract): NCC costs within
+1e-5 absolute of the oracle wherever both are defined, definedness equal
+except within rounding of an image edge (<= 0.2% of items); full depth maps
+within 0.5% relative depth of the oracle's on >= 99% of the pixels valid in
+both; the same inputs and schedule give the same maps (determinism), and a
+survey batch equals one-view calls.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from aerial_mvs import capi
+
+import oracle as orc
+from conftest import survey_cached
+
+pytestmark = pytest.mark.gpu
+
+COST_TOL = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _hyps(v, n, rng, near=0.6):
+    xs = rng.integers(0, v.w, n).astype(np.int32)
+    ys = rng.integers(0, v.h, n).astype(np.int32)
+    hy = np.zeros((n, 4), np.float32)
+    close = rng.uniform(size=n) < near
+    hy[:, 0] = np.where(close, v.depth_gt[ys, xs] * rng.uniform(0.98, 1.02, n),
+                        rng.uniform(30, 90, n))
+    nrm = rng.normal([0, 0, -1], np.where(close, 0.05, 0.4)[:, None], (n, 3))
+    hy[:, 1:] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    return xs, ys, hy
+
+
+@pytest.mark.parametrize("name,taps,stride,k,split", [
+    ("tiny", 5, 2, 4, 1), ("small", 7, 1, 8, 1), ("small", 7, 1, 8, 2), ("small", 7, 1, 4, 4),
+    ("small", 11, 1, 8, 2), ("small", 11, 1, 4, 4)])
+def test_fast_costs_within_tolerance(gpu_ctx, name, taps, stride, k, split):
+    s = survey_cached(name)
+    ref = s.reference_index()
+    nb = s.neighbours[ref][:k]
+    v = s.views[ref]
+    srcs = [s.views[j] for j in nb]
+    rng = np.random.default_rng(300 + taps + k + split)
+    xs, ys, hy = _hyps(v, 4000, rng)
+    o = orc.OracleDepth(v, srcs, *s.spec.depth_range, taps=taps, stride=stride)
+    want = o.cost(xs, ys, hy)
+    gpu_ctx.set_schedule(split, 1, 16, False)
+    try:
+        got = gpu_ctx.ncc_cost(v, srcs, *s.spec.depth_range,
+                               capi.params(taps=taps, stride=stride, fast=True), xs, ys, hy)
+    finally:
+        gpu_ctx.set_schedule(0, 0, 16, True)
+    dw, dg = want < 2.0, got < 2.0
+    mismatch = np.mean(dw != dg)
+    both = dw & dg
+    diff = np.abs(got[both].astype(np.float64) - want[both])
+    _say(f"fast {name} {taps}x{taps} K={k} split={split}: defined {dw.mean():.3f} "
+         f"mismatch {mismatch:.5f} max|diff| {diff.max():.3g} median {np.median(diff):.3g}")
+    assert mismatch <= 0.002
+    assert diff.max() <= COST_TOL
+
+
+@pytest.mark.parametrize("name,k,iters", [("tiny", 4, 3), ("small", 4, 4), ("small", 8, 4)])
+def test_fast_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
+    s = survey_cached(name)
+    spec = s.spec
+    ref = s.reference_index()
+    nb = s.neighbours[ref][:k]
+    v = s.views[ref]
+    srcs = [s.views[j] for j in nb]
+    taps, stride = spec.taps, spec.stride
+    o = orc.OracleDepth(v, srcs, *spec.depth_range, taps=taps, stride=stride)
+    od, _, _, _ = o.depth_map(iterations=iters)
+    prm = capi.params(taps=taps, stride=stride, iterations=iters, fast=True)
+    _say(f"fast depth map {name} K={k} I={iters}")
+    d1, n1, c1 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+    d2, n2, c2 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+    assert np.array_equal(d1, d2) and np.array_equal(c1, c2)  # deterministic
+    both = (d1 > 0) & (od > 0)
+    rel = np.abs(d1[both] - od[both]) / od[both]
+    within = np.mean(rel <= 0.005)
+    valid_agree = np.mean((d1 > 0) == (od > 0))
+    _say(f"  valid both {both.mean():.3f}, valid agree {valid_agree:.4f}, "
+         f"within 0.5% {within:.5f}, identical {np.mean(d1[both] == od[both]):.3f}")
+    assert both.sum() > 1000
+    assert within >= 0.99
+    assert valid_agree >= 0.98
+
+
+def test_fast_survey_batch_matches_single_view(gpu_ctx, tiny_survey):
+    s = tiny_survey
+    prm = capi.params(taps=5, stride=2, iterations=3, fast=True, refine_steps=2)
+    refs = [0, 4, 7]
+    jobs = [(r, list(s.neighbours[r][:4]), s.spec.depth_range) for r in refs]
+    sv = capi.Survey(gpu_ctx, s.views)
+    try:
+        sv.depth_maps(jobs, prm)
+        for (r, nb, rng) in jobs:
+            db, _, _ = sv.download(r)
+            d1, _, _ = gpu_ctx.depth_map(s.views[r], [s.views[j] for j in nb], rng[0], rng[1],
+                                         prm)
+            assert np.array_equal(db, d1)
+    finally:
+        sv.close()
+
+
+def test_fast_mode_with_f32_input_runs_exact_kernels(gpu_ctx, tiny_survey):
+    """The fast kernels read the corner-packed 8-bit planes; views given as f32
+    grey run the exact kernels instead, so their costs equal the oracle's."""
+    import copy
+    s = tiny_survey
+    ref = s.reference_index()
+    lut = orc.gray_lut()
+    rng = np.random.default_rng(9)
+    views = []
+    for v in s.views:
+        v2 = copy.copy(v)
+        v2.gray_f32 = (lut[v.gray] * 0.9 + rng.uniform(0, 0.05, v.gray.shape)).astype(np.float32)
+        views.append(v2)
+    nb = s.neighbours[ref][:4]
+    xs, ys, hy = _hyps(views[ref], 2000, rng)
+    got = gpu_ctx.ncc_cost(views[ref], [views[j] for j in nb], 40, 80, capi.params(fast=True),
+                           xs, ys, hy)
+    want = orc.OracleDepth(views[ref], [views[j] for j in nb], 40, 80).cost(xs, ys, hy)
+    assert np.array_equal(got, want)
