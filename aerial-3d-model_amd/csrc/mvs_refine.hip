@@ -85,7 +85,6 @@ __device__ Terms<NDIR> source_terms(const RefJob &job, const SrcCam &S, const Pi
     const float ch[3] = {pr.rc0, pr.rc1, 1.0f};
     const ff FF{S.f, S.f_lo}, FCX{S.cx, S.cx_lo}, FCY{S.cy, S.cy_lo}, FK1{S.k1, S.k1_lo},
         FK2{S.k2, S.k2_lo};
-    gfloat *G = as_global(S.gray);
     const int sw = S.w;
     const float wm1 = (float)(S.w - 1), hm1 = (float)(S.h - 1);
     float s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
@@ -102,17 +101,19 @@ __device__ Terms<NDIR> source_terms(const RefJob &job, const SrcCam &S, const Pi
         const float rk[3] = {ray.x, ray.y, 1.0f};
         const float den = n[0] * rk[0] + n[1] * rk[1] + n[2];
         if (!(fabsf(den) >= 1e-6f)) return T;
-        const float dq = cc / den;
+        // hardware reciprocals: the refinement has no reference rounding to keep
+        const float rden = __builtin_amdgcn_rcpf(den);
+        const float dq = cc * rden;
         if (!(dq > 0.0f)) return T;
+        // w = R r (the ray in the source frame); the window point is dq w + t
         float xs[3], wv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            xs[i] = S.R[i * 3] * (dq * rk[0]) + S.R[i * 3 + 1] * (dq * rk[1]) +
-                    S.R[i * 3 + 2] * dq + S.t[i];
-            wv[i] = S.R[i * 3] * rk[0] + S.R[i * 3 + 1] * rk[1] + S.R[i * 3 + 2];
+            wv[i] = fmaf(S.R[i * 3], rk[0], fmaf(S.R[i * 3 + 1], rk[1], S.R[i * 3 + 2]));
+            xs[i] = fmaf(dq, wv[i], S.t[i]);
         }
         if (!(xs[2] > 0.0f)) return T;
-        const float inv = 1.0f / xs[2];
+        const float inv = __builtin_amdgcn_rcpf(xs[2]);
         const float q0 = xs[0] * inv, q1 = xs[1] * inv;
         float u, v;
         if (PINHOLE) {
@@ -125,13 +126,14 @@ __device__ Terms<NDIR> source_terms(const RefJob &job, const SrcCam &S, const Pi
         if (!(fx >= 0.0f && fy >= 0.0f && fx < wm1 && fy < hm1)) return T;
         const int x0 = (int)fx, y0 = (int)fy;
         const float ax = fx - (float)x0, ay = fy - (float)y0;
-        gfloat *gp = G + (y0 * sw + x0);
-        const float g00 = gp[0], g10 = gp[1], g01 = gp[sw], g11 = gp[sw + 1];
-        const float top = g00 * (1.0f - ax) + g10 * ax, bot = g01 * (1.0f - ax) + g11 * ax;
-        const float sv = top * (1.0f - ay) + bot * ay - pr.mr;  // shifted sample
+        // the four corners in one 16-byte load from the corner-packed plane
+        const f32x4 qv = as_global(S.quadf)[y0 * sw + x0];
+        const float g00 = qv.x, g10 = qv.y, g01 = qv.z, g11 = qv.w;
+        const float top = fmaf(ax, g10 - g00, g00), bot = fmaf(ax, g11 - g01, g01);
+        const float sv = fmaf(ay, bot - top, top) - pr.mr;  // shifted sample
         s1 += sv;
-        s2 += sv * sv;
-        sa += a * sv;
+        s2 = fmaf(sv, sv, s2);
+        sa = fmaf(a, sv, sa);
         if constexpr (NDIR > 0) {
             // chain: d s / d dq = grad I . d(u,v)/d(q0,q1) . d(q0,q1)/d dq
             float ux, uy, vx, vy;
@@ -147,32 +149,32 @@ __device__ Terms<NDIR> source_terms(const RefJob &job, const SrcCam &S, const Pi
                 vx = S.f * q1 * dD * q0;
                 vy = S.f * (D + q1 * dD * q1);
             }
-            const float Iu = (1.0f - ay) * (g10 - g00) + ay * (g11 - g01);
-            const float Iv = (1.0f - ax) * (g01 - g00) + ax * (g11 - g10);
+            const float Iu = fmaf(ay, (g11 - g01) - (g10 - g00), g10 - g00);
+            const float Iv = bot - top;
             const float dq0 = (wv[0] - q0 * wv[2]) * inv, dq1 = (wv[1] - q1 * wv[2]) * inv;
             const float dI = Iu * (ux * dq0 + uy * dq1) + Iv * (vx * dq0 + vy * dq1);
             // d dq / d theta: depth, then the normal directions
             float Dk[ND];
-            Dk[0] = dI * (nc / den);
+            Dk[0] = dI * (nc * rden);
 #pragma unroll
             for (int q = 1; q < NDIR; ++q) {
                 float g = 0.0f;
 #pragma unroll
                 for (int c3 = 0; c3 < 3; ++c3) g += (d * ch[c3] - dq * rk[c3]) * dirs[q - 1][c3];
-                Dk[q] = dI * (g / den);
+                Dk[q] = dI * (g * rden);
             }
 #pragma unroll
             for (int q = 0; q < NDIR; ++q) {
                 sD[q] += Dk[q];
-                sbD[q] += sv * Dk[q];
-                saD[q] += a * Dk[q];
+                sbD[q] = fmaf(sv, Dk[q], sbD[q]);
+                saD[q] = fmaf(a, Dk[q], saD[q]);
             }
             if constexpr (WANT_DD) {
                 int m = 0;
 #pragma unroll
                 for (int p = 0; p < NDIR; ++p)
 #pragma unroll
-                    for (int q = p; q < NDIR; ++q) sDD[m++] += Dk[p] * Dk[q];
+                    for (int q = p; q < NDIR; ++q, ++m) sDD[m] = fmaf(Dk[p], Dk[q], sDD[m]);
             }
         }
     }

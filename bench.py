@@ -36,9 +36,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE over the 12 half-sweeps of an 8-view batch of this
 # workload, scaled by a 1 GiB device copy (read counter x2.000, write x1.000);
-# profiles/k_pm_sweep_8x2000x1500_k8_7x7_counters.md holds the per-launch
-# figures, per arithmetic mode.
-BYTES_PER_EVAL = {"exact": (718.9, 7.42), "fast": (None, None)}
+# profiles/k_pm_sweep_{fast,exact}_8x2000x1500_k8_7x7_counters.md hold the
+# per-launch figures of each arithmetic mode.
+BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (650.1, 5.75)}
 
 
 def log(*a):
