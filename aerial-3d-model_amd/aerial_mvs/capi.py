@@ -30,6 +30,7 @@ EXPORTS = [
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
     "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair", "mvs_set_graphs",
+    "mvs_survey_depth_maps_ex",
 ]
 
 
@@ -104,6 +105,7 @@ def lib():
         L.mvs_survey_depth_map.argtypes = [vp, i32, vp, i32, f64, f64, P(MvsParams)]
         L.mvs_survey_depth_maps.argtypes = [vp, i32, vp, vp, vp, vp, P(MvsParams), vp,
                                             PROGRESS_FN, vp]
+        L.mvs_survey_depth_maps_ex.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, PROGRESS_FN, vp]
         L.mvs_survey_download.argtypes = [vp, i32, vp, vp, vp]
         L.mvs_survey_consistency.argtypes = [vp, i32, vp, i32, i32, vp, P(C.c_int64)]
         L.mvs_set_timing.argtypes = [vp, i32]
@@ -360,7 +362,8 @@ class Survey:
                               (MVS_OK, MVS_NO_NEIGHBOURS))
 
     def depth_maps(self, jobs, prm, progress=None):
-        """jobs: list of (ref, [srcs], (range0, range1))."""
+        """jobs: list of (ref, [srcs], (range0, range1)). prm: one MvsParams
+        for all jobs, or a list with one per job."""
         refs = np.array([j[0] for j in jobs], np.int32)
         off = np.zeros(len(jobs) + 1, np.int32)
         for i, j in enumerate(jobs):
@@ -368,9 +371,17 @@ class Survey:
         srcs = np.array([s for j in jobs for s in j[1]] or [0], np.int32)
         rng = np.array([x for j in jobs for x in j[2]], np.float64)
         cb = PROGRESS_FN(progress) if progress else PROGRESS_FN()
-        self.ctx.check(lib().mvs_survey_depth_maps(self._h, len(jobs), refs.ctypes.data,
-                                                   off.ctypes.data, srcs.ctypes.data,
-                                                   rng.ctypes.data, C.byref(prm), None, cb, None))
+        if isinstance(prm, (list, tuple)):
+            arr = (MvsParams * max(1, len(prm)))(*prm)
+            self.ctx.check(lib().mvs_survey_depth_maps_ex(self._h, len(jobs), refs.ctypes.data,
+                                                          off.ctypes.data, srcs.ctypes.data,
+                                                          rng.ctypes.data, C.addressof(arr),
+                                                          None, cb, None))
+        else:
+            self.ctx.check(lib().mvs_survey_depth_maps(self._h, len(jobs), refs.ctypes.data,
+                                                       off.ctypes.data, srcs.ctypes.data,
+                                                       rng.ctypes.data, C.byref(prm), None, cb,
+                                                       None))
 
     def download(self, view):
         h, w = self.shapes[view]

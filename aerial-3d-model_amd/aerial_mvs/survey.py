@@ -229,3 +229,102 @@ def make_survey(spec, nthreads=None, with_depth=True, views=None):
 
 def params_for(spec, api_params):
     return api_params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
+
+
+@dataclass
+class UnevenSpec:
+    """A survey of nadir and oblique views of unequal sizes, with per-view
+    neighbour counts and window sizes: the batched-uneven workload (300 views:
+    220 nadir 1600x1200 over [40, 80] m, 80 oblique 45-degree edge-ring views
+    1920x1080 over [50, 200] m, K uniform in 3..12, windows in {5, 7, 11}),
+    scaled by `scale` for tests."""
+    name: str
+    nadir_lines: int
+    nadir_per_line: int
+    nadir_size: tuple
+    oblique: int
+    oblique_size: tuple
+    scale: float = 1.0
+    altitude: float = 60.0
+    patch: tuple = (300.0, 220.0)
+    k_range: tuple = (3, 12)
+    windows: tuple = (5, 7, 11)
+    nadir_range: tuple = (40.0, 80.0)
+    oblique_range: tuple = (50.0, 200.0)
+    iterations: int = 4
+    seed: int = 0xA3D00003
+
+
+UNEVEN = {
+    "c3": UnevenSpec("c3", 11, 20, (1600, 1200), 80, (1920, 1080)),
+    # the same geometry with sensors and patch scaled down, for tests
+    "uneven_small": UnevenSpec("uneven_small", 3, 5, (320, 240), 6, (384, 216), scale=0.2,
+                               patch=(60.0, 44.0), iterations=3, seed=0xA3D0000C),
+}
+
+
+def make_uneven_survey(spec, nthreads=None):
+    """Render an uneven survey; returns (Survey, per-view dict(k, taps,
+    stride, range)). Nadir views fly a lawnmower block (C1 field of view,
+    75% / 65% overlap); oblique views sit on a ring around the block at the
+    same altitude, pitched 45 degrees down and facing the patch centre."""
+    if isinstance(spec, str):
+        spec = UNEVEN[spec]
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    L = _synth()
+    rng = np.random.default_rng(spec.seed + 1)
+    alt = spec.altitude * spec.scale
+    margin = 3.5 * alt
+    px, py = spec.patch
+    scene = L.synth_scene_create(-margin, -margin, px + 2 * margin, py + 2 * margin,
+                                 0.1 * max(spec.scale, 0.2), 50.0 * spec.scale, 6, 0.5,
+                                 15.0 * spec.scale, spec.seed, 0.2, 5, nthreads)
+    views, cams = [], []
+    try:
+        info = np.zeros(6)
+        L.synth_scene_info(scene, info)
+        ground = 0.5 * (info[0] + info[1])
+        nw, nh = spec.nadir_size
+        fn = 900.0 * nw / 1024
+        fw, fh = nw / fn * alt, nh / fn * alt
+        along, across = 0.25 * fh, 0.35 * fw
+        for r in range(spec.nadir_lines):
+            yaw = 90.0 if r % 2 == 0 else 270.0
+            for cidx in range(spec.nadir_per_line):
+                x = px / 2 + (cidx - (spec.nadir_per_line - 1) / 2) * along
+                y = py / 2 + (r - (spec.nadir_lines - 1) / 2) * across
+                dp = rng.normal(0.0, 0.05, 3)
+                da = rng.normal(0.0, 0.05, 3)
+                cams.append(("nadir", (x + dp[0], y + dp[1], alt + dp[2]), yaw + da[0],
+                             -90.0 + da[1], da[2]))
+        ow, oh = spec.oblique_size
+        ring = 0.5 * np.hypot(px, py) + 0.6 * alt
+        for o in range(spec.oblique):
+            a = 2 * np.pi * o / spec.oblique
+            x, y = px / 2 + ring * np.cos(a), py / 2 + ring * np.sin(a)
+            # heading (sin yaw, cos yaw) points from the camera to the centre
+            yaw = np.degrees(np.arctan2(px / 2 - x, py / 2 - y))
+            cams.append(("oblique", (x, y, alt), yaw, -45.0, 0.0))
+        for vi, (kind, c, yaw, pitch, roll) in enumerate(cams):
+            w, h = (nw, nh) if kind == "nadir" else (ow, oh)
+            f = (900.0 * w / 1024) if kind == "nadir" else (900.0 * w / 1024)
+            R, Cc = from_angles((c[0], c[1], ground + c[2]), yaw, pitch, roll)
+            img = np.zeros((h, w), np.uint8)
+            dep = np.zeros((h, w), np.float64)
+            L.synth_render(scene, w, h, f, w / 2.0, h / 2.0, 0.0, 0.0,
+                           np.ascontiguousarray(R.ravel()), Cc, spec.seed + 1000 + vi, 1.0,
+                           img.ravel(), dep.ctypes.data, nthreads)
+            views.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
+    finally:
+        L.synth_scene_destroy(scene)
+    cent = np.array([v.C for v in views])
+    nb = nearest_neighbours(cent, spec.k_range[1])
+    per = []
+    for vi, (kind, *_rest) in enumerate(cams):
+        k = int(rng.integers(spec.k_range[0], spec.k_range[1] + 1))
+        taps = int(rng.choice(spec.windows))
+        rng_v = spec.nadir_range if kind == "nadir" else spec.oblique_range
+        per.append(dict(kind=kind, k=k, taps=taps, stride=1 if taps > 5 else 2,
+                        range=tuple(r * spec.scale for r in rng_v)))
+    surv = Survey(SPECS["small"], views, nb, [p["range"] for p in per], ground)
+    return surv, per

@@ -76,6 +76,7 @@ struct JobSpec {
     const ViewDev *ref = nullptr;
     std::vector<const ViewDev *> srcs;
     double range0 = 0, range1 = 0;
+    mvs_params prm{};       // window, iterations, thresholds, refinement, mode
     MapDev *out = nullptr;  // thresholded outputs (nullable)
 };
 
@@ -356,6 +357,8 @@ int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
 struct KClass {
     int32_t srcl = 0, pinhole = 1, first = 0, count = 0;
     int32_t fast = 0;                     // MVS_MODE_FAST kernels
+    int32_t taps = 7;                     // window taps per side
+    int32_t steps = 0;                    // half-sweeps of its jobs
     int64_t cap_sweep = 0, cap_eval = 0;  // upper bounds of the list lengths
     std::vector<int64_t> caps;            // pixels of each job
     int32_t size = 0;                     // log2 pixels of the first job
@@ -510,10 +513,11 @@ struct SweepLaunch {
     int32_t grid, srcl, split, pinhole, persist, taps, fast;
 };
 
-int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes, int32_t it,
+int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32_t it,
                    int32_t parity, unsigned long long *evals, std::vector<SweepLaunch> &out) {
     out.clear();
-    for (const KClass &k : classes) {
+    for (const KClass *kp : classes) {
+        const KClass &k = *kp;
         SweepLaunch L;
         memset(&L, 0, sizeof L);
         L.a = base_args(c, k);
@@ -527,7 +531,7 @@ int sweep_launches(mvs_ctx *c, int32_t taps, const std::vector<KClass> &classes,
         L.srcl = k.srcl;
         L.split = k.sch.split;
         L.pinhole = k.pinhole;
-        L.taps = taps;
+        L.taps = k.taps;
         L.fast = k.fast;
         out.push_back(L);
     }
@@ -588,14 +592,21 @@ int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
     return MVS_OK;
 }
 
-int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *prm,
-              int32_t first_step, int32_t n_steps, const volatile int32_t *cancel,
+// Run PatchMatch for a batch of jobs, each with its own parameters. Jobs are
+// grouped into kernel classes (window, half-sweep count, source lanes,
+// projection, arithmetic); every half-sweep is one launch per active class.
+// first_step < 0: initialise first (from step 0); otherwise continue from
+// first_step on the jobs' current state. n_steps < 0: all 2 x iterations
+// half-sweeps of each job, then refinement and thresholded outputs for jobs
+// that have an output map.
+int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
+              int32_t n_steps, const volatile int32_t *cancel,
               std::vector<KClass> *classes_out) {
     const int32_t nj = (int32_t)jobs.size();
     if (nj == 0) return MVS_OK;
-    const int32_t taps = prm->taps, half = (taps / 2) * prm->stride;
     for (const JobSpec &j : jobs) {
         const int32_t K = (int32_t)j.srcs.size();
+        const int32_t half = (j.prm.taps / 2) * j.prm.stride;
         if (K < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
         if (lanes_for(K) < 0)
             return fail(c, MVS_ERR_ARG, "at most 16 source views per reference view");
@@ -610,11 +621,13 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
     HIP_TRY(c, c->prefix.ensure(sizeof(int32_t) * 9 * ((size_t)nj + 1)));
     HIP_TRY(c, c->counters.ensure(sizeof(int32_t) * 8));
 
-    // host job descriptions
+    // host job descriptions and their class keys
+    typedef std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t> ClassKey;
     std::vector<RefJob> hj((size_t)nj);
-    std::vector<int32_t> srcl((size_t)nj), pin((size_t)nj), fst((size_t)nj);
+    std::vector<ClassKey> ckey((size_t)nj);
     for (int32_t i = 0; i < nj; ++i) {
         const JobSpec &js = jobs[(size_t)i];
+        const mvs_params &prm = js.prm;
         JobWork &wk = c->work[(size_t)i];
         const ViewDev &ref = *js.ref;
         const int64_t n = (int64_t)ref.w * ref.h, cap = (n + 1) / 2;
@@ -628,10 +641,10 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         c->host_tables.emplace_back();
         std::vector<SrcCam> &tab = c->host_tables.back();
         bool pinhole = true, quads = true;
-        for (const ViewDev *s : js.srcs) {
-            tab.push_back(make_src(ref, *s));
+        for (const ViewDev *sv : js.srcs) {
+            tab.push_back(make_src(ref, *sv));
             pinhole = pinhole && tab.back().pinhole_exact;
-            quads = quads && s->has_quad;
+            quads = quads && sv->has_quad;
         }
         HIP_TRY(c, hipMemcpyAsync(wk.srcs.p, tab.data(), sizeof(SrcCam) * (size_t)K,
                                   hipMemcpyHostToDevice, c->st));
@@ -660,41 +673,44 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
         job.up0 = (float)ref.R[2];
         job.up1 = (float)ref.R[5];
         job.up2 = (float)ref.R[8];
-        job.texture_min = prm->texture_min;
-        job.cost_max = prm->cost_max;
-        job.stride = prm->stride;
-        job.half = half;
+        job.texture_min = prm.texture_min;
+        job.cost_max = prm.cost_max;
+        job.stride = prm.stride;
+        job.half = (prm.taps / 2) * prm.stride;
         job.pinhole = pinhole ? 1 : 0;
-        srcl[(size_t)i] = lanes_for(K);
-        pin[(size_t)i] = pinhole ? 1 : 0;
         // the fast kernels read the corner-packed 8-bit planes; views given
         // as f32 grey run the exact kernels
-        fst[(size_t)i] = (prm->mode == MVS_MODE_FAST && quads) ? 1 : 0;
+        const int32_t fast = (prm.mode == MVS_MODE_FAST && quads) ? 1 : 0;
+        const int32_t steps = n_steps < 0 ? 2 * prm.iterations : n_steps;
+        ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), pinhole ? 1 : 0, fast);
     }
 
-    // group jobs by kernel class (source lanes, projection)
+    // group jobs by kernel class; jobs of a class are contiguous in the table
+    std::map<ClassKey, std::vector<int32_t>> groups;
+    for (int32_t i = 0; i < nj; ++i) groups[ckey[(size_t)i]].push_back(i);
     std::vector<KClass> classes;
     std::vector<RefJob> table;
-    for (int32_t sl : {4, 8, 16})
-        for (int32_t ph : {1, 0})
-            for (int32_t fa : {1, 0}) {
-            KClass k;
-            k.srcl = sl;
-            k.pinhole = ph;
-            k.fast = fa;
-            k.first = (int32_t)table.size();
-            for (int32_t i = 0; i < nj; ++i)
-                if (srcl[(size_t)i] == sl && pin[(size_t)i] == ph && fst[(size_t)i] == fa) {
-                    table.push_back(hj[(size_t)i]);
-                    const int64_t n = (int64_t)hj[(size_t)i].w * hj[(size_t)i].h;
-                    k.cap_sweep += (n + 1) / 2;
-                    k.cap_eval += n;
-                    k.caps.push_back(n);
-                    if (k.count == 0) k.size = size_bucket(n);
-                    ++k.count;
-                }
-            if (k.count) classes.push_back(k);
+    std::vector<int32_t> order;  // table position -> job index
+    for (const auto &g : groups) {
+        KClass k;
+        k.taps = std::get<0>(g.first);
+        k.steps = std::get<1>(g.first);
+        k.srcl = std::get<2>(g.first);
+        k.pinhole = std::get<3>(g.first);
+        k.fast = std::get<4>(g.first);
+        k.first = (int32_t)table.size();
+        for (int32_t i : g.second) {
+            table.push_back(hj[(size_t)i]);
+            order.push_back(i);
+            const int64_t n = (int64_t)hj[(size_t)i].w * hj[(size_t)i].h;
+            k.cap_sweep += (n + 1) / 2;
+            k.cap_eval += n;
+            k.caps.push_back(n);
+            if (k.count == 0) k.size = size_bucket(n);
+            ++k.count;
         }
+        classes.push_back(k);
+    }
     const size_t tbl0 = c->host_jobs.size();
     c->host_jobs.insert(c->host_jobs.end(), table.begin(), table.end());
     HIP_TRY(c, c->jobs_dev.ensure(sizeof(RefJob) * table.size()));
@@ -712,15 +728,15 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
             HIP_TRY(c, hipEventRecord(ea, c->st));
         }
         for (int32_t i = 0; i < nj; ++i) {
-            launch_init(hj[(size_t)i], taps, first_step < 0 ? 1 : 0, c->st);
+            launch_init(hj[(size_t)i], jobs[(size_t)i].prm.taps, first_step < 0 ? 1 : 0, c->st);
             HIP_TRY(c, hipGetLastError());
         }
         if (first_step < 0) {
             for (KClass &k : classes) {
-                k.sch = candidates(taps, k.srcl)[0];
+                k.sch = candidates(k.taps, k.srcl)[0];
                 PmArgs a = base_args(c, k);
                 a.evals = evals;
-                TRY(pm_launch(c, PM_EVAL_CURRENT, taps, k, a));
+                TRY(pm_launch(c, PM_EVAL_CURRENT, k.taps, k, a));
             }
         }
         if (c->timing) {
@@ -730,27 +746,34 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
     }
     // half-sweeps (dense.rs:413-467); the schedule of each class is forced,
     // cached, or tuned per iteration on the class's first job. A half-sweep
-    // whose schedules are all known runs as a cached graph.
+    // whose schedules are all known runs as a cached graph. Classes with fewer
+    // half-sweeps drop out when theirs are done.
     const int32_t s0 = first_step < 0 ? 0 : first_step;
+    int32_t max_steps = 0;
+    for (const KClass &k : classes) max_steps = std::max(max_steps, k.steps);
     std::vector<SweepLaunch> ls;
-    for (int32_t s = s0; s < s0 + n_steps; ++s) {
+    std::vector<const KClass *> active;
+    for (int32_t s = s0; s < s0 + max_steps; ++s) {
         if (cancel && *cancel) return fail(c, MVS_ERR_CANCELLED, "cancelled");
         const int32_t it = s / 2, parity = s % 2;
         bool known = true;
+        active.clear();
         for (KClass &k : classes) {
+            if (s - s0 >= k.steps) continue;
+            active.push_back(&k);
             if (c->forced.split > 0) {
-                k.sch = forced_schedule(c, taps, k.srcl);
+                k.sch = forced_schedule(c, k.taps, k.srcl);
             } else {
-                TuneKey key(taps, k.srcl, k.pinhole, k.size, it, k.fast);
+                TuneKey key(k.taps, k.srcl, k.pinhole, k.size, it, k.fast);
                 auto f = c->tuned.find(key);
                 if (f != c->tuned.end()) {
                     k.sch = f->second;
                 } else if (c->autotune) {
-                    TRY(autotune_class(c, taps, k, table[(size_t)k.first], it, parity));
+                    TRY(autotune_class(c, k.taps, k, table[(size_t)k.first], it, parity));
                     c->tuned[key] = k.sch;
                     known = false;
                 } else {
-                    k.sch = candidates(taps, k.srcl)[0];
+                    k.sch = candidates(k.taps, k.srcl)[0];
                 }
             }
         }
@@ -760,7 +783,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
             eb = take_event(c);
             HIP_TRY(c, hipEventRecord(ea, c->st));
         }
-        TRY(sweep_launches(c, taps, classes, it, parity, evals + 1, ls));
+        TRY(sweep_launches(c, active, it, parity, evals + 1, ls));
         if (c->graphs && known)
             TRY(run_sweeps_graph(c, ls));
         else
@@ -773,20 +796,23 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, const mvs_params *pr
     for (const JobSpec &js : jobs) c->stats.pixels += (int64_t)js.ref->w * js.ref->h;
     // Gauss-Newton refinement of the accepted pixels (no reference
     // counterpart), before the outputs are thresholded
-    if (prm->refine_steps > 0 && first_step < 0) {
+    if (first_step < 0) {
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (c->timing) {
-            ea = take_event(c);
-            eb = take_event(c);
-            HIP_TRY(c, hipEventRecord(ea, c->st));
-        }
+        bool any = false;
         for (int32_t i = 0; i < nj; ++i) {
-            if (!jobs[(size_t)i].out) continue;
-            if (launch_refine(taps, hj[(size_t)i], prm->refine_steps, evals + 2, c->st))
+            const JobSpec &js = jobs[(size_t)i];
+            if (!js.out || js.prm.refine_steps <= 0) continue;
+            if (!any && c->timing) {
+                ea = take_event(c);
+                eb = take_event(c);
+                HIP_TRY(c, hipEventRecord(ea, c->st));
+            }
+            any = true;
+            if (launch_refine(js.prm.taps, hj[(size_t)i], js.prm.refine_steps, evals + 2, c->st))
                 return fail(c, MVS_ERR_ARG, "no refinement kernel for this window");
             HIP_TRY(c, hipGetLastError());
         }
-        if (c->timing) {
+        if (any && c->timing) {
             HIP_TRY(c, hipEventRecord(eb, c->st));
             c->pending.push_back({ea, eb, 2});
         }
@@ -1022,7 +1048,8 @@ int mvs_depth_map(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t
     jobs[0].range0 = range0;
     jobs[0].range1 = range1;
     jobs[0].out = &c->tmp_map;
-    int rc = run_batch(c, jobs, prm, -1, 2 * prm->iterations, cancel, nullptr);
+    jobs[0].prm = *prm;
+    int rc = run_batch(c, jobs, -1, -1, cancel, nullptr);
     if (rc) return abort_call(c, rc);
     return download_map(c, c->tmp_map, out_depth, out_normal, out_cost);
 }
@@ -1056,8 +1083,9 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     jobs[0].srcs = sv;
     jobs[0].range0 = range0;
     jobs[0].range1 = range1;
+    jobs[0].prm = *prm;
     std::vector<KClass> classes;
-    int rc = run_batch(c, jobs, prm, 0, 0, nullptr, &classes);
+    int rc = run_batch(c, jobs, 0, 0, nullptr, &classes);
     if (rc) return abort_call(c, rc);
     KClass k = classes[0];
     k.sch = c->forced.split > 0 ? forced_schedule(c, prm->taps, k.srcl)
@@ -1117,7 +1145,8 @@ int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     jobs[0].srcs = sv;
     jobs[0].range0 = range0;
     jobs[0].range1 = range1;
-    int rc = run_batch(c, jobs, prm, 0, 0, nullptr, nullptr);
+    jobs[0].prm = *prm;
+    int rc = run_batch(c, jobs, 0, 0, nullptr, nullptr);
     if (rc) return abort_call(c, rc);
     rc = launch_pair_list(prm->taps, c->host_jobs.back(), n, c->px.as<int32_t>(),
                           c->py.as<int32_t>(), c->hyp_in.as<float4>(), c->pairs.as<int32_t>(),
@@ -1168,7 +1197,8 @@ int mvs_patchmatch_steps(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, 
     jobs[0].srcs = sv;
     jobs[0].range0 = range0;
     jobs[0].range1 = range1;
-    int rc = run_batch(c, jobs, prm, first_step, n_steps, nullptr, nullptr);
+    jobs[0].prm = *prm;
+    int rc = run_batch(c, jobs, first_step, n_steps, nullptr, nullptr);
     if (rc) return abort_call(c, rc);
     HIP_TRY(c, hipMemcpyAsync(hyp4.data(), wk.hyp.p, n * 16, hipMemcpyDeviceToHost, c->st));
     HIP_TRY(c, hipMemcpyAsync(cst.data(), wk.cost.p, n * 4, hipMemcpyDeviceToHost, c->st));
@@ -1219,15 +1249,20 @@ int mvs_survey_set_view(mvs_survey *s, int32_t i, const mvs_view *v) {
     return upload_view(c, s->views[(size_t)i], v);
 }
 
-int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
-                          const int32_t *src_off, const int32_t *srcs, const double *ranges,
-                          const mvs_params *prm, const volatile int32_t *cancel,
-                          mvs_progress_fn progress, void *user) {
-    if (!s) return MVS_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// The survey loop of dense.rs:528-540 over resident views: jobs in CSR form,
+// params per job (prm_step 1) or shared (prm_step 0), batched by memory budget.
+int survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs, const int32_t *src_off,
+                      const int32_t *srcs, const double *ranges, const mvs_params *prms,
+                      int32_t prm_step, const volatile int32_t *cancel, mvs_progress_fn progress,
+                      void *user) {
     mvs_ctx *c = s->ctx;
-    TRY(check_params(c, prm));
-    if (njobs < 0 || (njobs > 0 && (!refs || !src_off || !ranges)))
+    if (njobs < 0 || (njobs > 0 && (!refs || !src_off || !ranges || !prms)))
         return fail(c, MVS_ERR_ARG, "bad job arrays");
+    for (int32_t j = 0; j < (prm_step ? njobs : 1); ++j) TRY(check_params(c, &prms[j]));
     HIP_TRY(c, hipSetDevice(c->device));
     const int32_t nv = (int32_t)s->views.size();
     // validate; views without neighbours get no map (the reference's Ok(None))
@@ -1246,6 +1281,7 @@ int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
         }
         js.range0 = ranges[2 * j];
         js.range1 = ranges[2 * j + 1];
+        js.prm = prms[prm_step ? j : 0];
         js.out = &s->maps[(size_t)ref];
         if (js.srcs.empty()) {
             s->maps[(size_t)ref].valid = false;
@@ -1266,7 +1302,7 @@ int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
             bytes += slot_bytes(*all[i].ref);
             batch.push_back(all[i++]);
         }
-        int rc = run_batch(c, batch, prm, -1, 2 * prm->iterations, cancel, nullptr);
+        int rc = run_batch(c, batch, -1, -1, cancel, nullptr);
         if (rc) return abort_call(c, rc);
         if (progress) {
             TRY(sync_ctx(c));
@@ -1279,6 +1315,28 @@ int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
         }
     }
     return sync_ctx(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvs_survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs,
+                          const int32_t *src_off, const int32_t *srcs, const double *ranges,
+                          const mvs_params *prm, const volatile int32_t *cancel,
+                          mvs_progress_fn progress, void *user) {
+    if (!s) return MVS_ERR_ARG;
+    return survey_depth_maps(s, njobs, refs, src_off, srcs, ranges, prm, 0, cancel, progress,
+                             user);
+}
+
+int mvs_survey_depth_maps_ex(mvs_survey *s, int32_t njobs, const int32_t *refs,
+                             const int32_t *src_off, const int32_t *srcs, const double *ranges,
+                             const mvs_params *prms, const volatile int32_t *cancel,
+                             mvs_progress_fn progress, void *user) {
+    if (!s) return MVS_ERR_ARG;
+    return survey_depth_maps(s, njobs, refs, src_off, srcs, ranges, prms, 1, cancel, progress,
+                             user);
 }
 
 int mvs_survey_depth_map(mvs_survey *s, int32_t ref, const int32_t *srcs, int32_t nsrc,

@@ -176,6 +176,17 @@ int mvs_survey_depth_maps(mvs_survey *sv, int32_t njobs, const int32_t *refs,
                           const mvs_params *prm, const volatile int32_t *cancel,
                           mvs_progress_fn progress, void *user);
 
+/* As mvs_survey_depth_maps with parameters per job (prms[j]): window size,
+ * stride, iterations, thresholds, refinement and mode may differ between the
+ * views of one batch (no reference counterpart; the reference runs one
+ * DenseParams per survey). Jobs are grouped into kernel classes (window,
+ * iterations, neighbour-count bucket, projection, mode); each half-sweep is
+ * one launch per class over all its views. */
+int mvs_survey_depth_maps_ex(mvs_survey *sv, int32_t njobs, const int32_t *refs,
+                             const int32_t *src_off, const int32_t *srcs, const double *ranges,
+                             const mvs_params *prms, const volatile int32_t *cancel,
+                             mvs_progress_fn progress, void *user);
+
 int mvs_survey_download(mvs_survey *sv, int32_t view, float *depth, float *normal, float *cost);
 
 /* Geometric consistency of view ref's depth map against its neighbours' maps
