@@ -30,7 +30,7 @@ EXPORTS = [
     "mvs_survey_download", "mvs_survey_consistency", "mvs_set_timing", "mvs_get_stats",
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
     "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair", "mvs_set_graphs",
-    "mvs_survey_depth_maps_ex",
+    "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
 ]
 
 
@@ -118,6 +118,9 @@ def lib():
         L.mvs_selftest.argtypes = [vp, i32, i64, C.c_uint64, P(C.c_int64)]
         L.mvs_save_depth.argtypes = [C.c_char_p, i32, i32, vp]
         L.mvs_load_depth.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, i64]
+        L.mvs_save_view.argtypes = [C.c_char_p, i32, i32, vp, vp]
+        L.mvs_load_view.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, vp, i64]
+        L.mvs_view_intrinsics.argtypes = [f64, f64, f64, i32, i32, P(C.c_double)]
         _lib = L
     return _lib
 
@@ -191,6 +194,38 @@ def load_depth(path):
     d = np.zeros((h.value, w.value), np.float32)
     rc = lib().mvs_load_depth(os.fsencode(path), C.byref(w), C.byref(h), d.ctypes.data, d.size)
     return None if rc else d
+
+
+def save_view(path, gray, rgb):
+    """A `.view` cache file: (h, w) f32 grey and (h, w, 3) u8 RGB."""
+    g = np.ascontiguousarray(gray, np.float32)
+    c = np.ascontiguousarray(rgb, np.uint8)
+    assert c.shape == g.shape + (3,)
+    rc = lib().mvs_save_view(os.fsencode(path), g.shape[1], g.shape[0], g.ctypes.data,
+                             c.ctypes.data)
+    if rc:
+        raise MvsError(rc, "save_view failed")
+
+
+def load_view(path):
+    """(grey (h, w) f32, rgb (h, w, 3) u8), or None when missing or damaged."""
+    w, h = C.c_int32(), C.c_int32()
+    if lib().mvs_load_view(os.fsencode(path), C.byref(w), C.byref(h), None, None, 0):
+        return None
+    g = np.zeros((h.value, w.value), np.float32)
+    c = np.zeros((h.value, w.value, 3), np.uint8)
+    if lib().mvs_load_view(os.fsencode(path), C.byref(w), C.byref(h), g.ctypes.data,
+                           c.ctypes.data, g.size):
+        return None
+    return g, c
+
+
+def view_intrinsics(f, cx, cy, full_width, w):
+    out = (C.c_double * 3)()
+    rc = lib().mvs_view_intrinsics(f, cx, cy, full_width, w, out)
+    if rc:
+        raise MvsError(rc, "bad intrinsics")
+    return tuple(out)
 
 
 class Context:

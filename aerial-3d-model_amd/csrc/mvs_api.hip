@@ -1513,4 +1513,79 @@ int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64
     return MVS_OK;
 }
 
+int mvs_save_view(const char *path, int32_t w, int32_t h, const float *gray, const uint8_t *rgb) {
+    if (!path || w < 0 || h < 0 || ((int64_t)w * h > 0 && (!gray || !rgb))) return MVS_ERR_ARG;
+    const size_t n = (size_t)w * h;
+    // prepare_views (dense.rs:101-113): header, grey, RGB, then tmp + rename
+    std::string tmp = std::string(path);
+    const size_t slash = tmp.find_last_of('/');
+    const size_t dot = tmp.find_last_of('.');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash))
+        tmp = tmp.substr(0, dot);
+    tmp += ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return MVS_ERR_IO;
+    const uint32_t hdr[2] = {(uint32_t)w, (uint32_t)h};  // little-endian hosts only
+    bool ok = fwrite(hdr, 4, 2, f) == 2;
+    ok = ok && fwrite(gray, 4, n, f) == n;
+    ok = ok && fwrite(rgb, 1, 3 * n, f) == 3 * n;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path) != 0) return MVS_ERR_IO;
+    return MVS_OK;
+}
+
+int mvs_load_view(const char *path, int32_t *w, int32_t *h, float *gray, uint8_t *rgb,
+                  int64_t cap) {
+    if (!path || !w || !h) return MVS_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return MVS_ERR_IO;
+    uint32_t hdr[2];
+    if (fread(hdr, 4, 2, f) != 2) {
+        fclose(f);
+        return MVS_ERR_FORMAT;
+    }
+    fseek(f, 0, SEEK_END);
+    const long len = ftell(f);
+    const int64_t n = (int64_t)hdr[0] * hdr[1];
+    // load_view (dense.rs:130-132): "damaged cache file" unless 8 + 7 w h bytes
+    if (len != 8 + 7 * n) {
+        fclose(f);
+        return MVS_ERR_FORMAT;
+    }
+    *w = (int32_t)hdr[0];
+    *h = (int32_t)hdr[1];
+    if (gray || rgb) {
+        if (cap < n) {
+            fclose(f);
+            return MVS_ERR_ARG;
+        }
+        bool ok = true;
+        if (gray) {
+            fseek(f, 8, SEEK_SET);
+            ok = fread(gray, 4, (size_t)n, f) == (size_t)n;
+        }
+        if (ok && rgb) {
+            fseek(f, 8 + 4 * n, SEEK_SET);
+            ok = fread(rgb, 1, 3 * (size_t)n, f) == 3 * (size_t)n;
+        }
+        if (!ok) {
+            fclose(f);
+            return MVS_ERR_IO;
+        }
+    }
+    fclose(f);
+    return MVS_OK;
+}
+
+int mvs_view_intrinsics(double f, double cx, double cy, int32_t full_width, int32_t w,
+                        double out[3]) {
+    if (!out || full_width <= 0 || w <= 0) return MVS_ERR_ARG;
+    // load_view (dense.rs:139-145): K scaled by s = w / full.width; k1, k2 kept
+    const double s = (double)w / (double)full_width;
+    out[0] = f * s;
+    out[1] = cx * s;
+    out[2] = cy * s;
+    return MVS_OK;
+}
+
 }  // extern "C"

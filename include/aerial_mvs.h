@@ -260,6 +260,20 @@ int mvs_save_depth(const char *path, int32_t w, int32_t h, const float *depth);
  * None (wrong length). With depth == NULL only w and h are read. */
 int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64_t cap);
 
+/* ---- `.view` cache files (dense.rs:92-154) ---- */
+/* Writes u32 LE w, u32 LE h, w*h f32 LE grey, w*h*3 u8 RGB to path's ".tmp"
+ * sibling, then renames (prepare_views, dense.rs:101-113). */
+int mvs_save_view(const char *path, int32_t w, int32_t h, const float *gray, const uint8_t *rgb);
+/* Reads a `.view` file (load_view, dense.rs:123-137): MVS_ERR_FORMAT unless the
+ * length is exactly 8 + 7 w h ("damaged cache file"). gray (w*h) and rgb
+ * (3*w*h) are nullable; with both NULL only w and h are read. cap = pixels. */
+int mvs_load_view(const char *path, int32_t *w, int32_t *h, float *gray, uint8_t *rgb,
+                  int64_t cap);
+/* The working-size intrinsics of a view (dense.rs:139-145): f, cx, cy scaled
+ * by w / full_width (k1, k2 unchanged). out = {f, cx, cy}. */
+int mvs_view_intrinsics(double f, double cx, double cy, int32_t full_width, int32_t w,
+                        double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

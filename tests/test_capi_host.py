@@ -88,6 +88,31 @@ def test_depth_file_round_trip(tmp_path):
     assert capi.load_depth(str(tmp_path / "missing.depth")) is None
 
 
+def test_view_file_round_trip(tmp_path):
+    """`.view` cache (dense.rs:92-154): u32 w, u32 h, w*h f32 grey, w*h*3 RGB;
+    any other length is a damaged file."""
+    rng = np.random.default_rng(2)
+    rgb = rng.integers(0, 256, (5, 9, 3)).astype(np.uint8)
+    f = rgb.astype(np.float32)
+    gray = ((np.float32(0.299) * f[..., 0] + np.float32(0.587) * f[..., 1])
+            + np.float32(0.114) * f[..., 2]) / np.float32(255.0)
+    p = str(tmp_path / "IMG_0001_960.view")
+    capi.save_view(p, gray, rgb)
+    assert not os.path.exists(str(tmp_path / "IMG_0001_960.tmp"))
+    raw = open(p, "rb").read()
+    assert len(raw) == 8 + 7 * gray.size
+    assert np.frombuffer(raw[:8], "<u4").tolist() == [9, 5]
+    assert np.array_equal(np.frombuffer(raw[8:8 + 4 * gray.size], "<f4").reshape(5, 9), gray)
+    assert raw[8 + 4 * gray.size:] == rgb.tobytes()
+    g2, c2 = capi.load_view(p)
+    assert np.array_equal(g2, gray) and np.array_equal(c2, rgb)
+    open(p, "wb").write(raw + b"\0")
+    assert capi.load_view(p) is None
+    assert capi.load_view(str(tmp_path / "missing.view")) is None
+    # working-size intrinsics: K scaled by w / full width (dense.rs:139-145)
+    assert capi.view_intrinsics(3000.0, 2000.0, 1500.0, 4000, 960) == (720.0, 480.0, 360.0)
+
+
 def test_no_device_fails_loudly():
     """Without a gfx950 device there is no fallback: creation fails with
     MVS_ERR_DEVICE and a message."""
