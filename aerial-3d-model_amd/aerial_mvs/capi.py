@@ -31,6 +31,8 @@ EXPORTS = [
     "mvs_reset_stats", "mvs_save_depth", "mvs_load_depth", "mvs_selftest",
     "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair", "mvs_set_graphs",
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
+    "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
+    "mvs_write_ply", "mvs_read_ply",
 ]
 
 
@@ -121,6 +123,13 @@ def lib():
         L.mvs_save_view.argtypes = [C.c_char_p, i32, i32, vp, vp]
         L.mvs_load_view.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, vp, i64]
         L.mvs_view_intrinsics.argtypes = [f64, f64, f64, i32, i32, P(C.c_double)]
+        L.mvs_survey_set_colour.argtypes = [vp, i32, vp]
+        L.mvs_survey_fuse.argtypes = [vp, i32, vp, vp, vp, i32, f64, P(C.c_int64)]
+        L.mvs_survey_fused_points.argtypes = [vp, vp, vp, i64, P(C.c_int64)]
+        L.mvs_fusion_voxel.argtypes = [vp, vp, i64, vp, i32, f64, i32, i32, i32, i64,
+                                       P(C.c_double)]
+        L.mvs_write_ply.argtypes = [C.c_char_p, vp, vp, i64, C.c_char_p]
+        L.mvs_read_ply.argtypes = [C.c_char_p, vp, vp, i64, P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -226,6 +235,40 @@ def view_intrinsics(f, cx, cy, full_width, w):
     if rc:
         raise MvsError(rc, "bad intrinsics")
     return tuple(out)
+
+
+def fusion_voxel(xs, ys, ranges, f, size, width, height, max_points):
+    xs = np.ascontiguousarray(xs, np.float64)
+    ys = np.ascontiguousarray(ys, np.float64)
+    rg = np.ascontiguousarray(ranges, np.float64).ravel()
+    out = C.c_double()
+    rc = lib().mvs_fusion_voxel(xs.ctypes.data, ys.ctypes.data, len(xs), rg.ctypes.data,
+                                len(rg) // 2, f, size, width, height, max_points, C.byref(out))
+    if rc:
+        raise MvsError(rc, "bad fusion inputs")
+    return out.value
+
+
+def write_ply(path, xyz, rgb, comment=""):
+    p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    c = np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3)
+    rc = lib().mvs_write_ply(os.fsencode(path), p.ctypes.data, c.ctypes.data, len(p),
+                             comment.encode())
+    if rc:
+        raise MvsError(rc, "write_ply failed")
+
+
+def read_ply(path):
+    n = C.c_int64()
+    rc = lib().mvs_read_ply(os.fsencode(path), None, None, 0, C.byref(n))
+    if rc:
+        raise MvsError(rc, "read_ply failed")
+    p = np.zeros((n.value, 3), np.float32)
+    c = np.zeros((n.value, 3), np.uint8)
+    rc = lib().mvs_read_ply(os.fsencode(path), p.ctypes.data, c.ctypes.data, n.value, C.byref(n))
+    if rc:
+        raise MvsError(rc, "read_ply failed")
+    return p, c
 
 
 class Context:
@@ -426,6 +469,30 @@ class Survey:
         self.ctx.check(lib().mvs_survey_download(self._h, view, d.ctypes.data, n.ctypes.data,
                                                  c.ctypes.data))
         return d, n, c
+
+    def set_colour(self, view, rgb):
+        c = np.ascontiguousarray(rgb, np.uint8)
+        self.ctx.check(lib().mvs_survey_set_colour(self._h, view, c.ctypes.data))
+
+    def fuse(self, refs, nbrs, voxel, rule=MVS_CONS_REFERENCE):
+        """Fuse the maps of refs; nbrs[r] lists ref r's neighbours. Returns the
+        cloud (xyz (n, 3) f32, rgb (n, 3) u8) in voxel-index order."""
+        refs_a = np.ascontiguousarray(refs, np.int32)
+        off = np.zeros(len(refs) + 1, np.int32)
+        for i, nb in enumerate(nbrs):
+            off[i + 1] = off[i] + len(nb)
+        flat = np.array([j for nb in nbrs for j in nb] or [0], np.int32)
+        nvox = C.c_int64()
+        self.ctx.check(lib().mvs_survey_fuse(self._h, len(refs_a), refs_a.ctypes.data,
+                                             off.ctypes.data, flat.ctypes.data, rule, voxel,
+                                             C.byref(nvox)))
+        n = C.c_int64()
+        self.ctx.check(lib().mvs_survey_fused_points(self._h, None, None, 0, C.byref(n)))
+        xyz = np.zeros((n.value, 3), np.float32)
+        rgb = np.zeros((n.value, 3), np.uint8)
+        self.ctx.check(lib().mvs_survey_fused_points(self._h, xyz.ctypes.data, rgb.ctypes.data,
+                                                     n.value, C.byref(n)))
+        return xyz, rgb
 
     def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE, download=True):
         """(agree counts or None, number of consistent pixels)."""

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <string>
 #include <tuple>
@@ -54,6 +55,8 @@ struct ViewDev {
     DevBuf quad;   // corner-packed 8-bit plane (8-bit inputs)
     DevBuf quadf;  // corner-packed grey plane
     bool has_quad = false;
+    DevBuf rgb;    // w*h*3 colour for the fused cloud (optional)
+    bool has_rgb = false;
     double f = 0, cx = 0, cy = 0, k1 = 0, k2 = 0;
     double R[9] = {0}, C[3] = {0};
     bool set = false;
@@ -131,6 +134,11 @@ struct mvs_survey {
     mvs_ctx *ctx = nullptr;
     std::vector<ViewDev> views;
     std::vector<MapDev> maps;
+    // voxel table of the last fusion
+    DevBuf fkeys, fsum, fcol, fcnt, fctr;
+    int64_t fcap = 0;
+    double fvoxel = 0.0;
+    bool fused = false;
 };
 
 namespace {
@@ -1227,8 +1235,11 @@ void mvs_survey_destroy(mvs_survey *s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->st);
+    DevBuf *fb[] = {&s->fkeys, &s->fsum, &s->fcol, &s->fcnt, &s->fctr};
+    for (DevBuf *b : fb) b->release();
     for (auto &v : s->views) {
         v.gray.release();
+        v.rgb.release();
         v.quadf.release();
         v.quad.release();
     }
@@ -1457,6 +1468,273 @@ int mvs_reset_stats(mvs_ctx *c) {
     if (!c) return MVS_ERR_ARG;
     TRY(sync_ctx(c));
     memset(&c->stats, 0, sizeof c->stats);
+    return MVS_OK;
+}
+
+int mvs_survey_set_colour(mvs_survey *s, int32_t view, const uint8_t *rgb) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    if (view < 0 || view >= (int32_t)s->views.size() || !s->views[(size_t)view].set)
+        return fail(c, MVS_ERR_ARG, "view missing");
+    if (!rgb) return fail(c, MVS_ERR_ARG, "null colour plane");
+    HIP_TRY(c, hipSetDevice(c->device));
+    ViewDev &v = s->views[(size_t)view];
+    const size_t n = (size_t)v.w * v.h * 3;
+    HIP_TRY(c, v.rgb.ensure(n));
+    HIP_TRY(c, hipMemcpyAsync(v.rgb.p, rgb, n, hipMemcpyHostToDevice, c->st));
+    HIP_TRY(c, hipStreamSynchronize(c->st));
+    v.has_rgb = true;
+    return MVS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Neighbour cameras of a fusion/consistency step (dense.rs:575-584): those of
+// the given list that have a depth map.
+void nbr_cams(mvs_survey *s, const int32_t *nbrs, int32_t nn, std::vector<NbrCam> &cams) {
+    cams.clear();
+    for (int32_t k = 0; k < nn; ++k) {
+        const int32_t j = nbrs[k];
+        if (j < 0 || j >= (int32_t)s->views.size() || !s->maps[(size_t)j].valid) continue;
+        const ViewDev &v = s->views[(size_t)j];
+        NbrCam nc;
+        memcpy(nc.R, v.R, sizeof nc.R);
+        memcpy(nc.C, v.C, sizeof nc.C);
+        nc.f = v.f;
+        nc.cx = v.cx;
+        nc.cy = v.cy;
+        nc.k1 = v.k1;
+        nc.k2 = v.k2;
+        nc.depth = s->maps[(size_t)j].depth.as<float>();
+        nc.w = v.w;
+        nc.h = v.h;
+        cams.push_back(nc);
+    }
+}
+
+RefCam64 ref_cam(const ViewDev &rv) {
+    RefCam64 r;
+    memcpy(r.R, rv.R, sizeof r.R);
+    memcpy(r.C, rv.C, sizeof r.C);
+    r.f = rv.f;
+    r.cx = rv.cx;
+    r.cy = rv.cy;
+    r.k1 = rv.k1;
+    r.k2 = rv.k2;
+    return r;
+}
+
+FuseTable fuse_table(mvs_survey *s) {
+    FuseTable t;
+    t.keys = s->fkeys.as<unsigned long long>();
+    t.sum = s->fsum.as<unsigned long long>();
+    t.col = s->fcol.as<unsigned int>();
+    t.cnt = s->fcnt.as<unsigned int>();
+    t.counters = s->fctr.as<unsigned long long>();
+    t.cap = s->fcap;
+    return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int32_t *nbr_off,
+                    const int32_t *nbrs, int32_t rule, double voxel, int64_t *n_voxels) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    if (rule != MVS_CONS_REFERENCE && rule != MVS_CONS_REPROJ)
+        return fail(c, MVS_ERR_ARG, "unknown consistency rule");
+    if (!(voxel > 0.0)) return fail(c, MVS_ERR_ARG, "voxel size must be positive");
+    if (nrefs < 0 || (nrefs > 0 && (!refs || !nbr_off))) return fail(c, MVS_ERR_ARG, "bad refs");
+    const int32_t nv = (int32_t)s->views.size();
+    int64_t pixels = 0;
+    for (int32_t r = 0; r < nrefs; ++r) {
+        if (refs[r] < 0 || refs[r] >= nv) return fail(c, MVS_ERR_ARG, "bad reference index");
+        if (s->maps[(size_t)refs[r]].valid)
+            pixels += (int64_t)s->views[(size_t)refs[r]].w * s->views[(size_t)refs[r]].h;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    // a table of about one slot per 4 pixels, doubled until nothing is dropped
+    int64_t cap = 1 << 16;
+    while (cap < pixels / 4 && cap < (1ll << 30)) cap <<= 1;
+    std::vector<NbrCam> cams;
+    for (int attempt = 0;; ++attempt) {
+        HIP_TRY(c, s->fkeys.ensure((size_t)cap * 8));
+        HIP_TRY(c, s->fsum.ensure((size_t)cap * 24));
+        HIP_TRY(c, s->fcol.ensure((size_t)cap * 12));
+        HIP_TRY(c, s->fcnt.ensure((size_t)cap * 4));
+        HIP_TRY(c, s->fctr.ensure(4 * 8));
+        s->fcap = cap;
+        const FuseTable t = fuse_table(s);
+        launch_fuse_clear(t, c->st);
+        HIP_TRY(c, hipGetLastError());
+        for (int32_t r = 0; r < nrefs; ++r) {
+            const int32_t ref = refs[r];
+            if (!s->maps[(size_t)ref].valid) continue;  // dense.rs:571-573
+            nbr_cams(s, nbrs + nbr_off[r], nbr_off[r + 1] - nbr_off[r], cams);
+            const int32_t nn = (int32_t)cams.size();
+            // dense.rs:585 for the reference rule; three views for reprojection
+            const int32_t need = rule == MVS_CONS_REPROJ ? 3 : (nn >= 3 ? 2 : 1);
+            HIP_TRY(c, c->nbr_cams.ensure(sizeof(NbrCam) * (cams.size() + 1)));
+            if (!cams.empty())
+                HIP_TRY(c, hipMemcpyAsync(c->nbr_cams.p, cams.data(), sizeof(NbrCam) * cams.size(),
+                                          hipMemcpyHostToDevice, c->st));
+            const ViewDev &rv = s->views[(size_t)ref];
+            launch_fuse(ref_cam(rv), s->maps[(size_t)ref].depth.as<float>(), rv.w, rv.h,
+                        c->nbr_cams.as<NbrCam>(), nn, rule, need, 1.0 / voxel,
+                        rv.has_rgb ? rv.rgb.as<uint8_t>() : nullptr,
+                        rv.has_quad ? rv.quad.as<uint32_t>() : nullptr, rv.gray.as<float>(), t,
+                        c->st);
+            HIP_TRY(c, hipGetLastError());
+            // the neighbour table is rewritten for the next view
+            HIP_TRY(c, hipStreamSynchronize(c->st));
+        }
+        unsigned long long ctr[4];
+        HIP_TRY(c, hipMemcpy(ctr, s->fctr.p, sizeof ctr, hipMemcpyDeviceToHost));
+        if (ctr[3] > 0)
+            return fail(c, MVS_ERR_ARG, "voxel index beyond +-2^20: voxel too small for the scene");
+        if (ctr[2] > 0 && cap < (1ll << 30)) {
+            cap <<= 2;  // table full: start again with a larger one
+            continue;
+        }
+        if (ctr[2] > 0) return fail(c, MVS_ERR_NOMEM, "fusion table full");
+        s->fvoxel = voxel;
+        s->fused = true;
+        if (n_voxels) *n_voxels = (int64_t)ctr[0];
+        return MVS_OK;
+    }
+}
+
+int mvs_survey_fused_points(mvs_survey *s, float *xyz, uint8_t *rgb, int64_t cap, int64_t *n) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    if (!s->fused) return fail(c, MVS_ERR_ARG, "no fusion has run");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t tc = s->fcap;
+    std::vector<unsigned long long> keys((size_t)tc), sum((size_t)tc * 3);
+    std::vector<unsigned int> col((size_t)tc * 3), cnt((size_t)tc);
+    HIP_TRY(c, hipMemcpy(keys.data(), s->fkeys.p, (size_t)tc * 8, hipMemcpyDeviceToHost));
+    std::vector<std::pair<unsigned long long, int64_t>> occ;
+    for (int64_t i = 0; i < tc; ++i)
+        if (keys[(size_t)i] != ~0ull) occ.emplace_back(keys[(size_t)i], i);
+    if (n) *n = (int64_t)occ.size();
+    if (!xyz && !rgb) return MVS_OK;
+    if (cap < (int64_t)occ.size()) return fail(c, MVS_ERR_ARG, "output arrays too small");
+    HIP_TRY(c, hipMemcpy(sum.data(), s->fsum.p, (size_t)tc * 24, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(col.data(), s->fcol.p, (size_t)tc * 12, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(cnt.data(), s->fcnt.p, (size_t)tc * 4, hipMemcpyDeviceToHost));
+    // one point per voxel, in voxel-index order (x, then y, then z)
+    std::sort(occ.begin(), occ.end());
+    for (size_t o = 0; o < occ.size(); ++o) {
+        const int64_t i = occ[o].second;
+        const unsigned int k = cnt[(size_t)i];
+        // dense.rs:637-641: the mean position and the mean colour, truncated
+        for (int r = 0; r < 3; ++r) {
+            if (xyz)
+                xyz[3 * o + r] =
+                    (float)((double)(long long)sum[(size_t)(3 * i + r)] / 1048576.0 / (double)k);
+            if (rgb) rgb[3 * o + r] = (uint8_t)((float)col[(size_t)(3 * i + r)] / (float)k);
+        }
+    }
+    return MVS_OK;
+}
+
+int mvs_fusion_voxel(const double *xs, const double *ys, int64_t npts, const double *ranges,
+                     int32_t ncams, double f, int32_t size, int32_t width, int32_t height,
+                     int64_t max_points, double *voxel) {
+    if (!voxel || npts < 1 || ncams < 1 || !xs || !ys || !ranges || size < 1 || width < 1 ||
+        height < 1 || max_points < 1)
+        return MVS_ERR_ARG;
+    // dense.rs:545-562
+    std::vector<double> sx(xs, xs + npts), sy(ys, ys + npts), dm((size_t)ncams);
+    std::sort(sx.begin(), sx.end());
+    std::sort(sy.begin(), sy.end());
+    for (int32_t i = 0; i < ncams; ++i) dm[(size_t)i] = (ranges[2 * i] + ranges[2 * i + 1]) / 2.0;
+    std::sort(dm.begin(), dm.end());
+    auto pct = [&](const std::vector<double> &v, double q) {
+        return v[(size_t)((double)(v.size() - 1) * q)];
+    };
+    double area = (pct(sx, 0.99) - pct(sx, 0.01)) * (pct(sy, 0.99) - pct(sy, 0.01));
+    if (!(area >= 1.0)) area = 1.0;
+    const double f0 = f * (double)size / (double)std::max(width, height);
+    const double gsd = dm[dm.size() / 2] / f0;
+    const double alt = std::sqrt(area * 2.5 / (double)max_points);
+    *voxel = std::max(gsd * 1.5, alt);
+    return MVS_OK;
+}
+
+int mvs_write_ply(const char *path, const float *xyz, const uint8_t *rgb, int64_t n,
+                  const char *comment) {
+    if (!path || n < 0 || (n > 0 && (!xyz || !rgb))) return MVS_ERR_ARG;
+    // write_ply (ply.rs:11-40): ASCII header, then per vertex 3 f32 LE and 3
+    // u8, written to path's ".ply.tmp" sibling and renamed
+    std::string tmp = std::string(path);
+    const size_t slash = tmp.find_last_of('/');
+    const size_t dot = tmp.find_last_of('.');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash))
+        tmp = tmp.substr(0, dot);
+    tmp += ".ply.tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return MVS_ERR_IO;
+    std::string hdr = "ply\nformat binary_little_endian 1.0\ncomment ";
+    hdr += comment ? comment : "";
+    hdr += "\nelement vertex " + std::to_string(n) +
+           "\nproperty float x\nproperty float y\nproperty float z\n"
+           "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n";
+    bool ok = fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size();
+    std::vector<uint8_t> rec((size_t)15 * 65536);
+    for (int64_t i0 = 0; ok && i0 < n; i0 += 65536) {
+        const int64_t m = std::min<int64_t>(65536, n - i0);
+        for (int64_t i = 0; i < m; ++i) {
+            memcpy(&rec[(size_t)(15 * i)], xyz + 3 * (i0 + i), 12);  // little-endian hosts
+            memcpy(&rec[(size_t)(15 * i + 12)], rgb + 3 * (i0 + i), 3);
+        }
+        ok = fwrite(rec.data(), 15, (size_t)m, f) == (size_t)m;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path) != 0) return MVS_ERR_IO;
+    return MVS_OK;
+}
+
+int mvs_read_ply(const char *path, float *xyz, uint8_t *rgb, int64_t cap, int64_t *n) {
+    if (!path || !n) return MVS_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return MVS_ERR_IO;
+    // read_ply (ply.rs:44-70): "element vertex N", then records after end_header
+    char line[4096];
+    int64_t count = -1;
+    bool header = false;
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, "element vertex ", 15) == 0) count = strtoll(line + 15, nullptr, 10);
+        if (strcmp(line, "end_header\n") == 0 || strcmp(line, "end_header\r\n") == 0) {
+            header = true;
+            break;
+        }
+    }
+    if (!header || count < 0) {
+        fclose(f);
+        return MVS_ERR_FORMAT;
+    }
+    *n = count;
+    if (xyz || rgb) {
+        if (cap < count) {
+            fclose(f);
+            return MVS_ERR_ARG;
+        }
+        uint8_t rec[15];
+        for (int64_t i = 0; i < count; ++i) {
+            if (fread(rec, 1, 15, f) != 15) {
+                fclose(f);
+                return MVS_ERR_FORMAT;
+            }
+            if (xyz) memcpy(xyz + 3 * i, rec, 12);
+            if (rgb) memcpy(rgb + 3 * i, rec + 12, 3);
+        }
+    }
+    fclose(f);
     return MVS_OK;
 }
 

@@ -120,6 +120,27 @@ void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int3
                         const NbrCam *nbrs, int32_t nn, int32_t need, int32_t rule,
                         int32_t *agree, unsigned long long *n_ok, hipStream_t st);
 void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st);
+// Voxel accumulation table of the fusion (open addressing, linear probing).
+// Keys pack the voxel index (21 bits per axis, offset 2^20); coordinate sums
+// are fixed point (2^20 units per metre) in 64-bit two's complement, so the
+// sums do not depend on the order points arrive in.
+struct FuseTable {
+    unsigned long long *keys;      // cap, empty = ~0
+    unsigned long long *sum;       // 3 per slot
+    unsigned int *col;             // 3 per slot: colour sums
+    unsigned int *cnt;             // 1 per slot
+    unsigned long long *counters;  // [0] voxels, [1] points, [2] table full, [3] key range
+    int64_t cap;                   // power of two
+};
+void launch_fuse_clear(const FuseTable &t, hipStream_t st);
+// One reference view: its confirmed pixels into the table (dense.rs:589-632).
+// Colour from rgb (w*h*3, nullable), else the pixel's 8-bit grey (quad byte 0,
+// nullable), else the f32 grey x 255.
+void launch_fuse(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
+                 const NbrCam *nbrs, int32_t nn, int32_t rule, int32_t need, double inv_voxel,
+                 const uint8_t *rgb, const uint32_t *quad, const float *gray, const FuseTable &t,
+                 hipStream_t st);
+
 // Gauss-Newton refinement of a job's accepted pixels (steps per pixel);
 // counters (nullable) [0] += accepted steps, [1] += pixels refined.
 int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long long *counters,

@@ -18,6 +18,7 @@
 // min/min reduction over the group's lanes.
 #include "mvs_internal.h"
 #include "mvs_device.h"
+#include "mvs_geometry.h"
 
 #include <math.h>
 
@@ -192,9 +193,9 @@ __global__ void k_threshold(RefJob job, float *__restrict__ depth, float *__rest
 }
 
 // Geometric consistency (dense.rs:585-615) in f64 like the reference's fusion.
-// rule 0: dense.rs:585-615 (relative depth difference < 1%); rule 1: the
-// neighbour's own point at the pixel the ref point lands on, back-projected
-// from that pixel's centre, reprojects within 1 px of the ref pixel centre.
+// Geometric consistency of one depth map, per pixel (agree_pixel in
+// mvs_geometry.h): agree counts (-1 where depth <= 0) and the number of
+// pixels with agree >= need.
 __global__ void k_consistency(RefCam64 rf, const float *__restrict__ depth, int32_t w, int32_t h,
                               const NbrCam *__restrict__ nbrs, int32_t nn, int32_t need,
                               int32_t rule, int32_t *__restrict__ agree,
@@ -202,99 +203,10 @@ __global__ void k_consistency(RefCam64 rf, const float *__restrict__ depth, int3
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool good = false;
     if (i < (int64_t)w * h) {
-        const float di = depth[i];
-        if (di <= 0.0f) {
-            agree[i] = -1;
-        } else {
-            const int64_t y = i / w, x = i - y * w;
-            const double qx = ((double)x + 0.5 - rf.cx) / rf.f;
-            const double qy = ((double)y + 0.5 - rf.cy) / rf.f;
-            double px = qx, py = qy;
-            for (int k = 0; k < 8; ++k) {
-                const double r2 = px * px + py * py;
-                double d = 1.0 + rf.k1 * r2;
-                d = d + (rf.k2 * r2) * r2;
-                px = qx / d;
-                py = qy / d;
-            }
-            const double dd = (double)di;
-            const double xc0 = px * dd, xc1 = py * dd, xc2 = dd;
-            double xw[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                double acc = rf.R[0 * 3 + r] * xc0;
-                acc = acc + rf.R[1 * 3 + r] * xc1;
-                acc = acc + rf.R[2 * 3 + r] * xc2;
-                xw[r] = acc + rf.C[r];
-            }
-            int32_t a = 0;
-            for (int32_t k = 0; k < nn; ++k) {
-                const NbrCam &v = nbrs[k];
-                const double e0 = xw[0] - v.C[0], e1 = xw[1] - v.C[1], e2 = xw[2] - v.C[2];
-                double xs[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    double acc = v.R[r * 3 + 0] * e0;
-                    acc = acc + v.R[r * 3 + 1] * e1;
-                    acc = acc + v.R[r * 3 + 2] * e2;
-                    xs[r] = acc;
-                }
-                if (xs[2] <= 0.0) continue;
-                const double ux = xs[0] / xs[2], uy = xs[1] / xs[2];
-                const double r2 = ux * ux + uy * uy;
-                double d = 1.0 + v.k1 * r2;
-                d = d + (v.k2 * r2) * r2;
-                const double fd = v.f * d;
-                const double u0 = fd * ux + v.cx, u1 = fd * uy + v.cy;
-                const double fx = floor(u0), fy = floor(u1);
-                if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)v.w) || !(fy < (double)v.h))
-                    continue;
-                const double dn = (double)v.depth[(int64_t)fy * v.w + (int64_t)fx];
-                if (rule == 0) {
-                    if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) ++a;
-                    continue;
-                }
-                if (!(dn > 0.0)) continue;
-                // back-project the neighbour pixel centre at its own depth
-                const double nqx = (fx + 0.5 - v.cx) / v.f, nqy = (fy + 0.5 - v.cy) / v.f;
-                double npx = nqx, npy = nqy;
-                for (int k2 = 0; k2 < 8; ++k2) {
-                    const double r2n = npx * npx + npy * npy;
-                    double dd = 1.0 + v.k1 * r2n;
-                    dd = dd + (v.k2 * r2n) * r2n;
-                    npx = nqx / dd;
-                    npy = nqy / dd;
-                }
-                const double xn0 = npx * dn, xn1 = npy * dn, xn2 = dn;
-                double er[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    double acc = v.R[0 * 3 + r] * xn0;
-                    acc = acc + v.R[1 * 3 + r] * xn1;
-                    acc = acc + v.R[2 * 3 + r] * xn2;
-                    er[r] = acc + v.C[r] - rf.C[r];
-                }
-                double xr[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    double acc = rf.R[r * 3 + 0] * er[0];
-                    acc = acc + rf.R[r * 3 + 1] * er[1];
-                    acc = acc + rf.R[r * 3 + 2] * er[2];
-                    xr[r] = acc;
-                }
-                if (!(xr[2] > 0.0)) continue;
-                const double rx = xr[0] / xr[2], ry = xr[1] / xr[2];
-                const double r2r = rx * rx + ry * ry;
-                double dr = 1.0 + rf.k1 * r2r;
-                dr = dr + (rf.k2 * r2r) * r2r;
-                const double fr = rf.f * dr;
-                const double ex = (fr * rx + rf.cx) - ((double)x + 0.5);
-                const double ey = (fr * ry + rf.cy) - ((double)y + 0.5);
-                if (ex * ex + ey * ey <= 1.0) ++a;
-            }
-            agree[i] = a;
-            good = a >= need;
-        }
+        double xw[3];
+        const int32_t a = agree_pixel(rf, depth, w, nbrs, nn, rule, i, xw);
+        agree[i] = a;
+        good = a >= 0 && a >= need;
     }
     const uint64_t m = __ballot(good);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_ok, (unsigned long long)__popcll(m));

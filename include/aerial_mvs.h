@@ -202,6 +202,36 @@ enum { MVS_CONS_REFERENCE = 0, MVS_CONS_REPROJ = 1 };
 int mvs_survey_consistency(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int32_t nn,
                            int32_t rule, int32_t *agree, int64_t *n_consistent);
 
+/* ---- fusion: consistent depths accumulated per voxel (dense.rs:542-651) ----
+ * Colour plane of a survey view for the fused cloud (w*h*3 RGB, the `.view`
+ * cache's rgb); views without one use their 8-bit grey. */
+int mvs_survey_set_colour(mvs_survey *sv, int32_t view, const uint8_t *rgb);
+/* Fuses the maps of refs[r] (views without a map are skipped, dense.rs:571):
+ * every pixel that the neighbours nbrs[nbr_off[r] .. nbr_off[r+1]) with maps
+ * confirm under `rule` (MVS_CONS_REFERENCE: dense.rs:585-612;
+ * MVS_CONS_REPROJ) adds its world point and colour to voxel floor(X/voxel)
+ * (dense.rs:617-628). Sums are exact fixed point, so the cloud does not depend
+ * on the order views or pixels arrive in (the reference's f32 HashMap merge
+ * does). n_voxels receives the number of voxels. */
+int mvs_survey_fuse(mvs_survey *sv, int32_t nrefs, const int32_t *refs, const int32_t *nbr_off,
+                    const int32_t *nbrs, int32_t rule, double voxel, int64_t *n_voxels);
+/* The last fusion's cloud, one point per voxel in voxel-index order: mean
+ * position (f32) and mean colour truncated to u8 (dense.rs:634-641). With
+ * xyz and rgb NULL only n is written; cap = points the arrays hold. */
+int mvs_survey_fused_points(mvs_survey *sv, float *xyz, uint8_t *rgb, int64_t cap, int64_t *n);
+/* The fusion voxel edge (dense.rs:545-562): sparse point plan extent (1%-99%
+ * percentiles of xs, ys), median of the cameras' mid depths (ranges: 2 per
+ * camera), focal f of the first camera at full size width x height, working
+ * size `size` and the profile's max_points. */
+int mvs_fusion_voxel(const double *xs, const double *ys, int64_t npts, const double *ranges,
+                     int32_t ncams, double f, int32_t size, int32_t width, int32_t height,
+                     int64_t max_points, double *voxel);
+/* Binary PLY as write_ply / read_ply (ply.rs:11-70): header, then per point
+ * 3 x f32 LE and 3 x u8; written to the ".ply.tmp" sibling and renamed. */
+int mvs_write_ply(const char *path, const float *xyz, const uint8_t *rgb, int64_t n,
+                  const char *comment);
+int mvs_read_ply(const char *path, float *xyz, uint8_t *rgb, int64_t cap, int64_t *n);
+
 /* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
 typedef struct {
     int64_t sweep_launches;   /* half-sweeps (every kernel class of a batch) */

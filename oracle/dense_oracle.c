@@ -561,82 +561,84 @@ typedef struct {
     int32_t rule; /* 0: dense.rs:585-615; 1: reprojection <= 1 px */
 } cons_arg;
 
+/* dense.rs:585-612 for one pixel: -1 where depth <= 0, else the number of
+ * neighbours that agree (rule 0: relative depth difference < 1%; rule 1:
+ * reprojection within 1 px); xw receives the pixel's world point. */
+static int32_t cons_pixel(const cons_arg *a, int32_t x, int32_t y, double xw[3]) {
+    const orc_view *rf = a->ref;
+    const size_t i = (size_t)y * (size_t)rf->w + (size_t)x;
+    const float di = a->depth[i];
+    if (di <= 0.0f) return -1;
+    double pp[2];
+    orc_undistort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, (double)x + 0.5, (double)y + 0.5, pp);
+    double xc[3] = {pp[0] * (double)di, pp[1] * (double)di, (double)di};
+    for (int r = 0; r < 3; ++r) {
+        /* rwc = R^T: row r of R^T is column r of R */
+        double acc = rf->R[0 * 3 + r] * xc[0];
+        acc = acc + rf->R[1 * 3 + r] * xc[1];
+        acc = acc + rf->R[2 * 3 + r] * xc[2];
+        xw[r] = acc + rf->C[r];
+    }
+    int32_t agree = 0;
+    for (int32_t k = 0; k < a->nnbr; ++k) {
+        const orc_view *v = a->nbrs[k].view;
+        double dx[3] = {xw[0] - v->C[0], xw[1] - v->C[1], xw[2] - v->C[2]};
+        double xs[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = v->R[r * 3 + 0] * dx[0];
+            acc = acc + v->R[r * 3 + 1] * dx[1];
+            acc = acc + v->R[r * 3 + 2] * dx[2];
+            xs[r] = acc;
+        }
+        if (xs[2] <= 0.0) continue;
+        double u[2];
+        orc_distort(v->f, v->cx, v->cy, v->k1, v->k2, xs[0] / xs[2], xs[1] / xs[2], u);
+        double fx = floor(u[0]), fy = floor(u[1]);
+        if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)v->w) || !(fy < (double)v->h))
+            continue;
+        int64_t ux = (int64_t)fx, uy = (int64_t)fy;
+        double dn = (double)a->nbrs[k].depth[(size_t)uy * (size_t)v->w + (size_t)ux];
+        if (a->rule == 0) {
+            if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) agree += 1;
+            continue;
+        }
+        /* reprojection: the neighbour's own point at that pixel, back in
+         * the reference image, within 1 px of the pixel centre */
+        if (!(dn > 0.0)) continue;
+        double pn[2];
+        orc_undistort(v->f, v->cx, v->cy, v->k1, v->k2, (double)ux + 0.5, (double)uy + 0.5, pn);
+        double xn[3] = {pn[0] * dn, pn[1] * dn, dn}, xwn[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = v->R[0 * 3 + r] * xn[0];
+            acc = acc + v->R[1 * 3 + r] * xn[1];
+            acc = acc + v->R[2 * 3 + r] * xn[2];
+            xwn[r] = acc + v->C[r];
+        }
+        double er[3] = {xwn[0] - rf->C[0], xwn[1] - rf->C[1], xwn[2] - rf->C[2]}, xr[3];
+        for (int r = 0; r < 3; ++r) {
+            double acc = rf->R[r * 3 + 0] * er[0];
+            acc = acc + rf->R[r * 3 + 1] * er[1];
+            acc = acc + rf->R[r * 3 + 2] * er[2];
+            xr[r] = acc;
+        }
+        if (!(xr[2] > 0.0)) continue;
+        double ur[2];
+        orc_distort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, xr[0] / xr[2], xr[1] / xr[2], ur);
+        double ex = ur[0] - ((double)x + 0.5), ey = ur[1] - ((double)y + 0.5);
+        if (ex * ex + ey * ey <= 1.0) agree += 1;
+    }
+    return agree;
+}
+
 /* dense.rs:585-615 */
 static void cons_rows(void *p, int32_t y0, int32_t y1, int64_t *cnt) {
     cons_arg *a = (cons_arg *)p;
-    const orc_view *rf = a->ref;
-    const int32_t w = rf->w;
+    const int32_t w = a->ref->w;
     for (int32_t y = y0; y < y1; ++y)
         for (int32_t x = 0; x < w; ++x) {
-            size_t i = (size_t)y * (size_t)w + (size_t)x;
-            float di = a->depth[i];
-            if (di <= 0.0f) {
-                a->agree[i] = -1;
-                continue;
-            }
-            double pp[2];
-            orc_undistort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, (double)x + 0.5, (double)y + 0.5,
-                          pp);
-            double xc[3] = {pp[0] * (double)di, pp[1] * (double)di, (double)di};
             double xw[3];
-            for (int r = 0; r < 3; ++r) {
-                /* rwc = R^T: row r of R^T is column r of R */
-                double acc = rf->R[0 * 3 + r] * xc[0];
-                acc = acc + rf->R[1 * 3 + r] * xc[1];
-                acc = acc + rf->R[2 * 3 + r] * xc[2];
-                xw[r] = acc + rf->C[r];
-            }
-            int32_t agree = 0;
-            for (int32_t k = 0; k < a->nnbr; ++k) {
-                const orc_view *v = a->nbrs[k].view;
-                double dx[3] = {xw[0] - v->C[0], xw[1] - v->C[1], xw[2] - v->C[2]};
-                double xs[3];
-                for (int r = 0; r < 3; ++r) {
-                    double acc = v->R[r * 3 + 0] * dx[0];
-                    acc = acc + v->R[r * 3 + 1] * dx[1];
-                    acc = acc + v->R[r * 3 + 2] * dx[2];
-                    xs[r] = acc;
-                }
-                if (xs[2] <= 0.0) continue;
-                double u[2];
-                orc_distort(v->f, v->cx, v->cy, v->k1, v->k2, xs[0] / xs[2], xs[1] / xs[2], u);
-                double fx = floor(u[0]), fy = floor(u[1]);
-                if (!(fx >= 0.0) || !(fy >= 0.0) || !(fx < (double)v->w) || !(fy < (double)v->h))
-                    continue;
-                int64_t ux = (int64_t)fx, uy = (int64_t)fy;
-                double dn = (double)a->nbrs[k].depth[(size_t)uy * (size_t)v->w + (size_t)ux];
-                if (a->rule == 0) {
-                    if (dn > 0.0 && fabs((dn - xs[2]) / xs[2]) < 0.01) agree += 1;
-                    continue;
-                }
-                /* reprojection: the neighbour's own point at that pixel, back in
-                 * the reference image, within 1 px of the pixel centre */
-                if (!(dn > 0.0)) continue;
-                double pn[2];
-                orc_undistort(v->f, v->cx, v->cy, v->k1, v->k2, (double)ux + 0.5,
-                              (double)uy + 0.5, pn);
-                double xn[3] = {pn[0] * dn, pn[1] * dn, dn}, xwn[3];
-                for (int r = 0; r < 3; ++r) {
-                    double acc = v->R[0 * 3 + r] * xn[0];
-                    acc = acc + v->R[1 * 3 + r] * xn[1];
-                    acc = acc + v->R[2 * 3 + r] * xn[2];
-                    xwn[r] = acc + v->C[r];
-                }
-                double er[3] = {xwn[0] - rf->C[0], xwn[1] - rf->C[1], xwn[2] - rf->C[2]}, xr[3];
-                for (int r = 0; r < 3; ++r) {
-                    double acc = rf->R[r * 3 + 0] * er[0];
-                    acc = acc + rf->R[r * 3 + 1] * er[1];
-                    acc = acc + rf->R[r * 3 + 2] * er[2];
-                    xr[r] = acc;
-                }
-                if (!(xr[2] > 0.0)) continue;
-                double ur[2];
-                orc_distort(rf->f, rf->cx, rf->cy, rf->k1, rf->k2, xr[0] / xr[2], xr[1] / xr[2],
-                            ur);
-                double ex = ur[0] - ((double)x + 0.5), ey = ur[1] - ((double)y + 0.5);
-                if (ex * ex + ey * ey <= 1.0) agree += 1;
-            }
-            a->agree[i] = agree;
+            const int32_t agree = cons_pixel(a, x, y, xw);
+            a->agree[(size_t)y * (size_t)w + (size_t)x] = agree;
             if (agree >= a->need) *cnt += 1;
         }
 }
@@ -651,6 +653,71 @@ int64_t orc_consistency_reproj(const orc_view *ref, const float *depth, const or
                                int32_t nnbr, int32_t need, int32_t *agree, int32_t nthreads) {
     cons_arg a = {ref, depth, nbrs, nnbr, agree, need, 1};
     return run_rows(cons_rows, &a, ref->h, nthreads);
+}
+
+/* dense.rs:589-632 for one reference view: every pixel whose depth the
+ * neighbours confirm (agree >= need under `rule`) emits its voxel key
+ * floor(xw / voxel) (dense.rs:617-621), its world point (f64) and its pixel
+ * index (for the colour, dense.rs:622). Sequential; returns the count. */
+int64_t orc_fuse_points(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                        int32_t nnbr, int32_t rule, int32_t need, double inv_voxel,
+                        int32_t *keys, double *xw_out, int32_t *pix) {
+    cons_arg a = {ref, depth, nbrs, nnbr, NULL, need, rule};
+    int64_t n = 0;
+    for (int32_t y = 0; y < ref->h; ++y)
+        for (int32_t x = 0; x < ref->w; ++x) {
+            double xw[3];
+            const int32_t agree = cons_pixel(&a, x, y, xw);
+            if (agree < 0 || agree < need) continue;
+            for (int r = 0; r < 3; ++r) {
+                keys[3 * n + r] = (int32_t)floor(xw[r] * inv_voxel);
+                xw_out[3 * n + r] = xw[r];
+            }
+            pix[n] = y * ref->w + x;
+            ++n;
+        }
+    return n;
+}
+
+/* dense.rs:545-562: the voxel edge from the sparse points' 1%-99% plan extent,
+ * the cameras' mid depths and the working-size focal length. */
+double orc_fusion_voxel(const double *xs, const double *ys, int64_t npts, const double *ranges,
+                        int32_t ncams, double f, int32_t size, int32_t width, int32_t height,
+                        int64_t max_points) {
+    double *sx = (double *)malloc(sizeof(double) * (size_t)npts);
+    double *sy = (double *)malloc(sizeof(double) * (size_t)npts);
+    double *dm = (double *)malloc(sizeof(double) * (size_t)ncams);
+    for (int64_t i = 0; i < npts; ++i) {
+        sx[i] = xs[i];
+        sy[i] = ys[i];
+    }
+    for (int32_t c = 0; c < ncams; ++c) dm[c] = (ranges[2 * c] + ranges[2 * c + 1]) / 2.0;
+    /* insertion sorts keep the oracle free of library ordering subtleties */
+    for (int64_t i = 1; i < npts; ++i)
+        for (int64_t j = i; j > 0 && sx[j - 1] > sx[j]; --j) {
+            double t = sx[j]; sx[j] = sx[j - 1]; sx[j - 1] = t;
+        }
+    for (int64_t i = 1; i < npts; ++i)
+        for (int64_t j = i; j > 0 && sy[j - 1] > sy[j]; --j) {
+            double t = sy[j]; sy[j] = sy[j - 1]; sy[j - 1] = t;
+        }
+    for (int32_t i = 1; i < ncams; ++i)
+        for (int32_t j = i; j > 0 && dm[j - 1] > dm[j]; --j) {
+            double t = dm[j]; dm[j] = dm[j - 1]; dm[j - 1] = t;
+        }
+#define ORC_PCT(v, q) (v)[(size_t)((double)(npts - 1) * (q))]
+    double area = (ORC_PCT(sx, 0.99) - ORC_PCT(sx, 0.01)) * (ORC_PCT(sy, 0.99) - ORC_PCT(sy, 0.01));
+#undef ORC_PCT
+    if (!(area >= 1.0)) area = 1.0;
+    const double f0 = f * (double)size / (double)(width > height ? width : height);
+    const double gsd = dm[ncams / 2] / f0;
+    double voxel = gsd * 1.5;
+    const double alt = sqrt(area * 2.5 / (double)max_points);
+    if (alt > voxel) voxel = alt;
+    free(sx);
+    free(sy);
+    free(dm);
+    return voxel;
 }
 
 /* ------------------------------------------------------------------------ */

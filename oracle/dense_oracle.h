@@ -115,6 +115,19 @@ int64_t orc_consistency(const orc_view *ref, const float *depth, const orc_nbr_d
 int64_t orc_consistency_reproj(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
                                int32_t nnbr, int32_t need, int32_t *agree, int32_t nthreads);
 
+/* Fusion's point emission for one view (dense.rs:589-632): for every pixel
+ * with agree >= need under `rule` (0 reference, 1 reprojection), its voxel key
+ * floor(xw * inv_voxel) (3 x i32), world point (3 x f64) and pixel index.
+ * Arrays hold w*h entries; returns the count. The per-voxel sums are the
+ * caller's (the reference's f32 HashMap merge order is not deterministic). */
+int64_t orc_fuse_points(const orc_view *ref, const float *depth, const orc_nbr_depth *nbrs,
+                        int32_t nnbr, int32_t rule, int32_t need, double inv_voxel,
+                        int32_t *keys, double *xw, int32_t *pix);
+/* The fusion voxel edge (dense.rs:545-562). ranges: 2 per camera. */
+double orc_fusion_voxel(const double *xs, const double *ys, int64_t npts, const double *ranges,
+                        int32_t ncams, double f, int32_t size, int32_t width, int32_t height,
+                        int64_t max_points);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,13 @@ def lib():
                                       C.c_int32, C.POINTER(C.c_double), _f64p,
                                       C.POINTER(C.c_uint64)]
         L.orc_pair_cost64.restype = C.c_int
+        L.orc_fuse_points.argtypes = [C.POINTER(OrcView), _f32p, C.POINTER(OrcNbrDepth),
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, _i32p,
+                                      _f64p, _i32p]
+        L.orc_fuse_points.restype = C.c_int64
+        L.orc_fusion_voxel.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int32, C.c_double,
+                                       C.c_int32, C.c_int32, C.c_int32, C.c_int64]
+        L.orc_fusion_voxel.restype = C.c_double
         _lib = L
     return _lib
 
@@ -319,3 +326,42 @@ def consistency(ref, depth, nbrs, nthreads=8, rule=0, need=3):
         n = lib().orc_consistency_reproj(C.byref(rv), depth.ravel(), arr, len(nbrs), need,
                                          agree.ravel(), nthreads)
     return agree, n
+
+
+def _nbr_arrays(nbrs):
+    keep = []
+    arr = (OrcNbrDepth * max(len(nbrs), 1))()
+    views = (OrcView * max(len(nbrs), 1))()
+    for i, (v, d) in enumerate(nbrs):
+        d = np.ascontiguousarray(d, np.float32)
+        keep.append(d)
+        views[i] = _orc_view(v, None)
+        arr[i].view = C.pointer(views[i])
+        arr[i].depth = d.ctypes.data
+    return arr, views, keep
+
+
+def fuse_points(ref, depth, nbrs, rule=0, need=None, voxel=0.1):
+    """Fusion's per-pixel emission for one reference view (dense.rs:589-632):
+    (keys (n, 3) int32, world points (n, 3) f64, pixel indices (n,))."""
+    if need is None:
+        need = (2 if len(nbrs) >= 3 else 1) if rule == 0 else 3
+    rv = _orc_view(ref, None)
+    arr, views, keep = _nbr_arrays(nbrs)
+    depth = np.ascontiguousarray(depth, np.float32)
+    n = depth.size
+    keys = np.zeros(3 * n, np.int32)
+    xw = np.zeros(3 * n, np.float64)
+    pix = np.zeros(n, np.int32)
+    m = lib().orc_fuse_points(C.byref(rv), depth.ravel(), arr, len(nbrs), rule, need,
+                              1.0 / voxel, keys, xw, pix)
+    return keys[:3 * m].reshape(-1, 3), xw[:3 * m].reshape(-1, 3), pix[:m]
+
+
+def fusion_voxel(xs, ys, ranges, f, size, width, height, max_points):
+    """dense.rs:545-562."""
+    xs = np.ascontiguousarray(xs, np.float64)
+    ys = np.ascontiguousarray(ys, np.float64)
+    rg = np.ascontiguousarray(ranges, np.float64).ravel()
+    return lib().orc_fusion_voxel(xs, ys, len(xs), rg, len(rg) // 2, f, size, width, height,
+                                  max_points)
