@@ -1557,9 +1557,10 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
             pixels += (int64_t)s->views[(size_t)refs[r]].w * s->views[(size_t)refs[r]].h;
     }
     HIP_TRY(c, hipSetDevice(c->device));
-    // a table of about one slot per 4 pixels, doubled until nothing is dropped
+    // a table of about one slot per 32 pixels (a voxel gathers tens of
+    // points); grown 4x and refilled when it passes 70% occupancy
     int64_t cap = 1 << 16;
-    while (cap < pixels / 4 && cap < (1ll << 30)) cap <<= 1;
+    while (cap < pixels / 32 && cap < (1ll << 30)) cap <<= 1;
     std::vector<NbrCam> cams;
     for (int attempt = 0;; ++attempt) {
         HIP_TRY(c, s->fkeys.ensure((size_t)cap * 8));
@@ -1571,7 +1572,8 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
         const FuseTable t = fuse_table(s);
         launch_fuse_clear(t, c->st);
         HIP_TRY(c, hipGetLastError());
-        for (int32_t r = 0; r < nrefs; ++r) {
+        bool grow = false;
+        for (int32_t r = 0; r < nrefs && !grow; ++r) {
             const int32_t ref = refs[r];
             if (!s->maps[(size_t)ref].valid) continue;  // dense.rs:571-573
             nbr_cams(s, nbrs + nbr_off[r], nbr_off[r + 1] - nbr_off[r], cams);
@@ -1591,13 +1593,16 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
             HIP_TRY(c, hipGetLastError());
             // the neighbour table is rewritten for the next view
             HIP_TRY(c, hipStreamSynchronize(c->st));
+            unsigned long long v[4];
+            HIP_TRY(c, hipMemcpy(v, s->fctr.p, sizeof v, hipMemcpyDeviceToHost));
+            grow = (v[2] > 0 || (double)v[0] > 0.7 * (double)cap) && cap < (1ll << 30);
         }
         unsigned long long ctr[4];
         HIP_TRY(c, hipMemcpy(ctr, s->fctr.p, sizeof ctr, hipMemcpyDeviceToHost));
         if (ctr[3] > 0)
             return fail(c, MVS_ERR_ARG, "voxel index beyond +-2^20: voxel too small for the scene");
-        if (ctr[2] > 0 && cap < (1ll << 30)) {
-            cap <<= 2;  // table full: start again with a larger one
+        if (grow) {
+            cap <<= 2;  // start again with a larger table
             continue;
         }
         if (ctr[2] > 0) return fail(c, MVS_ERR_NOMEM, "fusion table full");

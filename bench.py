@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--mode", choices=("fast", "exact"), default="fast",
                     help="arithmetic of the batched path's PatchMatch kernels")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--cpu-crop", default="1280x960")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--plain-views", type=int, default=8)
@@ -313,6 +314,32 @@ def main():
         err["vs_plain_exact_within_0p5"] = float(np.mean([within(maps[i], plain_maps[i])
                                                           for i in common]))
 
+    # fusion of the timed views into the voxel cloud (dense.rs:542-651), timed
+    # beside the depth metric: voxel edge from a sample of ground-truth points
+    # standing in for the sparse cloud, Standard-profile point budget
+    fusion = None
+    if not args.no_fusion:
+        pts = []
+        for (i, _, _) in timed[::max(1, len(timed) // 16)]:
+            v = surv.views[i]
+            yy, xx = np.mgrid[0:v.h:97, 0:v.w:97]
+            d = v.depth_gt[yy, xx]
+            q = np.stack([(xx + 0.5 - v.cx) / v.f * d, (yy + 0.5 - v.cy) / v.f * d, d], -1)
+            pts.append((q.reshape(-1, 3) @ np.asarray(v.R)) + v.C)
+        pts = np.concatenate(pts)
+        voxel = capi.fusion_voxel(pts[:, 0], pts[:, 1], [j[2] for j in jobs], spec.focal,
+                                  max(spec.width, spec.height), spec.width, spec.height,
+                                  5_000_000)
+        refs = [i for (i, _, _) in timed]
+        ctx.synchronize()
+        tf = time.perf_counter()
+        xyz, rgb = sv.fuse(refs, [cons_nbrs[i] for i in refs], voxel, rule=capi.MVS_CONS_REFERENCE)
+        dtf = time.perf_counter() - tf
+        fusion = dict(voxel_m=voxel, points=int(len(xyz)), seconds=dtf,
+                      rule="dense.rs:585-612 (relative depth < 1% in >= 2 of the K+1 nearest)",
+                      ply_bytes=int(len(xyz)) * 15)
+        log(f"fusion: {len(xyz)} voxels of {voxel:.4f} m in {dtf:.2f} s")
+
     cpu = None
     if not args.no_cpu:
         cw, ch = (int(a) for a in args.cpu_crop.split("x"))
@@ -344,6 +371,7 @@ def main():
         refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
         graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
+        fusion=fusion,
     )
     print(json.dumps(line), flush=True)
     sv.close()
