@@ -53,6 +53,7 @@ struct ViewDev {
     int32_t w = 0, h = 0;
     DevBuf gray;
     DevBuf quad;   // corner-packed 8-bit plane (8-bit inputs)
+    DevBuf quadh;  // corner-packed f16 plane with column differences (8-bit inputs)
     DevBuf quadf;  // corner-packed grey plane
     bool has_quad = false;
     DevBuf rgb;    // w*h*3 colour for the fused cloud (optional)
@@ -116,7 +117,7 @@ struct mvs_ctx {
     // host copies that must outlive the asynchronous uploads of a call
     std::vector<RefJob> host_jobs;
     std::vector<std::vector<SrcCam>> host_tables;
-    // schedule: forced (split > 0) or tuned per class
+    // schedule: forced (split != 0) or tuned per class
     Schedule forced{0, 0, 0};
     bool autotune = true;
     std::map<TuneKey, Schedule> tuned;
@@ -259,6 +260,10 @@ int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v) {
         launch_quad_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride,
                             dv.quad.as<uint32_t>(), c->st);
         HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, dv.quadh.ensure(n * sizeof(uint2)));
+        launch_quadh_from_u8(c->up_u8.as<uint8_t>(), v->w, v->h, v->stride,
+                             dv.quadh.as<uint2>(), c->st);
+        HIP_TRY(c, hipGetLastError());
         dv.has_quad = true;
     } else {
         dv.has_quad = false;
@@ -340,6 +345,7 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
     split_f64(s.k2, c.k2, c.k2_lo);
     c.gray = s.gray.as<float>();
     c.quad = s.has_quad ? s.quad.as<uint32_t>() : nullptr;
+    c.quadh = s.has_quad ? s.quadh.as<uint2>() : nullptr;
     c.quadf = s.quadf.as<float4>();
     c.w = s.w;
     c.h = s.h;
@@ -397,7 +403,8 @@ PmArgs base_args(mvs_ctx *c, const KClass &k) {
 // items of job j in this launch.
 template <class F>
 int32_t grid_for(mvs_ctx *c, const KClass &k, bool persist, F items_of) {
-    const int64_t pix = 64 / (k.srcl * k.sch.split), chunk = pix * k.sch.gpc;
+    const int64_t pix = k.sch.split < 0 ? 64 : 64 / (k.srcl * k.sch.split),
+                  chunk = pix * k.sch.gpc;
     int64_t chunks = 0, seq = 0;
     for (int32_t j = 0; j < k.count; ++j) {
         const int64_t n = (items_of(j) + chunk - 1) / chunk;
@@ -419,35 +426,42 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
         grid = grid_for(c, k, persist, [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
     else
         grid = grid_for(c, k, persist, [&](int32_t j) { return k.caps[(size_t)j]; });
-    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole != 0, persist, k.fast != 0, a, grid,
+    if (launch_pm(mode, taps, k.srcl, k.sch.split, k.pinhole, persist, k.fast != 0, a, grid,
                   c->st))
         return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
     HIP_TRY(c, hipGetLastError());
     return MVS_OK;
 }
 
-// Candidate schedules of a class: tap split x chunk size.
-std::vector<Schedule> candidates(int32_t taps, int32_t srcl) {
+// Candidate schedules of a class: tap split x chunk size, and for the fast
+// arithmetic on undistorted cameras the one-lane-per-pixel kernels (split -1).
+std::vector<Schedule> candidates(const KClass &k) {
     int32_t sp[4];
-    const int ns = splits_for(taps, sp);
+    const int ns = splits_for(k.taps, sp);
     std::vector<Schedule> cand;
     for (int i = 0; i < ns; ++i) {
-        if (srcl * sp[i] > 64) continue;
+        if (k.srcl * sp[i] > 64) continue;
         cand.push_back(Schedule{sp[i], 1, 0});   // static grid
         cand.push_back(Schedule{sp[i], 1, 16});  // persistent
         cand.push_back(Schedule{sp[i], 4, 16});
+    }
+    if (k.fast && k.pinhole == 2) {
+        cand.push_back(Schedule{-1, 1, 0});
+        cand.push_back(Schedule{-1, 1, 16});
+        cand.push_back(Schedule{-1, 1, 8});
     }
     return cand;
 }
 
 // Time the half-sweep (it, parity) of the first job of a class per candidate
-// schedule on a copy of its state, keep the fastest. All schedules give
-// bit-identical results; only their speed differs, and it differs between the
-// first iterations (random planes, scattered gathers) and the converged ones,
-// so the choice is made per iteration.
+// schedule on a copy of its state, keep the fastest. With the exact arithmetic
+// all schedules give bit-identical results; with the fast arithmetic they sum
+// in different orders, so costs differ within the fast tolerance. Speed
+// differs between the first iterations (random planes, scattered gathers) and
+// the converged ones, so the choice is made per iteration.
 int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_t it,
                    int32_t parity) {
-    const std::vector<Schedule> cand = candidates(taps, k.srcl);
+    const std::vector<Schedule> cand = candidates(k);
     if (cand.empty()) return fail(c, MVS_ERR_ARG, "no schedule for this class");
     if (cand.size() == 1) {
         k.sch = cand[0];
@@ -497,8 +511,8 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
     return MVS_OK;
 }
 
-Schedule forced_schedule(mvs_ctx *c, int32_t taps, int32_t srcl) {
-    const std::vector<Schedule> cand = candidates(taps, srcl);
+Schedule forced_schedule(mvs_ctx *c, const KClass &k) {
+    const std::vector<Schedule> cand = candidates(k);
     Schedule s = c->forced;
     if (s.gpc <= 0) s.gpc = 8;
     bool ok = false;
@@ -548,7 +562,7 @@ int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32
 
 int enqueue_sweeps(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
     for (const SweepLaunch &L : ls) {
-        if (launch_pm(PM_SWEEP, L.taps, L.srcl, L.split, L.pinhole != 0, L.persist != 0,
+        if (launch_pm(PM_SWEEP, L.taps, L.srcl, L.split, L.pinhole, L.persist != 0,
                       L.fast != 0, L.a,
                       L.grid, c->st))
             return fail(c, MVS_ERR_ARG, "kernel class not instantiated");
@@ -648,11 +662,14 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
         c->host_tables.emplace_back();
         std::vector<SrcCam> &tab = c->host_tables.back();
-        bool pinhole = true, quads = true;
+        bool pinhole = true, quads = true, lane_size = true;
         for (const ViewDev *sv : js.srcs) {
             tab.push_back(make_src(ref, *sv));
             pinhole = pinhole && tab.back().pinhole_exact;
             quads = quads && sv->has_quad;
+            // the one-lane-per-pixel kernels address a source with 32-bit byte
+            // offsets into its 8-byte-per-pixel plane
+            lane_size = lane_size && (int64_t)sv->w * sv->h <= (1ll << 28);
         }
         HIP_TRY(c, hipMemcpyAsync(wk.srcs.p, tab.data(), sizeof(SrcCam) * (size_t)K,
                                   hipMemcpyHostToDevice, c->st));
@@ -686,11 +703,16 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         job.stride = prm.stride;
         job.half = (prm.taps / 2) * prm.stride;
         job.pinhole = pinhole ? 1 : 0;
+        job.ray_step = (float)((double)prm.stride / ref.f);
         // the fast kernels read the corner-packed 8-bit planes; views given
         // as f32 grey run the exact kernels
         const int32_t fast = (prm.mode == MVS_MODE_FAST && quads) ? 1 : 0;
         const int32_t steps = n_steps < 0 ? 2 * prm.iterations : n_steps;
-        ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), pinhole ? 1 : 0, fast);
+        // 2: the reference is undistorted as well (its window rays are affine)
+        // and 8-bit (its grey is a byte over 255)
+        const int32_t ph =
+            pinhole ? (ref.k1 == 0.0 && ref.k2 == 0.0 && ref.has_quad && lane_size ? 2 : 1) : 0;
+        ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), ph, fast);
     }
 
     // group jobs by kernel class; jobs of a class are contiguous in the table
@@ -741,7 +763,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         }
         if (first_step < 0) {
             for (KClass &k : classes) {
-                k.sch = candidates(k.taps, k.srcl)[0];
+                k.sch = candidates(k)[0];
                 PmArgs a = base_args(c, k);
                 a.evals = evals;
                 TRY(pm_launch(c, PM_EVAL_CURRENT, k.taps, k, a));
@@ -769,8 +791,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         for (KClass &k : classes) {
             if (s - s0 >= k.steps) continue;
             active.push_back(&k);
-            if (c->forced.split > 0) {
-                k.sch = forced_schedule(c, k.taps, k.srcl);
+            if (c->forced.split != 0) {
+                k.sch = forced_schedule(c, k);
             } else {
                 TuneKey key(k.taps, k.srcl, k.pinhole, k.size, it, k.fast);
                 auto f = c->tuned.find(key);
@@ -781,7 +803,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
                     c->tuned[key] = k.sch;
                     known = false;
                 } else {
-                    k.sch = candidates(k.taps, k.srcl)[0];
+                    k.sch = candidates(k)[0];
                 }
             }
         }
@@ -987,6 +1009,7 @@ void mvs_destroy(mvs_ctx *c) {
         v.gray.release();
         v.quadf.release();
         v.quad.release();
+        v.quadh.release();
     }
     (void)hipStreamDestroy(c->st);
     delete c;
@@ -1001,7 +1024,7 @@ int mvs_synchronize(mvs_ctx *c) {
 
 int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune) {
-    if (!c || split < 0 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
+    if (!c || split < -1 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
         return MVS_ERR_ARG;
     c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;
@@ -1096,8 +1119,7 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     int rc = run_batch(c, jobs, 0, 0, nullptr, &classes);
     if (rc) return abort_call(c, rc);
     KClass k = classes[0];
-    k.sch = c->forced.split > 0 ? forced_schedule(c, prm->taps, k.srcl)
-                                : candidates(prm->taps, k.srcl)[0];
+    k.sch = c->forced.split != 0 ? forced_schedule(c, k) : candidates(k)[0];
     PmArgs a = base_args(c, k);
     a.px = c->px.as<int32_t>();
     a.py = c->py.as<int32_t>();
@@ -1242,6 +1264,7 @@ void mvs_survey_destroy(mvs_survey *s) {
         v.rgb.release();
         v.quadf.release();
         v.quad.release();
+        v.quadh.release();
     }
     for (auto &m : s->maps) {
         m.depth.release();

@@ -93,6 +93,20 @@ typedef const __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ gfloat *as_global(const float *p) { return (gfloat *)p; }
 typedef const __attribute__((address_space(1))) uint32_t gu32;
 __device__ __forceinline__ gu32 *as_global(const uint32_t *p) { return (gu32 *)p; }
+// A copy of *p read through the constant address space, for tables that no
+// lane writes while the kernel runs: at a wave-uniform address the reads
+// become scalar loads shared by the wave.
+template <class T>
+__device__ __forceinline__ T scalar_load(const T *p) {
+    typedef const __attribute__((address_space(4))) uint32_t cu32;
+    static_assert(sizeof(T) % 4 == 0, "word-sized tables only");
+    T v;
+    uint32_t *d = reinterpret_cast<uint32_t *>(&v);
+    const cu32 *s = (const cu32 *)p;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = s[i];
+    return v;
+}
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) f32x4 gf32x4;
 __device__ __forceinline__ gf32x4 *as_global(const float4 *p) { return (gf32x4 *)p; }

@@ -22,6 +22,10 @@ struct SrcCam {
     // 8-bit inputs only (else null): per pixel the bytes of (x, y), (x+1, y),
     // (x, y+1), (x+1, y+1), low to high — a bilinear sample's four corners
     const uint32_t *quad;
+    // 8-bit inputs only: per pixel {g00 | (g10 - g00) << 16, g01 | (g11 - g01)
+    // << 16} as f16 (integers, exact): a bilinear sample's corners and column
+    // differences in one 8-byte load
+    const uint2 *quadh;
     // per pixel the grey of the same four corners (every view)
     const float4 *quadf;
     int32_t w, h;
@@ -46,6 +50,8 @@ struct RefJob {
     int32_t stride;       // tap spacing
     int32_t half;         // window half footprint, pixels
     int32_t pinhole;      // every source has pinhole_exact set
+    float ray_step;       // tap spacing / f of the reference camera
+    int32_t pad;
 };
 
 // Arguments of one batched PatchMatch launch (k_pm in mvs_pm_impl.h).
@@ -86,6 +92,8 @@ void launch_gray_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t strid
 void launch_quad_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride,
                          uint32_t *dst, hipStream_t st);
 void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st);
+void launch_quadh_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride, uint2 *dst,
+                          hipStream_t st);
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                  float2 *rays, hipStream_t st);
 // write_hyp = 0 keeps a caller-provided state and only rebuilds mask + lists.
@@ -98,9 +106,15 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // persist: resident workgroups with atomic work fetch (half-sweeps only);
 // otherwise one workgroup per band position. fast: the contracted-arithmetic
-// kernels (MVS_MODE_FAST) instead of the reference-rounding ones.
-int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
-              bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
+// kernels (MVS_MODE_FAST) instead of the reference-rounding ones. pinhole: 0
+// some source distorted, 1 undistorted sources, 2 undistorted sources and
+// reference. split = -1 selects the one-lane-per-pixel kernels (fast and
+// pinhole = 2 only).
+int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
+              bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
+// One lane per reference pixel (mvs_pm_lane.hip); 64 pixels per group.
+int launch_pml(int mode, int32_t taps, bool persist, const PmArgs &a, int32_t grid,
+               hipStream_t st);
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st);
 int launch_pm_t7(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,

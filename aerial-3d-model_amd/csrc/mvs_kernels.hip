@@ -56,6 +56,24 @@ __global__ void k_quad_from_u8(const uint8_t *__restrict__ src, int32_t w, int32
              ((uint32_t)r1[x1] << 24);
 }
 
+// Corner-packed f16 plane for the one-lane-per-pixel kernels: pixel (x, y)
+// holds b(x, y), b(x+1, y) - b(x, y), b(x, y+1), b(x+1, y+1) - b(x, y+1) as f16
+// (small integers, exact; clamped at the last column/row), so one 8-byte load
+// gives a bilinear sample's corners in the form its two row interpolations use.
+__global__ void k_quadh_from_u8(const uint8_t *__restrict__ src, int32_t w, int32_t h,
+                                int32_t stride, uint2 *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    const int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    const int32_t x1 = x + 1 < w ? x + 1 : x, y1 = y + 1 < h ? y + 1 : y;
+    const uint8_t *r0 = src + (int64_t)y * stride, *r1 = src + (int64_t)y1 * stride;
+    auto h16 = [](int v) -> uint32_t {
+        return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(float)v);
+    };
+    dst[i] = make_uint2(h16(r0[x]) | (h16((int)r0[x1] - (int)r0[x]) << 16),
+                        h16(r1[x]) | (h16((int)r1[x1] - (int)r1[x]) << 16));
+}
+
 // Corner-packed grey plane: pixel (x, y) holds the grey of (x, y), (x+1, y),
 // (x, y+1), (x+1, y+1) (clamped at the last column/row), so one 16-byte load
 // fetches a bilinear sample's four corners with their exact values.
@@ -254,6 +272,13 @@ void launch_quad_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t strid
                        w, h, stride, dst);
 }
 
+void launch_quadh_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride, uint2 *dst,
+                          hipStream_t st) {
+    const int64_t n = (int64_t)w * h;
+    hipLaunchKernelGGL(k_quadh_from_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src,
+                       w, h, stride, dst);
+}
+
 void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st) {
     const int64_t n = (int64_t)w * h;
     hipLaunchKernelGGL(k_quadf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gray, w, h,
@@ -278,17 +303,21 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
                        write_hyp);
 }
 
-int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, bool pinhole, bool persist,
-              bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
+int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
+              bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
+    if (split < 0) {
+        if (!fast || pinhole != 2) return -1;
+        return launch_pml(mode, taps, persist, a, grid, st);
+    }
     if (fast) {
-        if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole, persist, a, grid, st);
-        if (taps == 7) return launch_pm_t7f(mode, srcl, split, pinhole, persist, a, grid, st);
-        if (taps == 11) return launch_pm_t11f(mode, srcl, split, pinhole, persist, a, grid, st);
+        if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole != 0, persist, a, grid, st);
+        if (taps == 7) return launch_pm_t7f(mode, srcl, split, pinhole != 0, persist, a, grid, st);
+        if (taps == 11) return launch_pm_t11f(mode, srcl, split, pinhole != 0, persist, a, grid, st);
         return -1;
     }
-    if (taps == 5) return launch_pm_t5(mode, srcl, split, pinhole, persist, a, grid, st);
-    if (taps == 7) return launch_pm_t7(mode, srcl, split, pinhole, persist, a, grid, st);
-    if (taps == 11) return launch_pm_t11(mode, srcl, split, pinhole, persist, a, grid, st);
+    if (taps == 5) return launch_pm_t5(mode, srcl, split, pinhole != 0, persist, a, grid, st);
+    if (taps == 7) return launch_pm_t7(mode, srcl, split, pinhole != 0, persist, a, grid, st);
+    if (taps == 11) return launch_pm_t11(mode, srcl, split, pinhole != 0, persist, a, grid, st);
     return -1;
 }
 

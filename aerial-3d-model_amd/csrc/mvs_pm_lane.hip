@@ -1,0 +1,404 @@
+// PatchMatch with one lane per reference pixel, for the fast arithmetic
+// (MVS_MODE_FAST) on undistorted cameras: the cost of dense.rs:181-263 and the
+// half-sweep of :413-467, with the rounding rules of the fast source-lane
+// kernels (contracted multiply-adds, hardware reciprocals, sums shifted by the
+// window's first sample), laid out so that every gather instruction reads one
+// source image: the wavefront's 64 lanes hold 64 pixels of one reference view
+// and step through the sources together, so one gather touches the few cache
+// lines under 64 neighbouring pixels rather than lines of eight images.
+//
+// Window geometry. For an undistorted reference camera the window rays are
+// affine in the tap offsets, r = r_c + dx e_x + dy e_y (e = tap spacing / f).
+// On the plane n . X = c (c = d n . r_c, dense.rs:187-190) the window point of
+// ray r is X = r c / (n . r), and in the source camera
+//     X_s = R X + t = (c / (n . r)) (R + t n^T / c) r = dq H r,
+// dq = c / (n . r) the window depth of dense.rs:201-211. Scaled by the centre
+// tap's depth dq_c, the source point is affine in (dx, dy):
+//     (dq_c / dq) X_s = X_c + dx dq_c H e_x + dy dq_c H e_y,
+// X_c the centre tap's source point (formed as the source-lane kernels form
+// every tap's, for their accuracy), dq_c H e = R e dq_c + t (n . e) / (n . r_c).
+// Three multiply-adds per tap give it; its third coordinate has the sign of
+// X_s.z (dq > 0), and the pixel is f times the first two over the third plus
+// the principal point, added last so that its rounding stays that of one add.
+// dense.rs requires |n . r| >= 1e-6 and dq > 0 at every tap; n . r is affine
+// over the window, so both hold at every tap when they hold at the four
+// corner taps.
+#include "mvs_pm_impl.h"
+
+namespace amvs {
+namespace {
+
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// Word 3 of a raw buffer resource on gfx9-family chips (32-bit data format,
+// no swizzle): byte offsets, reads past num_records return zero.
+constexpr int32_t kBufferWord3 = 0x00020000;
+
+template <int TAPS, int MODE, bool PERSIST>
+__global__ __launch_bounds__(64) void k_pml(PmArgs a) {
+    using namespace dev;
+    constexpr int N = TAPS * TAPS;
+    constexpr int H = TAPS / 2;
+    // each lane's reference window as 8-bit values less an integer near their
+    // mean (exact in f16), tap-major so that the per-tap reads of a wavefront
+    // are free of bank conflicts; a lane reads only what it wrote, so no
+    // barrier is needed
+    __shared__ _Float16 Ah[N][64];
+
+    const int lane = threadIdx.x;
+    const int chunk = 64 * a.gpc;
+    const int32_t nj = a.njobs;
+    const int home = blockIdx.x % kNumXcd;
+    unsigned long long n_evals = 0;
+
+    for (int attempt = 0; attempt < (PERSIST ? kNumXcd : 1); ++attempt) {
+        const int xcd = (home + attempt) % kNumXcd;
+        const int32_t *bp = a.prefix + (int64_t)(1 + xcd) * (nj + 1);
+        const int32_t total = bp[nj];
+        bool first = true;
+        while (true) {
+            int32_t v = 0;
+            if (PERSIST) {
+                if (lane == 0) v = atomicAdd(&a.counters[xcd], 1);
+                v = __builtin_amdgcn_readfirstlane(__shfl(v, 0));
+            } else {
+                if (!first) break;
+                first = false;
+                v = (int32_t)(blockIdx.x / kNumXcd);
+            }
+            if (v >= total) break;
+            // job of sequence position v: the last j with bp[j] <= v
+            int32_t jl = 0, jh = nj - 1;
+            while (jl < jh) {
+                const int32_t mid = (jl + jh + 1) >> 1;
+                if (bp[mid] <= v)
+                    jl = mid;
+                else
+                    jh = mid - 1;
+            }
+            const int32_t jj = __builtin_amdgcn_readfirstlane(jl);
+            const int32_t njc = a.prefix[jj + 1] - a.prefix[jj];
+            const int32_t c = a.prefix[jj] + njc * xcd / kNumXcd + (v - bp[jj]);
+            const RefJob job = scalar_load(a.jobs + jj);
+            const int64_t items = pm_items(job, a, MODE);
+            const int64_t base = (int64_t)(c - a.prefix[jj]) * chunk;
+            const int32_t w = job.w, h = job.h, K = job.nsrc, stride = job.stride,
+                          half = job.half;
+            const float rstep = job.ray_step;
+
+            for (int b = 0; b < a.gpc; ++b) {
+                const int64_t item0 = base + (int64_t)b * 64;
+                if (item0 >= items) break;  // uniform
+                const int64_t item = item0 + lane;
+                const bool have = item < items;
+                int32_t x = 0, y = 0;
+                if (have) {
+                    int32_t pix;
+                    if (MODE == MODE_EVAL_CURRENT) {
+                        const int64_t c0 = job.counts[0];
+                        pix = item < c0 ? job.list[item] : job.list[job.list_cap + (item - c0)];
+                    } else if (MODE == MODE_SWEEP) {
+                        pix = job.list[(int64_t)a.parity * job.list_cap + item];
+                    } else {
+                        pix = -1;
+                        x = a.px[item];
+                        y = a.py[item];
+                    }
+                    if (pix >= 0) {
+                        y = pix / w;
+                        x = pix - y * w;
+                    }
+                }
+                const int64_t pidx = (int64_t)y * w + x;
+                const bool inside =
+                    have && x >= half && y >= half && x < w - half && y < h - half;
+
+                // reference window statistics (dense.rs:213, mean_sd :266-271,
+                // sums in tap order). The window's grey is the 8-bit value over
+                // 255 (to the table's rounding), so its bytes b are rint(255 g);
+                // the costs use a = b - mi (mi an integer near the byte mean)
+                // and the exact correction delta = mean(b) - mi.
+                float mr = 0.0f, sr = 0.0f, delta = 0.0f;
+                if (inside) {
+                    float s = 0.0f, sb = 0.0f;
+#pragma unroll 1
+                    for (int ty = 0; ty < TAPS; ++ty) {
+                        const float *row = job.gray + pidx + (int64_t)((ty - H) * stride) * w;
+#pragma unroll
+                        for (int tx = 0; tx < TAPS; ++tx) {
+                            const float g = row[(tx - H) * stride];
+                            s = s + g;
+                            sb += rintf(g * 255.0f);
+                        }
+                    }
+                    mr = s / (float)N;
+                    const float mi = rintf(sb * (1.0f / (float)N));
+                    delta = (sb - (float)N * mi) * (1.0f / (float)N);
+                    float var = 0.0f;
+#pragma unroll 1
+                    for (int ty = 0; ty < TAPS; ++ty) {
+                        const float *row = job.gray + pidx + (int64_t)((ty - H) * stride) * w;
+#pragma unroll
+                        for (int tx = 0; tx < TAPS; ++tx) {
+                            const float g = row[(tx - H) * stride];
+                            const float d = g - mr;
+                            var = var + d * d;
+                            Ah[ty * TAPS + tx][lane] = (_Float16)(rintf(g * 255.0f) - mi);
+                        }
+                    }
+                    var = var / (float)N;
+                    sr = sqrtf(var);
+                }
+                const bool pixel_ok = inside && !(sr < job.texture_min);
+                const float2 rc = pixel_ok ? job.rays[pidx] : make_float2(0.0f, 0.0f);
+                const float nsr = (float)N * (sr * 255.0f);  // N sd(b)
+
+                // Cost of plane (d, n) at this lane's pixel. Every lane calls it;
+                // lanes without a hypothesis to score pass act = false.
+                auto eval = [&](bool act, float d, float n0, float n1, float n2) -> float {
+                    act = act && pixel_ok;
+                    if (act) ++n_evals;
+                    // dense.rs:187-190: c = n . (ray d)
+                    const float xp0 = rc.x * d, xp1 = rc.y * d;
+                    float cc = n0 * xp0 + n1 * xp1;
+                    cc = cc + n2 * d;
+                    // |n . r| >= 1e-6 and c / (n . r) > 0 at the window corners
+                    const float dc = fmaf(n0, rc.x, fmaf(n1, rc.y, n2));
+                    const float ex = n0 * (rstep * (float)H), ey = n1 * (rstep * (float)H);
+                    bool plane_ok = true;
+#pragma unroll
+                    for (int cr = 0; cr < 4; ++cr) {
+                        const float den = dc + ((cr & 1) ? ex : -ex) + ((cr & 2) ? ey : -ey);
+                        plane_ok = plane_ok && fabsf(den) >= 1e-6f && cc * den > 0.0f;
+                    }
+                    act = act && plane_ok;
+                    // window depth at the centre ray and the source-space steps
+                    // per tap column / row scaled by it (see the file comment)
+                    const float rden = act ? __builtin_amdgcn_rcpf(dc) : 0.0f;
+                    const float dqc = cc * rden;
+                    const float pc0 = rc.x * dqc, pc1 = rc.y * dqc;
+                    const float sx = dqc * rstep;
+                    const float tnx = (n0 * rstep) * rden, tny = (n1 * rstep) * rden;
+                    float m1 = INFINITY, m2 = INFINITY;
+                    if (act) {
+                        for (int k = 0; k < K; ++k) {
+                            const SrcCam S = scalar_load(job.srcs + k);
+                            const float F = S.f, CXh = S.cx - 0.5f, CYh = S.cy - 0.5f;
+                            // centre tap in the source camera (dense.rs:222-226)
+                            // and the steps dq_c H e_x, dq_c H e_y, focal
+                            // folded into the first two rows
+                            float B0[3], B1[3], B2[3];
+#pragma unroll
+                            for (int i = 0; i < 3; ++i) {
+                                const float xc = fmaf(S.R[3 * i], pc0,
+                                                      fmaf(S.R[3 * i + 1], pc1,
+                                                           fmaf(S.R[3 * i + 2], dqc, S.t[i])));
+                                const float c1 = fmaf(S.t[i], tnx, S.R[3 * i] * sx);
+                                const float c2 = fmaf(S.t[i], tny, S.R[3 * i + 1] * sx);
+                                const float sc = i < 2 ? F : 1.0f;
+                                B0[i] = sc * xc;
+                                B1[i] = sc * c1;
+                                B2[i] = sc * c2;
+                            }
+                            const float wm1 = (float)(S.w - 1), hm1 = (float)(S.h - 1);
+                            // gray_at (dense.rs:69-85) succeeds at every tap iff
+                            // it does at the corner taps: the third coordinate is
+                            // affine (positive at the corners, positive on the
+                            // window), and a projective map that keeps it
+                            // positive sends the window's rectangle onto the
+                            // quadrilateral of the corner images, so the taps'
+                            // bounding box is the corners'. NaN fails.
+                            bool ok = true;
+#pragma unroll
+                            for (int cr = 0; cr < 4; ++cr) {
+                                const float fdx = (cr & 1) ? (float)H : -(float)H;
+                                const float fdy = (cr & 2) ? (float)H : -(float)H;
+                                const float P0 = fmaf(fdx, B1[0], fmaf(fdy, B2[0], B0[0]));
+                                const float P1 = fmaf(fdx, B1[1], fmaf(fdy, B2[1], B0[1]));
+                                const float P2 = fmaf(fdx, B1[2], fmaf(fdy, B2[2], B0[2]));
+                                const float ip = __builtin_amdgcn_rcpf(P2);
+                                const float fx = fmaf(P0, ip, CXh), fy = fmaf(P1, ip, CYh);
+                                ok = ok && (P2 > 0.0f) && fx >= 0.0f && fy >= 0.0f && fx < wm1 &&
+                                     fy < hm1;
+                            }
+                            float cs = 2.0f;  // a failing source scores 2.0
+                            if (ok) {
+                                const uint32_t sw = (uint32_t)S.w;
+                                // the plane as a buffer: 32-bit offsets, and the
+                                // hardware range check as a second bound; the
+                                // resource is wave-uniform (scalar registers)
+                                // (readfirstlane returns a signed int: each half
+                                // goes through uint32_t, never sign-extended)
+                                const uint64_t qa = (uint64_t)(uintptr_t)S.quadh;
+                                const uint64_t qlo =
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
+                                const uint64_t qhi =
+                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
+                                const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
+                                    (void *)(uintptr_t)(qlo | (qhi << 32)), 0,
+                                    __builtin_amdgcn_readfirstlane(S.w * S.h * 8), kBufferWord3);
+                                float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
+                                // one tap row per trip: a row's gathers are in
+                                // flight together while the registers they hold
+                                // leave room for several wavefronts per SIMD
+#pragma unroll 1
+                                for (int ty = 0; ty < TAPS; ++ty) {
+                                    const float fdy = (float)(ty - H);
+                                    const float Y0 = fmaf(fdy, B2[0], B0[0]);
+                                    const float Y1 = fmaf(fdy, B2[1], B0[1]);
+                                    const float Y2 = fmaf(fdy, B2[2], B0[2]);
+#pragma unroll
+                                    for (int tx = 0; tx < TAPS; ++tx) {
+                                        const float fdx = (float)(tx - H);
+                                        const float P0 = tx == H ? Y0 : fmaf(fdx, B1[0], Y0);
+                                        const float P1 = tx == H ? Y1 : fmaf(fdx, B1[1], Y1);
+                                        const float P2 = tx == H ? Y2 : fmaf(fdx, B1[2], Y2);
+                                        const float ip = __builtin_amdgcn_rcpf(P2);
+                                        // inside the image up to rounding; the
+                                        // clamp (NaN -> 0) keeps every address
+                                        // in bounds
+                                        const float fx =
+                                            __builtin_amdgcn_fmed3f(fmaf(P0, ip, CXh), 0.0f, wm1);
+                                        const float fy =
+                                            __builtin_amdgcn_fmed3f(fmaf(P1, ip, CYh), 0.0f, hm1);
+                                        const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+                                        const float ax = __builtin_amdgcn_fractf(fx),
+                                                    ay = __builtin_amdgcn_fractf(fy);
+                                        // corners and column differences, f16
+                                        const u32x2 qv = __builtin_amdgcn_raw_buffer_load_b64(
+                                            Qh, (__umul24(y0, sw) + x0) * 8u, 0, 0);
+                                        const uint32_t q0 = qv.x, q1 = qv.y;
+                                        const h2 r0 = __builtin_bit_cast(h2, q0);
+                                        const h2 r1 = __builtin_bit_cast(h2, q1);
+                                        const float top = fmaf(ax, (float)r0.y, (float)r0.x);
+                                        const float bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                        const float val = fmaf(ay, bot - top, top);
+                                        if (tx == 0 && ty == 0) c0 = val;
+                                        const float dd = val - c0;
+                                        s1 += dd;
+                                        s2 = fmaf(dd, dd, s2);
+                                        sa = fmaf((float)Ah[ty * TAPS + tx][lane], dd, sa);
+                                    }
+                                }
+                                // NCC from the shifted sums, in byte units:
+                                // N var = s2 - s1^2 / N and
+                                // cov = sum (b - mean b) dd = sa - delta s1
+                                const float moff = s1 * (1.0f / (float)N);
+                                const float var =
+                                    fmaxf(fmaf(-s1, moff, s2), 0.0f) * (1.0f / (float)N);
+                                const float cov = fmaf(-delta, s1, sa);
+                                const float ss = __builtin_amdgcn_sqrtf(var);
+                                // dense.rs:246-248's 1e-3 grey units are 0.255
+                                // byte units
+                                if (!(ss < 1e-3f * 255.0f))
+                                    cs = 1.0f - cov * __builtin_amdgcn_rcpf(nsr * ss);
+                            }
+                            // best two (dense.rs:255-262)
+                            const float lo = fminf(m1, cs);
+                            m2 = fminf(fmaxf(m1, cs), m2);
+                            m1 = lo;
+                        }
+                    }
+                    float cst = K >= 2 ? (m1 + m2) * 0.5f : m1;
+                    if (!act) cst = 2.0f;
+                    return cst;
+                };
+
+                if (MODE == MODE_COST_LIST) {
+                    const float4 hp = have ? a.hyp_in[item] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
+                    if (have) a.out[item] = cst;
+                } else if (MODE == MODE_EVAL_CURRENT) {
+                    const float4 hp = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
+                    if (have) job.cost[pidx] = cst;
+                } else {
+                    // the ten proposals of dense.rs:427-462 in the reference's
+                    // order (propagation in CAND order, then two random
+                    // refinements), as in the source-lane kernels
+                    float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    float bcost = have ? job.cost[pidx] : 2.0f;
+                    uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
+                    const float span = job.imax - job.imin;
+#pragma unroll 1
+                    for (int k = 0; k < 10; ++k) {
+                        float4 hp;
+                        bool act;
+                        if (k < 8) {
+                            const int dir = k & 3, mag = k < 4 ? 1 : 5;
+                            const int nx = x + (dir == 0 ? -mag : (dir == 1 ? mag : 0));
+                            const int ny = y + (dir == 2 ? -mag : (dir == 3 ? mag : 0));
+                            const bool inb = have && nx >= 0 && ny >= 0 && nx < w && ny < h;
+                            hp = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                            if (inb) hp = job.hyp[(int64_t)ny * w + nx];
+                            const bool dup = hp.x == best.x && hp.y == best.y &&
+                                             hp.z == best.z && hp.w == best.w;
+                            act = inb && !dup;
+                        } else {
+                            float stp = (rnd_next(s) - 0.5f) * a.range;
+                            stp = stp * 0.25f;
+                            stp = stp * span;
+                            const float id = 1.0f / best.x + stp;
+                            hp.x = (id > job.imin * 0.5f && id < job.imax * 1.5f) ? 1.0f / id
+                                                                                  : best.x;
+                            random_normal(s, rc.x, rc.y, best.y, best.z, best.w, 0.25f * a.range,
+                                          hp.y, hp.z, hp.w);
+                            act = have;
+                        }
+                        const float cst = eval(act, hp.x, hp.y, hp.z, hp.w);
+                        if (act && cst < bcost) {
+                            best = hp;
+                            bcost = cst;
+                        }
+                    }
+                    if (have) {
+                        job.hyp[pidx] = best;
+                        job.cost[pidx] = bcost;
+                    }
+                }
+            }
+        }
+    }
+
+    if (a.evals) {
+        unsigned long long e = n_evals;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off);
+        if (lane == 0 && e) atomicAdd(a.evals, e);
+    }
+}
+
+template <int TAPS, int MODE, bool PERSIST>
+void launch_pml_class(const PmArgs &a, int32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_pm_prep, dim3(1), dim3(64), 0, st, a, (int32_t)MODE, 64 * a.gpc);
+    hipLaunchKernelGGL((k_pml<TAPS, MODE, PERSIST>), dim3((unsigned)grid), dim3(64), 0, st, a);
+}
+
+template <int TAPS>
+void pml_taps(int mode, bool persist, const PmArgs &a, int32_t grid, hipStream_t st) {
+    if (mode == MODE_EVAL_CURRENT)
+        launch_pml_class<TAPS, MODE_EVAL_CURRENT, false>(a, grid, st);
+    else if (mode == MODE_SWEEP && persist)
+        launch_pml_class<TAPS, MODE_SWEEP, true>(a, grid, st);
+    else if (mode == MODE_SWEEP)
+        launch_pml_class<TAPS, MODE_SWEEP, false>(a, grid, st);
+    else
+        launch_pml_class<TAPS, MODE_COST_LIST, false>(a, grid, st);
+}
+
+}  // namespace
+
+int launch_pml(int mode, int32_t taps, bool persist, const PmArgs &a, int32_t grid,
+               hipStream_t st) {
+    if (taps == 5)
+        pml_taps<5>(mode, persist, a, grid, st);
+    else if (taps == 7)
+        pml_taps<7>(mode, persist, a, grid, st);
+    else if (taps == 11)
+        pml_taps<11>(mode, persist, a, grid, st);
+    else
+        return -1;
+    return 0;
+}
+
+}  // namespace amvs

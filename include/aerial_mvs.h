@@ -79,7 +79,7 @@ typedef struct {
     int32_t mode;            /* MVS_MODE_EXACT (0): PatchMatch rounds operation for
                               * operation like the reference (bit-identical maps);
                               * MVS_MODE_FAST (1): contracted multiply-adds and
-                              * hardware reciprocals, costs within 1e-5 of exact */
+                              * hardware reciprocals, costs within 2e-5 of exact */
     int32_t reserved[1];     /* zero */
 } mvs_params;
 
@@ -242,7 +242,8 @@ typedef struct {
     int64_t pixels;           /* reference pixels processed */
     int64_t init_evals;       /* of which in the initialisation */
     int64_t sweep_evals;      /* of which in the half-sweeps */
-    int32_t split;            /* schedule of the last batch: lanes per source */
+    int32_t split;            /* schedule of the last batch: lanes per source
+                               * (-1: one lane per pixel) */
     int32_t gpc;              /* and pixel groups per work chunk */
     int64_t refine_launches;  /* Gauss-Newton refinement kernel */
     double refine_ms;
@@ -254,9 +255,12 @@ typedef struct {
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
 
-/* Kernel schedule (no reference counterpart; results do not depend on it):
- * split = lanes sharing one source's window (0 = choose automatically), gpc =
- * pixel groups per work chunk (0 = default), per_cu = persistent workgroups per
+/* Kernel schedule (no reference counterpart; with MVS_MODE_EXACT results do
+ * not depend on it, with MVS_MODE_FAST they vary within the fast tolerance):
+ * split = lanes sharing one source's window (0 = choose automatically; -1 =
+ * one lane per reference pixel, for MVS_MODE_FAST with undistorted reference
+ * and source cameras, other classes take their first candidate), gpc = pixel
+ * groups per work chunk (0 = default), per_cu = persistent workgroups per
  * compute unit (0 = one workgroup per chunk). autotune = 1 times the candidate
  * schedules per kernel class and iteration on the first batch and keeps the
  * fastest. */
@@ -269,8 +273,9 @@ int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t per_cu,
  * and frees the cached graphs. */
 int mvs_set_graphs(mvs_ctx *ctx, int32_t enable);
 /* The autotuner's choices so far, 9 int32 per entry: taps, source lanes,
- * pinhole, log2 pixels, iteration, mode, split, gpc, per_cu. Returns the entry
- * count (writes at most cap entries). */
+ * pinhole (0 some camera distorted, 1 undistorted sources, 2 undistorted
+ * sources and reference), log2 pixels, iteration, mode, split, gpc, per_cu.
+ * Returns the entry count (writes at most cap entries). */
 int32_t mvs_tuned_schedules(mvs_ctx *ctx, int32_t *out, int32_t cap);
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats *out);
 int mvs_reset_stats(mvs_ctx *ctx);
