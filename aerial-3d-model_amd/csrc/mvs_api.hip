@@ -511,6 +511,14 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
     return MVS_OK;
 }
 
+// Schedule of the initial-cost evaluation (one cost per pixel, scattered
+// random planes): the one-lane-per-pixel kernels where they apply, whose
+// gathers each read one source image, else the first source-lane candidate.
+Schedule eval_schedule(const KClass &k) {
+    if (k.fast && k.pinhole == 2) return Schedule{-1, 1, 0};
+    return candidates(k)[0];
+}
+
 Schedule forced_schedule(mvs_ctx *c, const KClass &k) {
     const std::vector<Schedule> cand = candidates(k);
     Schedule s = c->forced;
@@ -763,7 +771,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         }
         if (first_step < 0) {
             for (KClass &k : classes) {
-                k.sch = candidates(k)[0];
+                k.sch = eval_schedule(k);
                 PmArgs a = base_args(c, k);
                 a.evals = evals;
                 TRY(pm_launch(c, PM_EVAL_CURRENT, k.taps, k, a));
