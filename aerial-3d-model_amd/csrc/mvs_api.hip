@@ -434,7 +434,8 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
 }
 
 // Candidate schedules of a class: tap split x chunk size, and for the fast
-// arithmetic on undistorted cameras the one-lane-per-pixel kernels (split -1).
+// arithmetic on undistorted cameras the one-lane-per-pixel kernels (split -1
+// on the f16 corner plane, -2 on the 8-bit one).
 std::vector<Schedule> candidates(const KClass &k) {
     int32_t sp[4];
     const int ns = splits_for(k.taps, sp);
@@ -446,9 +447,10 @@ std::vector<Schedule> candidates(const KClass &k) {
         cand.push_back(Schedule{sp[i], 4, 16});
     }
     if (k.fast && k.pinhole == 2) {
-        cand.push_back(Schedule{-1, 1, 0});
-        cand.push_back(Schedule{-1, 1, 16});
+        cand.push_back(Schedule{-1, 1, 16});  // f16 corner plane
         cand.push_back(Schedule{-1, 1, 8});
+        cand.push_back(Schedule{-2, 1, 16});  // 8-bit corner plane
+        cand.push_back(Schedule{-2, 1, 8});
     }
     return cand;
 }
@@ -513,9 +515,10 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
 
 // Schedule of the initial-cost evaluation (one cost per pixel, scattered
 // random planes): the one-lane-per-pixel kernels where they apply, whose
-// gathers each read one source image, else the first source-lane candidate.
+// gathers each read one source image, on the 8-bit corner plane (scattered
+// gathers favour the smaller plane), else the first source-lane candidate.
 Schedule eval_schedule(const KClass &k) {
-    if (k.fast && k.pinhole == 2) return Schedule{-1, 1, 0};
+    if (k.fast && k.pinhole == 2) return Schedule{-2, 1, 0};
     return candidates(k)[0];
 }
 
@@ -1032,7 +1035,7 @@ int mvs_synchronize(mvs_ctx *c) {
 
 int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune) {
-    if (!c || split < -1 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
+    if (!c || split < -2 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
         return MVS_ERR_ARG;
     c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;

@@ -109,11 +109,12 @@ enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // kernels (MVS_MODE_FAST) instead of the reference-rounding ones. pinhole: 0
 // some source distorted, 1 undistorted sources, 2 undistorted sources and
 // reference. split = -1 selects the one-lane-per-pixel kernels (fast and
-// pinhole = 2 only).
+// pinhole = 2 only) sampling the f16 corner plane, split = -2 the same kernels
+// sampling the 8-bit corner plane.
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
 // One lane per reference pixel (mvs_pm_lane.hip); 64 pixels per group.
-int launch_pml(int mode, int32_t taps, bool persist, const PmArgs &a, int32_t grid,
+int launch_pml(int mode, int32_t taps, bool f16, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st);
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st);

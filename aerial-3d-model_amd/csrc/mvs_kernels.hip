@@ -306,8 +306,8 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (split < 0) {
-        if (!fast || pinhole != 2) return -1;
-        return launch_pml(mode, taps, persist, a, grid, st);
+        if (!fast || pinhole != 2 || split < -2) return -1;
+        return launch_pml(mode, taps, split == -1, persist, a, grid, st);
     }
     if (fast) {
         if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole != 0, persist, a, grid, st);

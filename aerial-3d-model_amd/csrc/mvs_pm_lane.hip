@@ -34,7 +34,10 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // no swizzle): byte offsets, reads past num_records return zero.
 constexpr int32_t kBufferWord3 = 0x00020000;
 
-template <int TAPS, int MODE, bool PERSIST>
+// F16: samples from the 8-byte f16 corner plane (fewer ALU operations per
+// sample); otherwise from the 4-byte 8-bit corner plane (half the bytes per
+// gather). Both give the same values.
+template <int TAPS, int MODE, bool PERSIST, bool F16>
 __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
@@ -229,14 +232,17 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 // resource is wave-uniform (scalar registers)
                                 // (readfirstlane returns a signed int: each half
                                 // goes through uint32_t, never sign-extended)
-                                const uint64_t qa = (uint64_t)(uintptr_t)S.quadh;
+                                constexpr uint32_t QB = F16 ? 8u : 4u;  // bytes per pixel
+                                const uint64_t qa = F16 ? (uint64_t)(uintptr_t)S.quadh
+                                                        : (uint64_t)(uintptr_t)S.quad;
                                 const uint64_t qlo =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
                                 const uint64_t qhi =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
                                 const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
                                     (void *)(uintptr_t)(qlo | (qhi << 32)), 0,
-                                    __builtin_amdgcn_readfirstlane(S.w * S.h * 8), kBufferWord3);
+                                    __builtin_amdgcn_readfirstlane(S.w * S.h * (int)QB),
+                                    kBufferWord3);
                                 float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
                                 // one tap row per trip: a row's gathers are in
                                 // flight together while the registers they hold
@@ -264,14 +270,28 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                         const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
                                         const float ax = __builtin_amdgcn_fractf(fx),
                                                     ay = __builtin_amdgcn_fractf(fy);
-                                        // corners and column differences, f16
-                                        const u32x2 qv = __builtin_amdgcn_raw_buffer_load_b64(
-                                            Qh, (__umul24(y0, sw) + x0) * 8u, 0, 0);
-                                        const uint32_t q0 = qv.x, q1 = qv.y;
-                                        const h2 r0 = __builtin_bit_cast(h2, q0);
-                                        const h2 r1 = __builtin_bit_cast(h2, q1);
-                                        const float top = fmaf(ax, (float)r0.y, (float)r0.x);
-                                        const float bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                        const uint32_t off = (__umul24(y0, sw) + x0) * QB;
+                                        float top, bot;
+                                        if constexpr (F16) {
+                                            // corners and column differences, f16
+                                            const u32x2 qv =
+                                                __builtin_amdgcn_raw_buffer_load_b64(Qh, off, 0, 0);
+                                            const uint32_t q0 = qv.x, q1 = qv.y;
+                                            const h2 r0 = __builtin_bit_cast(h2, q0);
+                                            const h2 r1 = __builtin_bit_cast(h2, q1);
+                                            top = fmaf(ax, (float)r0.y, (float)r0.x);
+                                            bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                        } else {
+                                            // the four corners as bytes
+                                            const uint32_t qd =
+                                                __builtin_amdgcn_raw_buffer_load_b32(Qh, off, 0, 0);
+                                            const float g00 = (float)(qd & 0xffu);
+                                            const float g10 = (float)((qd >> 8) & 0xffu);
+                                            const float g01 = (float)((qd >> 16) & 0xffu);
+                                            const float g11 = (float)(qd >> 24);
+                                            top = fmaf(ax, g10 - g00, g00);
+                                            bot = fmaf(ax, g11 - g01, g01);
+                                        }
                                         const float val = fmaf(ay, bot - top, top);
                                         if (tx == 0 && ty == 0) c0 = val;
                                         const float dd = val - c0;
@@ -368,34 +388,38 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     }
 }
 
-template <int TAPS, int MODE, bool PERSIST>
+template <int TAPS, int MODE, bool PERSIST, bool F16>
 void launch_pml_class(const PmArgs &a, int32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_pm_prep, dim3(1), dim3(64), 0, st, a, (int32_t)MODE, 64 * a.gpc);
-    hipLaunchKernelGGL((k_pml<TAPS, MODE, PERSIST>), dim3((unsigned)grid), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((k_pml<TAPS, MODE, PERSIST, F16>), dim3((unsigned)grid), dim3(64), 0, st,
+                       a);
 }
 
-template <int TAPS>
+template <int TAPS, bool F16>
 void pml_taps(int mode, bool persist, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (mode == MODE_EVAL_CURRENT)
-        launch_pml_class<TAPS, MODE_EVAL_CURRENT, false>(a, grid, st);
+        launch_pml_class<TAPS, MODE_EVAL_CURRENT, false, F16>(a, grid, st);
     else if (mode == MODE_SWEEP && persist)
-        launch_pml_class<TAPS, MODE_SWEEP, true>(a, grid, st);
+        launch_pml_class<TAPS, MODE_SWEEP, true, F16>(a, grid, st);
     else if (mode == MODE_SWEEP)
-        launch_pml_class<TAPS, MODE_SWEEP, false>(a, grid, st);
+        launch_pml_class<TAPS, MODE_SWEEP, false, F16>(a, grid, st);
     else
-        launch_pml_class<TAPS, MODE_COST_LIST, false>(a, grid, st);
+        launch_pml_class<TAPS, MODE_COST_LIST, false, F16>(a, grid, st);
 }
 
 }  // namespace
 
-int launch_pml(int mode, int32_t taps, bool persist, const PmArgs &a, int32_t grid,
+int launch_pml(int mode, int32_t taps, bool f16, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st) {
     if (taps == 5)
-        pml_taps<5>(mode, persist, a, grid, st);
+        f16 ? pml_taps<5, true>(mode, persist, a, grid, st)
+            : pml_taps<5, false>(mode, persist, a, grid, st);
     else if (taps == 7)
-        pml_taps<7>(mode, persist, a, grid, st);
+        f16 ? pml_taps<7, true>(mode, persist, a, grid, st)
+            : pml_taps<7, false>(mode, persist, a, grid, st);
     else if (taps == 11)
-        pml_taps<11>(mode, persist, a, grid, st);
+        f16 ? pml_taps<11, true>(mode, persist, a, grid, st)
+            : pml_taps<11, false>(mode, persist, a, grid, st);
     else
         return -1;
     return 0;

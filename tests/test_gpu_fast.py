@@ -46,7 +46,8 @@ def _hyps(v, n, rng, near=0.6):
     ("tiny", 5, 2, 4, 1), ("small", 7, 1, 8, 1), ("small", 7, 1, 8, 2), ("small", 7, 1, 4, 4),
     ("small", 11, 1, 8, 2), ("small", 11, 1, 4, 4), ("tiny", 5, 2, 4, -1),
     ("small", 7, 1, 8, -1), ("small", 7, 1, 3, -1), ("small", 11, 1, 8, -1),
-    ("small", 5, 2, 1, -1), ("small", 7, 1, 1, 1), ("small", 7, 1, 1, -1)])
+    ("small", 5, 2, 1, -1), ("small", 7, 1, 1, 1), ("small", 7, 1, 1, -1),
+    ("tiny", 5, 2, 4, -2), ("small", 7, 1, 8, -2), ("small", 11, 1, 4, -2)])
 def test_fast_costs_within_tolerance(gpu_ctx, name, taps, stride, k, split):
     s = survey_cached(name)
     ref = s.reference_index()
@@ -100,9 +101,11 @@ def test_fast_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
     assert valid_agree >= 0.98
 
 
-@pytest.mark.parametrize("name,k,iters", [("tiny", 4, 3), ("small", 8, 4)])
-def test_pixel_lane_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
-    """Depth maps with every launch on the one-lane-per-pixel kernels."""
+@pytest.mark.parametrize("name,k,iters,split", [("tiny", 4, 3, -1), ("small", 8, 4, -1),
+                                                ("small", 8, 4, -2)])
+def test_pixel_lane_depth_maps_within_tolerance(gpu_ctx, name, k, iters, split):
+    """Depth maps with every half-sweep on the one-lane-per-pixel kernels
+    (split -1: f16 corner plane, -2: 8-bit corner plane)."""
     s = survey_cached(name)
     spec = s.spec
     ref = s.reference_index()
@@ -111,8 +114,8 @@ def test_pixel_lane_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
     o = orc.OracleDepth(v, srcs, *spec.depth_range, taps=spec.taps, stride=spec.stride)
     od, _, _, _ = o.depth_map(iterations=iters)
     prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=iters, fast=True)
-    _say(f"pixel-lane depth map {name} K={k} I={iters}")
-    gpu_ctx.set_schedule(-1, 1, 16, False)
+    _say(f"pixel-lane depth map {name} K={k} I={iters} split={split}")
+    gpu_ctx.set_schedule(split, 1, 16, False)
     try:
         gpu_ctx.reset_stats()
         d1, _, c1 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
@@ -120,7 +123,7 @@ def test_pixel_lane_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
         d2, _, c2 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
-    assert st["split"] == -1
+    assert st["split"] == split
     assert np.array_equal(d1, d2) and np.array_equal(c1, c2)  # deterministic
     both = (d1 > 0) & (od > 0)
     rel = np.abs(d1[both] - od[both]) / od[both]
