@@ -723,6 +723,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         // and 8-bit (its grey is a byte over 255)
         const int32_t ph =
             pinhole ? (ref.k1 == 0.0 && ref.k2 == 0.0 && ref.has_quad && lane_size ? 2 : 1) : 0;
+        job.lane_ok = ph == 2 && quads && K <= 16 ? 1 : 0;
         ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), ph, fast);
     }
 

@@ -51,7 +51,8 @@ struct RefJob {
     int32_t half;         // window half footprint, pixels
     int32_t pinhole;      // every source has pinhole_exact set
     float ray_step;       // tap spacing / f of the reference camera
-    int32_t pad;
+    int32_t lane_ok;      // 8-bit views, undistorted reference and sources: the
+                          // one-lane-per-pixel kernels apply
 };
 
 // Arguments of one batched PatchMatch launch (k_pm in mvs_pm_impl.h).
@@ -157,9 +158,12 @@ void launch_fuse(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                  hipStream_t st);
 
 // Gauss-Newton refinement of a job's accepted pixels (steps per pixel);
-// counters (nullable) [0] += accepted steps, [1] += pixels refined.
+// counters (nullable) [0] += accepted steps, [1] += pixels refined. Jobs with
+// lane_ok run the one-lane-per-pixel kernel (mvs_refine_lane.hip).
 int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long long *counters,
                   hipStream_t st);
+int launch_refine_lane(int32_t taps, const RefJob &job, int32_t steps,
+                       unsigned long long *counters, hipStream_t st);
 // Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
 int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
                      const int32_t *py, const float4 *hyp, int32_t *pairs, const float *gout,

@@ -470,6 +470,7 @@ void pair_list_taps(const RefJob &job, int64_t n, const int32_t *px, const int32
 
 int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long long *counters,
                   hipStream_t st) {
+    if (job.lane_ok) return launch_refine_lane(taps, job, steps, counters, st);
     switch (taps) {
         case 5: refine_taps<5>(job, steps, counters, st); return 0;
         case 7: refine_taps<7>(job, steps, counters, st); return 0;
