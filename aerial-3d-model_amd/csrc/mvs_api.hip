@@ -1607,33 +1607,39 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
         const FuseTable t = fuse_table(s);
         launch_fuse_clear(t, c->st);
         HIP_TRY(c, hipGetLastError());
-        bool grow = false;
-        for (int32_t r = 0; r < nrefs && !grow; ++r) {
+        // every view's neighbour cameras in one upload, then one launch per
+        // view with no host round trip; the counters are read once at the end
+        // (an insert into a full table is counted, never lost silently)
+        std::vector<NbrCam> all;
+        std::vector<int64_t> first((size_t)nrefs + 1, 0);
+        for (int32_t r = 0; r < nrefs; ++r) {
+            nbr_cams(s, nbrs + nbr_off[r], nbr_off[r + 1] - nbr_off[r], cams);
+            all.insert(all.end(), cams.begin(), cams.end());
+            first[(size_t)r + 1] = (int64_t)all.size();
+        }
+        HIP_TRY(c, c->nbr_cams.ensure(sizeof(NbrCam) * (all.size() + 1)));
+        if (!all.empty())
+            HIP_TRY(c, hipMemcpyAsync(c->nbr_cams.p, all.data(), sizeof(NbrCam) * all.size(),
+                                      hipMemcpyHostToDevice, c->st));
+        for (int32_t r = 0; r < nrefs; ++r) {
             const int32_t ref = refs[r];
             if (!s->maps[(size_t)ref].valid) continue;  // dense.rs:571-573
-            nbr_cams(s, nbrs + nbr_off[r], nbr_off[r + 1] - nbr_off[r], cams);
-            const int32_t nn = (int32_t)cams.size();
+            const int32_t nn = (int32_t)(first[(size_t)r + 1] - first[(size_t)r]);
             // dense.rs:585 for the reference rule; three views for reprojection
             const int32_t need = rule == MVS_CONS_REPROJ ? 3 : (nn >= 3 ? 2 : 1);
-            HIP_TRY(c, c->nbr_cams.ensure(sizeof(NbrCam) * (cams.size() + 1)));
-            if (!cams.empty())
-                HIP_TRY(c, hipMemcpyAsync(c->nbr_cams.p, cams.data(), sizeof(NbrCam) * cams.size(),
-                                          hipMemcpyHostToDevice, c->st));
             const ViewDev &rv = s->views[(size_t)ref];
             launch_fuse(ref_cam(rv), s->maps[(size_t)ref].depth.as<float>(), rv.w, rv.h,
-                        c->nbr_cams.as<NbrCam>(), nn, rule, need, 1.0 / voxel,
+                        c->nbr_cams.as<NbrCam>() + first[(size_t)r], nn, rule, need, 1.0 / voxel,
                         rv.has_rgb ? rv.rgb.as<uint8_t>() : nullptr,
                         rv.has_quad ? rv.quad.as<uint32_t>() : nullptr, rv.gray.as<float>(), t,
                         c->st);
             HIP_TRY(c, hipGetLastError());
-            // the neighbour table is rewritten for the next view
-            HIP_TRY(c, hipStreamSynchronize(c->st));
-            unsigned long long v[4];
-            HIP_TRY(c, hipMemcpy(v, s->fctr.p, sizeof v, hipMemcpyDeviceToHost));
-            grow = (v[2] > 0 || (double)v[0] > 0.7 * (double)cap) && cap < (1ll << 30);
         }
+        // the host vector must outlive the asynchronous upload
+        HIP_TRY(c, hipStreamSynchronize(c->st));
         unsigned long long ctr[4];
         HIP_TRY(c, hipMemcpy(ctr, s->fctr.p, sizeof ctr, hipMemcpyDeviceToHost));
+        const bool grow = (ctr[2] > 0 || (double)ctr[0] > 0.7 * (double)cap) && cap < (1ll << 30);
         if (ctr[3] > 0)
             return fail(c, MVS_ERR_ARG, "voxel index beyond +-2^20: voxel too small for the scene");
         if (grow) {

@@ -31,68 +31,121 @@ __global__ void k_fuse_clear(FuseTable t) {
     if (blockIdx.x == 0 && threadIdx.x < 4) t.counters[threadIdx.x] = 0;
 }
 
-__global__ void k_fuse(RefCam64 rf, const float *__restrict__ depth, int32_t w, int32_t h,
-                       const NbrCam *__restrict__ nbrs, int32_t nn, int32_t rule, int32_t need,
-                       double inv_voxel, const uint8_t *__restrict__ rgb,
-                       const uint32_t *__restrict__ quad, const float *__restrict__ gray,
-                       FuseTable t) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)w * h) return;
-    double xw[3];
-    const int32_t agree = agree_pixel(rf, depth, w, nbrs, nn, rule, i, xw);
-    if (agree < 0 || agree < need) return;  // dense.rs:614-616
-    // dense.rs:617-621: floor(xw / voxel) per axis
-    int64_t k3[3];
-    bool in_range = true;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        const double f = floor(xw[r] * inv_voxel);
-        in_range = in_range && f >= (double)-kKeyHalf && f < (double)kKeyHalf;
-        k3[r] = in_range ? (int64_t)f : 0;
-    }
-    if (!in_range) {
-        atomicAdd(&t.counters[3], 1ull);
-        return;
-    }
-    const unsigned long long key = ((unsigned long long)(k3[0] + kKeyHalf) << 42) |
-                                   ((unsigned long long)(k3[1] + kKeyHalf) << 21) |
-                                   (unsigned long long)(k3[2] + kKeyHalf);
+// Global insert of one voxel's aggregated sums (open addressing, linear
+// probing bounded by the capacity). Returns false when the table is full.
+__device__ bool fuse_insert(const FuseTable &t, unsigned long long key,
+                            const unsigned long long s3[3], const unsigned int c3[3],
+                            unsigned int n) {
     const unsigned long long mask = (unsigned long long)t.cap - 1ull;
     unsigned long long slot = dev::mix64(key) & mask;
-    bool found = false;
     for (int64_t probe = 0; probe < t.cap; ++probe) {
         const unsigned long long prev = atomicCAS(&t.keys[slot], kEmpty, key);
         if (prev == kEmpty) atomicAdd(&t.counters[0], 1ull);
         if (prev == kEmpty || prev == key) {
-            found = true;
-            break;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) atomicAdd(&t.sum[3 * slot + r], s3[r]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) atomicAdd(&t.col[3 * slot + r], c3[r]);
+            atomicAdd(&t.cnt[slot], n);
+            return true;
         }
         slot = (slot + 1ull) & mask;
     }
-    if (!found) {
-        atomicAdd(&t.counters[2], 1ull);
-        return;
-    }
+    return false;
+}
+
+// One 16 x 16 pixel tile per workgroup. Confirmed pixels first meet in an LDS
+// table of the tile's voxels (at most 256 distinct keys, so the 256-slot table
+// never fills); each distinct voxel then goes to the global table once. The
+// sums are integers, so the result does not depend on this grouping.
+constexpr int kTile = 16;
+constexpr int kTileSlots = kTile * kTile;
+
+__global__ __launch_bounds__(kTileSlots) void k_fuse(
+    RefCam64 rf, const float *__restrict__ depth, int32_t w, int32_t h,
+    const NbrCam *__restrict__ nbrs, int32_t nn, int32_t rule, int32_t need, double inv_voxel,
+    const uint8_t *__restrict__ rgb, const uint32_t *__restrict__ quad,
+    const float *__restrict__ gray, FuseTable t) {
+    __shared__ unsigned long long lkey[kTileSlots];
+    __shared__ unsigned long long lsum[kTileSlots][3];
+    __shared__ unsigned int lcol[kTileSlots][3];
+    __shared__ unsigned int lcnt[kTileSlots];
+    __shared__ unsigned int lrange;
+    const int tid = threadIdx.y * kTile + threadIdx.x;
+    lkey[tid] = kEmpty;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-        atomicAdd(&t.sum[3 * slot + r], (unsigned long long)(long long)llrint(xw[r] * kFix));
-    // dense.rs:622-628: the pixel's colour
-    unsigned int c0, c1, c2;
-    if (rgb) {
-        c0 = rgb[3 * i];
-        c1 = rgb[3 * i + 1];
-        c2 = rgb[3 * i + 2];
-    } else {
-        const unsigned int g =
-            quad ? (quad[i] & 0xffu)
-                 : (unsigned int)fminf(fmaxf(gray[i] * 255.0f + 0.5f, 0.0f), 255.0f);
-        c0 = c1 = c2 = g;
+    for (int r = 0; r < 3; ++r) {
+        lsum[tid][r] = 0;
+        lcol[tid][r] = 0;
     }
-    atomicAdd(&t.col[3 * slot], c0);
-    atomicAdd(&t.col[3 * slot + 1], c1);
-    atomicAdd(&t.col[3 * slot + 2], c2);
-    atomicAdd(&t.cnt[slot], 1u);
-    atomicAdd(&t.counters[1], 1ull);
+    lcnt[tid] = 0;
+    if (tid == 0) lrange = 0;
+    __syncthreads();
+    const int32_t x = blockIdx.x * kTile + threadIdx.x, y = blockIdx.y * kTile + threadIdx.y;
+    if (x < w && y < h) {
+        const int64_t i = (int64_t)y * w + x;
+        double xw[3];
+        const int32_t agree = agree_pixel(rf, depth, w, nbrs, nn, rule, i, xw);
+        if (agree >= 0 && agree >= need) {  // dense.rs:614-616
+            // dense.rs:617-621: floor(xw / voxel) per axis
+            int64_t k3[3];
+            bool in_range = true;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double f = floor(xw[r] * inv_voxel);
+                in_range = in_range && f >= (double)-kKeyHalf && f < (double)kKeyHalf;
+                k3[r] = in_range ? (int64_t)f : 0;
+            }
+            if (!in_range) {
+                atomicAdd(&lrange, 1u);
+            } else {
+                const unsigned long long key = ((unsigned long long)(k3[0] + kKeyHalf) << 42) |
+                                               ((unsigned long long)(k3[1] + kKeyHalf) << 21) |
+                                               (unsigned long long)(k3[2] + kKeyHalf);
+                // dense.rs:622-628: the pixel's colour
+                unsigned int c[3];
+                if (rgb) {
+                    c[0] = rgb[3 * i];
+                    c[1] = rgb[3 * i + 1];
+                    c[2] = rgb[3 * i + 2];
+                } else {
+                    const unsigned int g =
+                        quad ? (quad[i] & 0xffu)
+                             : (unsigned int)fminf(fmaxf(gray[i] * 255.0f + 0.5f, 0.0f), 255.0f);
+                    c[0] = c[1] = c[2] = g;
+                }
+                // LDS slot of the key: at most 256 keys in 256 slots, so the
+                // probe ends within 256 steps
+                unsigned int slot = (unsigned int)dev::mix64(key) & (kTileSlots - 1);
+                for (int probe = 0; probe < kTileSlots; ++probe) {
+                    const unsigned long long prev = atomicCAS(&lkey[slot], kEmpty, key);
+                    if (prev == kEmpty || prev == key) break;
+                    slot = (slot + 1) & (kTileSlots - 1);
+                }
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    atomicAdd(&lsum[slot][r], (unsigned long long)(long long)llrint(xw[r] * kFix));
+                    atomicAdd(&lcol[slot][r], c[r]);
+                }
+                atomicAdd(&lcnt[slot], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && lrange) atomicAdd(&t.counters[3], (unsigned long long)lrange);
+    const unsigned int n = lcnt[tid];
+    bool stored = false;
+    if (n) {
+        const unsigned long long s3[3] = {lsum[tid][0], lsum[tid][1], lsum[tid][2]};
+        const unsigned int c3[3] = {lcol[tid][0], lcol[tid][1], lcol[tid][2]};
+        stored = fuse_insert(t, lkey[tid], s3, c3, n);
+        if (!stored) atomicAdd(&t.counters[2], (unsigned long long)n);  // points dropped
+    }
+    // points accumulated: one atomic per wavefront
+    unsigned long long pts = stored ? n : 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) pts += __shfl_xor(pts, off);
+    if ((tid & 63) == 0 && pts) atomicAdd(&t.counters[1], pts);
 }
 
 }  // namespace
@@ -106,9 +159,9 @@ void launch_fuse(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                  const NbrCam *nbrs, int32_t nn, int32_t rule, int32_t need, double inv_voxel,
                  const uint8_t *rgb, const uint32_t *quad, const float *gray, const FuseTable &t,
                  hipStream_t st) {
-    const int64_t n = (int64_t)w * h;
-    hipLaunchKernelGGL(k_fuse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ref, depth, w,
-                       h, nbrs, nn, rule, need, inv_voxel, rgb, quad, gray, t);
+    const dim3 grid((unsigned)((w + kTile - 1) / kTile), (unsigned)((h + kTile - 1) / kTile));
+    hipLaunchKernelGGL(k_fuse, grid, dim3(kTile, kTile), 0, st, ref, depth, w, h, nbrs, nn, rule,
+                       need, inv_voxel, rgb, quad, gray, t);
 }
 
 }  // namespace amvs

@@ -187,14 +187,32 @@ def test_pair_argument_errors(gpu_ctx):
     assert len(c) == 0
 
 
-@pytest.mark.parametrize("name,k,taps,stride,iters", [("tiny", 4, 5, 2, 3),
-                                                      ("small", 4, 7, 1, 4)])
-def test_refinement_lowers_cost_and_depth_error(gpu_ctx, name, k, taps, stride, iters):
+def _as_f32(views):
+    """The same views given as f32 grey (the table values of their bytes): the
+    refinement then runs the two-lanes-per-pixel kernel instead of the
+    one-lane-per-pixel kernel of 8-bit views."""
+    import copy
+    lut = orc.gray_lut()
+    out = []
+    for v in views:
+        v2 = copy.copy(v)
+        v2.gray_f32 = lut[v.gray].astype(np.float32)
+        out.append(v2)
+    return out
+
+
+@pytest.mark.parametrize("name,k,taps,stride,iters,f32", [("tiny", 4, 5, 2, 3, False),
+                                                          ("small", 4, 7, 1, 4, False),
+                                                          ("small", 4, 7, 1, 4, True)])
+def test_refinement_lowers_cost_and_depth_error(gpu_ctx, name, k, taps, stride, iters, f32):
     s, v, srcs, o = _setup(name, k, taps, stride)
+    if f32:
+        v, srcs = _as_f32([v])[0], _as_f32(srcs)
     rng0, rng1 = s.spec.depth_range
     base = capi.params(taps=taps, stride=stride, iterations=iters)
     refp = capi.params(taps=taps, stride=stride, iterations=iters, refine_steps=4)
-    _say(f"refine {name}: PatchMatch then 4 Gauss-Newton steps")
+    _say(f"refine {name} ({'f32' if f32 else '8-bit'} views): PatchMatch then 4 Gauss-Newton "
+         "steps")
     d0, n0, c0 = gpu_ctx.depth_map(v, srcs, rng0, rng1, base)
     gpu_ctx.reset_stats()
     d1, n1, c1 = gpu_ctx.depth_map(v, srcs, rng0, rng1, refp)
