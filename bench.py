@@ -36,9 +36,10 @@ PEAK_HBM_GBS = 8000.0
 # HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE over the 12 half-sweeps of an 8-view batch of this
 # workload, scaled by a 1 GiB device copy (read counter x2.000, write x1.000);
-# profiles/k_pm_sweep_{fast,exact}_8x2000x1500_k8_7x7_counters.md hold the
-# per-launch figures of each arithmetic mode.
-BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (650.1, 5.75)}
+# profiles/k_pml_sweep_fast_8x2000x1500_k8_7x7_counters.md (fast arithmetic,
+# one lane per pixel) and profiles/k_pm_sweep_exact_8x2000x1500_k8_7x7_counters.md
+# hold the per-launch figures.
+BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (1084.7, 4.99)}
 
 
 def log(*a):
@@ -281,13 +282,27 @@ def main():
     achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
     rd, wr = BYTES_PER_EVAL[args.mode]
     traffic = (rd + wr) * evals_per_launch if rd is not None else None
-    roofline = dict(bound="valu", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+    sched = [t for t in ctx.tuned_schedules()
+             if t["mode"] == (1 if args.mode == "fast" else 0) and t["taps"] == spec.taps]
+    splits = sorted({t["split"] for t in sched})
+    if args.mode == "fast" and splits and max(splits) < 0:
+        kname = (f"k_pml<{spec.taps},sweep,persistent> (one lane per pixel; corner plane per "
+                 "iteration: " + ", ".join(f"{t['iteration']}:{'f16' if t['split'] == -1 else '8-bit'}"
+                                          for t in sorted(sched, key=lambda t: t['iteration']))
+                 + ")")
+    else:
+        kname = (f"k_pm<{spec.taps},8,split {'/'.join(map(str, splits))},pinhole,sweep,persistent,"
+                 f"{'fast' if args.mode == 'fast' else 'exact'}>")
+    roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
+                    bound_note=("compute roofline at MI355X's FP32 peak (157.3 TFLOP/s, its figure "
+                                "for vector and matrix FP32 alike); the kernel computes on the "
+                                "vector ALUs, no matrix instructions, and its measured limiter is "
+                                "the texture-address unit of the sample gathers"),
                     traffic_note=("HBM-side bytes per launch: rocprofv3 FETCH_SIZE/WRITE_SIZE per "
                                   "evaluation (copy-calibrated) x evaluations per launch"),
                     hbm_gbs=(traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                    kernel=(f"k_pm<{spec.taps},8,{st['split']},pinhole,sweep,persist,"
-                            f"{'fast' if args.mode == 'fast' else 'exact'}>"),
+                    kernel=kname,
                     avg_launch_ms=avg_ms, evals_per_launch=evals_per_launch, flops_per_eval=fpe,
                     launches=st["sweep_launches"])
 
