@@ -15,6 +15,12 @@ LIB_PATH = _build.LIB_MVS
 
 MVS_OK = 0
 MVS_NO_NEIGHBOURS = 1
+MVS_ERR_ARG = -1
+MVS_ERR_DEVICE = -2
+MVS_ERR_CANCELLED = -3
+MVS_ERR_NOMEM = -4
+MVS_ERR_IO = -5
+MVS_ERR_FORMAT = -6
 MVS_GRAY_U8 = 0
 MVS_GRAY_F32 = 1
 MVS_CONS_REFERENCE = 0
@@ -32,7 +38,7 @@ EXPORTS = [
     "mvs_set_schedule", "mvs_tuned_schedules", "mvs_ncc_pair", "mvs_set_graphs",
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
-    "mvs_write_ply", "mvs_read_ply",
+    "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
 ]
 
 
@@ -123,6 +129,9 @@ def lib():
         L.mvs_save_view.argtypes = [C.c_char_p, i32, i32, vp, vp]
         L.mvs_load_view.argtypes = [C.c_char_p, P(C.c_int32), P(C.c_int32), vp, vp, i64]
         L.mvs_view_intrinsics.argtypes = [f64, f64, f64, i32, i32, P(C.c_double)]
+        L.mvs_prepared_size.argtypes = [i32, i32, i32, P(C.c_int32), P(C.c_int32),
+                                        P(C.c_double)]
+        L.mvs_prepare_view.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
         L.mvs_survey_set_colour.argtypes = [vp, i32, vp]
         L.mvs_survey_fuse.argtypes = [vp, i32, vp, vp, vp, i32, f64, P(C.c_int64)]
         L.mvs_survey_fused_points.argtypes = [vp, vp, vp, i64, P(C.c_int64)]
@@ -227,6 +236,15 @@ def load_view(path):
                            c.ctypes.data, g.size):
         return None
     return g, c
+
+
+def prepared_size(w, h, max_side):
+    """shrink_to's output size and scale (imgops.rs:145-171)."""
+    ow, oh, sc = C.c_int32(), C.c_int32(), C.c_double()
+    rc = lib().mvs_prepared_size(w, h, max_side, C.byref(ow), C.byref(oh), C.byref(sc))
+    if rc:
+        raise MvsError(rc, "bad sizes")
+    return ow.value, oh.value, sc.value
 
 
 def view_intrinsics(f, cx, cy, full_width, w):
@@ -388,6 +406,19 @@ class Context:
 
     def reset_stats(self):
         self.check(lib().mvs_reset_stats(self._h))
+
+    def prepare_view(self, rgb, max_side):
+        """prepare_views' per-photo work on the GPU (dense.rs:100-112): an
+        (h, w, 3) uint8 photo -> (working-size (h', w', 3) uint8 RGB, (h', w')
+        f32 grey), the contents of a `.view` file."""
+        img = np.ascontiguousarray(rgb, np.uint8)
+        h, w = img.shape[:2]
+        ow, oh, _ = prepared_size(w, h, max_side)
+        grey = np.zeros((oh, ow), np.float32)
+        out = np.zeros((oh, ow, 3), np.uint8)
+        self.check(lib().mvs_prepare_view(self._h, img.ctypes.data, w, h, w * 3, max_side,
+                                          grey.ctypes.data, out.ctypes.data))
+        return out, grey
 
     def set_schedule(self, split=0, gpc=0, per_cu=16, autotune=True):
         self.check(lib().mvs_set_schedule(self._h, split, gpc, per_cu, 1 if autotune else 0))

@@ -112,6 +112,7 @@ struct mvs_ctx {
     std::vector<ViewDev> tmp_views;  // host-view calls
     MapDev tmp_map;
     DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
+    DevBuf prep_box, prep_rgb, prep_grey;  // view preparation
     DevBuf pairs, grad_io, gout;  // pair-cost lists
     std::map<RayKey, DevBuf *> rays;
     // host copies that must outlive the asynchronous uploads of a call
@@ -1015,7 +1016,8 @@ void mvs_destroy(mvs_ctx *c) {
                       &c->up_u8,    &c->px,     &c->py,       &c->hyp_in, &c->cost_out,
                       &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost,
                       &c->pairs,    &c->grad_io, &c->gout,
-                      &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost};
+                      &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost,
+                      &c->prep_box, &c->prep_rgb, &c->prep_grey};
     for (DevBuf *b : bufs) b->release();
     for (auto &v : c->tmp_views) {
         v.gray.release();
@@ -1834,6 +1836,43 @@ int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64
         }
     }
     fclose(f);
+    return MVS_OK;
+}
+
+int mvs_prepared_size(int32_t w, int32_t h, int32_t max_side, int32_t *out_w, int32_t *out_h,
+                      double *scale) {
+    if (w < 1 || h < 1 || max_side < 1 || !out_w || !out_h) return MVS_ERR_ARG;
+    PrepSizes z;
+    prep_sizes(w, h, max_side, z);
+    *out_w = z.ow;
+    *out_h = z.oh;
+    if (scale) *scale = z.scale;
+    return MVS_OK;
+}
+
+int mvs_prepare_view(mvs_ctx *c, const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
+                     int32_t max_side, float *gray, uint8_t *rgb_out) {
+    if (!c) return MVS_ERR_ARG;
+    if (!rgb || !gray || !rgb_out || w < 1 || h < 1 || stride < 3 * w || max_side < 1)
+        return fail(c, MVS_ERR_ARG, "bad view preparation arguments");
+    if ((int64_t)w * h > (1ll << 31) / 3)
+        return fail(c, MVS_ERR_ARG, "photo too large (3 w h must stay below 2^31)");
+    PrepSizes z;
+    prep_sizes(w, h, max_side, z);
+    const size_t in_bytes = (size_t)stride * (h - 1) + (size_t)w * 3;
+    const size_t n = (size_t)z.ow * z.oh;
+    HIP_TRY(c, c->up_u8.ensure(in_bytes));
+    HIP_TRY(c, c->prep_box.ensure(z.f >= 2 ? (size_t)z.bw * z.bh * 3 : 1));
+    HIP_TRY(c, c->prep_rgb.ensure(n * 3));
+    HIP_TRY(c, c->prep_grey.ensure(n * sizeof(float)));
+    HIP_TRY(c, hipMemcpyAsync(c->up_u8.p, rgb, in_bytes, hipMemcpyHostToDevice, c->st));
+    launch_prepare_view(c->up_u8.as<uint8_t>(), w, h, stride, z, c->prep_box.as<uint8_t>(),
+                        c->prep_rgb.as<uint8_t>(), c->prep_grey.as<float>(), c->st);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(gray, c->prep_grey.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                              c->st));
+    HIP_TRY(c, hipMemcpyAsync(rgb_out, c->prep_rgb.p, n * 3, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(c, hipStreamSynchronize(c->st));
     return MVS_OK;
 }
 

@@ -164,6 +164,18 @@ int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long 
                   hipStream_t st);
 int launch_refine_lane(int32_t taps, const RefJob &job, int32_t steps,
                        unsigned long long *counters, hipStream_t st);
+// View preparation (mvs_prep.hip): shrink_to's sizes (box factor f, 0 =
+// none; boxed size; output size; scale new/old) and the launches from a
+// device copy of the photo to the working-size RGB8 and grey.
+struct PrepSizes {
+    int32_t f, bw, bh, ow, oh;
+    double scale;
+};
+void prep_sizes(int32_t w, int32_t h, int32_t max_side, PrepSizes &z);
+void launch_prepare_view(const uint8_t *src, int32_t w, int32_t h, int32_t stride,
+                         const PrepSizes &z, uint8_t *boxed, uint8_t *out_rgb, float *grey,
+                         hipStream_t st);
+
 // Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
 int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
                      const int32_t *py, const float4 *hyp, int32_t *pairs, const float *gout,

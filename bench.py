@@ -105,6 +105,53 @@ def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm_exact,
                 sample_fast_valid_agree=float(np.mean((gf > 0) == (od > 0))))
 
 
+def view_prep_stage(ctx, spec, photos=8):
+    """prepare_views' per-photo work (dense.rs:100-112) on the GPU for photos
+    twice the survey's working size (4000x3000 RGB8 for c2), to the working
+    size (box by 2) and to 0.7 of it (box, then bilinear resize): input Mpix/s
+    end to end (upload, kernels, download). Returns the figures, the photo and
+    the GPU's outputs (for the CPU-baseline leg's check)."""
+    h, w = 2 * spec.height, 2 * spec.width
+    rng = np.random.default_rng(11)
+    base = rng.integers(0, 256, (h // 8 + 1, w // 8 + 1, 3), dtype=np.uint8)
+    photo = np.repeat(np.repeat(base, 8, 0), 8, 1)[:h, :w]
+    photo = np.clip(photo.astype(np.int16) + rng.integers(-20, 21, photo.shape), 0, 255)
+    photo = np.ascontiguousarray(photo.astype(np.uint8))
+    out, outputs = {}, {}
+    for tag, m in (("box", max(spec.width, spec.height)),
+                   ("box_resize", int(0.7 * max(spec.width, spec.height)))):
+        ctx.prepare_view(photo, m)  # warm
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(photos):
+            rgb, grey = ctx.prepare_view(photo, m)
+        dt = (time.perf_counter() - t) / photos
+        out[tag] = dict(max_side=m, out=f"{grey.shape[1]}x{grey.shape[0]}",
+                        gpu_mpix_s=w * h / dt / 1e6, gpu_ms_per_photo=dt * 1e3)
+        outputs[tag] = (m, rgb, grey)
+        log(f"view prep {tag}: {out[tag]}")
+    return (dict(photo=f"{w}x{h} RGB8 (synthetic)", photos=photos,
+                 path="mvs_prepare_view: upload, box downsample, bilinear resize, grey, download",
+                 **out), photo, outputs)
+
+
+def cpu_baseline_view_prep(photo, outputs):
+    """CPU-baseline leg of the view-preparation stage: the oracle on the same
+    photo (one thread), and the GPU's outputs checked identical to it."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    h, w = photo.shape[:2]
+    res = {}
+    for tag, (m, rgb, grey) in outputs.items():
+        tc = time.perf_counter()
+        want_rgb, want_grey = orc.prepare_view(photo, m)
+        dc = time.perf_counter() - tc
+        res[tag] = dict(cpu_mpix_s=w * h / dc / 1e6, cores=1, kind="port",
+                        identical=bool(np.array_equal(rgb, want_rgb)
+                                       and np.array_equal(grey, want_grey)))
+    return res
+
+
 def run_uneven(args):
     """The batched uneven workload: every view of an uneven survey (nadir and
     oblique views of two sizes, per-view neighbour count 3..12 and window
@@ -175,6 +222,8 @@ def main():
                     help="arithmetic of the batched path's PatchMatch kernels")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--no-prep", action="store_true",
+                    help="skip the view-preparation stage (RGB photo -> .view contents)")
     ap.add_argument("--cpu-crop", default="1280x960")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--plain-views", type=int, default=8)
@@ -355,12 +404,19 @@ def main():
                       ply_bytes=int(len(xyz)) * 15)
         log(f"fusion: {len(xyz)} voxels of {voxel:.4f} m in {dtf:.2f} s")
 
+    prep = prep_photo = prep_out = None
+    if not args.no_prep:
+        prep, prep_photo, prep_out = view_prep_stage(ctx, spec)
+
     cpu = None
     if not args.no_cpu:
         cw, ch = (int(a) for a in args.cpu_crop.split("x"))
         log("CPU baseline (oracle) ...")
         cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx, prm_exact, prm_fast0)
         log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
+        if prep_out:
+            cpu["view_prep"] = cpu_baseline_view_prep(prep_photo, prep_out)
+            log(f"CPU view prep: {cpu['view_prep']}")
 
     line = dict(
         metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,
@@ -386,7 +442,7 @@ def main():
         refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
         graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
-        fusion=fusion,
+        fusion=fusion, view_prep=prep,
     )
     print(json.dumps(line), flush=True)
     sv.close()

@@ -311,6 +311,23 @@ int mvs_load_view(const char *path, int32_t *w, int32_t *h, float *gray, uint8_t
 int mvs_view_intrinsics(double f, double cx, double cy, int32_t full_width, int32_t w,
                         double out[3]);
 
+/* ---- view preparation (dense.rs:92-121, imgops.rs:89-171) ---- */
+/* shrink_to's output size for a w x h photo and working size max_side
+ * (imgops.rs:145-171): unchanged when the long side fits, else an integer box
+ * downsample by long / max_side (when >= 2) and a bilinear resize to the exact
+ * size; scale = max_side / long (1 when unchanged). Host-only, no context. */
+int mvs_prepared_size(int32_t w, int32_t h, int32_t max_side, int32_t *out_w, int32_t *out_h,
+                      double *scale);
+/* The per-photo work of prepare_views after decoding (dense.rs:100-112) on
+ * the GPU: rgb is the decoded RGB8 photo (w x h, stride bytes per row >= 3 w),
+ * max_side the profile's working size (DenseParams.size). Writes the
+ * working-size grey (out_w x out_h f32, dense.rs:108) and RGB8 (3 out_w out_h
+ * bytes) with out_w, out_h from mvs_prepared_size — the contents of a `.view`
+ * file (mvs_save_view). Byte- and bit-identical to the reference's shrink_to
+ * and grey formula. */
+int mvs_prepare_view(mvs_ctx *ctx, const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
+                     int32_t max_side, float *gray, uint8_t *rgb_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -843,3 +843,106 @@ int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4],
     if (cells) *cells = cell;
     return 1;
 }
+
+
+/* ---- view preparation ---------------------------------------------------- */
+
+/* imgops.rs:145-171 shrink_to: the sizes only. */
+void orc_shrink_size(int32_t w, int32_t h, int32_t max_side, int32_t *f, int32_t *bw,
+                     int32_t *bh, int32_t *ow, int32_t *oh, double *scale) {
+    const int32_t lng = w > h ? w : h;
+    if (lng <= max_side) {
+        *f = 0;
+        *bw = *ow = w;
+        *bh = *oh = h;
+        *scale = 1.0;
+        return;
+    }
+    const int32_t fac = lng / max_side;
+    *f = fac >= 2 ? fac : 0;
+    *bw = fac >= 2 ? w / fac : w; /* box_down: w / f, h / f (imgops.rs:94-95) */
+    *bh = fac >= 2 ? h / fac : h;
+    *scale = (double)max_side / (double)lng;
+    const int32_t bl = *bw > *bh ? *bw : *bh;
+    const double k = (double)max_side / (double)bl;
+    *ow = (int32_t)round((double)*bw * k);
+    *oh = (int32_t)round((double)*bh * k);
+    if (*ow < 1) *ow = 1;
+    if (*oh < 1) *oh = 1;
+}
+
+/* imgops.rs:89-118 box_down: u32 sums over f x f, (sum + area/2) / area. */
+void orc_box_down(const uint8_t *src, int32_t w, int32_t h, int32_t stride, int32_t f,
+                  uint8_t *dst) {
+    const int32_t ow = w / f, oh = h / f;
+    const uint32_t area = (uint32_t)(f * f);
+    for (int32_t oy = 0; oy < oh; ++oy)
+        for (int32_t ox = 0; ox < ow; ++ox)
+            for (int c = 0; c < 3; ++c) {
+                uint32_t a = 0;
+                for (int32_t sy = oy * f; sy < oy * f + f; ++sy)
+                    for (int32_t k = 0; k < f; ++k)
+                        a += src[(int64_t)sy * stride + (int64_t)(ox * f + k) * 3 + c];
+                dst[((int64_t)oy * ow + ox) * 3 + c] = (uint8_t)((a + area / 2) / area);
+            }
+}
+
+/* imgops.rs:120-143 resize_bilinear, f64, left to right, rounded half away. */
+void orc_resize_bilinear(const uint8_t *src, int32_t sw, int32_t sh, int32_t stride,
+                         uint8_t *dst, int32_t w, int32_t h) {
+    const double sx = (double)sw / (double)w, sy = (double)sh / (double)h;
+    for (int32_t y = 0; y < h; ++y) {
+        double fy = ((double)y + 0.5) * sy - 0.5;
+        fy = fy < 0.0 ? 0.0 : (fy > (double)(sh - 1) ? (double)(sh - 1) : fy);
+        const int32_t y0 = (int32_t)fy;
+        const int32_t y1 = y0 + 1 < sh - 1 ? y0 + 1 : sh - 1;
+        const double ay = fy - (double)y0;
+        for (int32_t x = 0; x < w; ++x) {
+            double fx = ((double)x + 0.5) * sx - 0.5;
+            fx = fx < 0.0 ? 0.0 : (fx > (double)(sw - 1) ? (double)(sw - 1) : fx);
+            const int32_t x0 = (int32_t)fx;
+            const int32_t x1 = x0 + 1 < sw - 1 ? x0 + 1 : sw - 1;
+            const double ax = fx - (double)x0;
+            for (int c = 0; c < 3; ++c) {
+                const double p00 = src[(int64_t)y0 * stride + (int64_t)x0 * 3 + c];
+                const double p10 = src[(int64_t)y0 * stride + (int64_t)x1 * 3 + c];
+                const double p01 = src[(int64_t)y1 * stride + (int64_t)x0 * 3 + c];
+                const double p11 = src[(int64_t)y1 * stride + (int64_t)x1 * 3 + c];
+                double top = p00 * (1.0 - ax);
+                top = top + p10 * ax;
+                double bot = p01 * (1.0 - ax);
+                bot = bot + p11 * ax;
+                double v = top * (1.0 - ay);
+                v = v + bot * ay;
+                dst[((int64_t)y * w + x) * 3 + c] = (uint8_t)round(v);
+            }
+        }
+    }
+}
+
+/* dense.rs:100-112: shrink_to, then the grey of every pixel (dense.rs:108). */
+void orc_prepare_view(const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
+                      int32_t max_side, uint8_t *out_rgb, float *grey) {
+    int32_t f, bw, bh, ow, oh;
+    double scale;
+    orc_shrink_size(w, h, max_side, &f, &bw, &bh, &ow, &oh, &scale);
+    uint8_t *boxed = NULL;
+    const uint8_t *b = rgb;
+    int32_t bstride = stride;
+    if (f >= 2) {
+        boxed = (uint8_t *)malloc((size_t)bw * bh * 3);
+        orc_box_down(rgb, w, h, stride, f, boxed);
+        b = boxed;
+        bstride = bw * 3;
+    }
+    if (ow == bw && oh == bh) {
+        for (int32_t y = 0; y < oh; ++y)
+            memcpy(out_rgb + (int64_t)y * ow * 3, b + (int64_t)y * bstride, (size_t)ow * 3);
+    } else {
+        orc_resize_bilinear(b, bw, bh, bstride, out_rgb, ow, oh);
+    }
+    free(boxed);
+    const int64_t n = (int64_t)ow * oh;
+    for (int64_t i = 0; i < n; ++i)
+        grey[i] = orc_gray_rgb(out_rgb[3 * i], out_rgb[3 * i + 1], out_rgb[3 * i + 2]);
+}

@@ -128,6 +128,20 @@ double orc_fusion_voxel(const double *xs, const double *ys, int64_t npts, const 
                         int32_t ncams, double f, int32_t size, int32_t width, int32_t height,
                         int64_t max_points);
 
+/* View preparation (dense.rs:92-121 with imgops.rs:89-171): a decoded RGB8
+ * photo (stride bytes per row) shrunk so its long side is at most max_side,
+ * then its grey. orc_shrink_size gives the box factor f (0: none), the boxed
+ * size, the output size and the scale new/old; sizes in, sizes out. */
+void orc_shrink_size(int32_t w, int32_t h, int32_t max_side, int32_t *f, int32_t *bw,
+                     int32_t *bh, int32_t *ow, int32_t *oh, double *scale);
+void orc_box_down(const uint8_t *src, int32_t w, int32_t h, int32_t stride, int32_t f,
+                  uint8_t *dst);
+void orc_resize_bilinear(const uint8_t *src, int32_t sw, int32_t sh, int32_t stride,
+                         uint8_t *dst, int32_t w, int32_t h);
+/* the whole of it: out_rgb (ow*oh*3) and grey (ow*oh), sizes from orc_shrink_size */
+void orc_prepare_view(const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
+                      int32_t max_side, uint8_t *out_rgb, float *grey);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,14 @@ def lib():
         L.orc_fusion_voxel.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int32, C.c_double,
                                        C.c_int32, C.c_int32, C.c_int32, C.c_int64]
         L.orc_fusion_voxel.restype = C.c_double
+        _i32o = C.POINTER(C.c_int32)
+        L.orc_shrink_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, _i32o, _i32o, _i32o,
+                                      _i32o, _i32o, C.POINTER(C.c_double)]
+        L.orc_box_down.argtypes = [_u8p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u8p]
+        L.orc_resize_bilinear.argtypes = [_u8p, C.c_int32, C.c_int32, C.c_int32, _u8p,
+                                          C.c_int32, C.c_int32]
+        L.orc_prepare_view.argtypes = [_u8p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u8p,
+                                       _f32p]
         _lib = L
     return _lib
 
@@ -365,3 +373,43 @@ def fusion_voxel(xs, ys, ranges, f, size, width, height, max_points):
     rg = np.ascontiguousarray(ranges, np.float64).ravel()
     return lib().orc_fusion_voxel(xs, ys, len(xs), rg, len(rg) // 2, f, size, width, height,
                                   max_points)
+
+
+# ---- view preparation (dense.rs:92-121, imgops.rs:89-171) --------------------
+
+def shrink_size(w, h, max_side):
+    """imgops.rs:145-171: box factor (0 = none), boxed size, output size, scale."""
+    v = [C.c_int32() for _ in range(5)]
+    sc = C.c_double()
+    lib().orc_shrink_size(w, h, max_side, *[C.byref(x) for x in v], C.byref(sc))
+    f, bw, bh, ow, oh = (x.value for x in v)
+    return dict(f=f, bw=bw, bh=bh, w=ow, h=oh, scale=sc.value)
+
+
+def box_down(img, f):
+    """imgops.rs:89-118 on an (h, w, 3) uint8 image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    out = np.zeros((h // f, w // f, 3), np.uint8)
+    lib().orc_box_down(img.ravel(), w, h, w * 3, f, out.ravel())
+    return out
+
+
+def resize_bilinear(img, w, h):
+    """imgops.rs:120-143 on an (h, w, 3) uint8 image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    sh, sw = img.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    lib().orc_resize_bilinear(img.ravel(), sw, sh, sw * 3, out.ravel(), w, h)
+    return out
+
+
+def prepare_view(img, max_side):
+    """dense.rs:100-112: (working-size RGB8 (h, w, 3), grey f32 (h, w))."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    z = shrink_size(w, h, max_side)
+    rgb = np.zeros((z["h"], z["w"], 3), np.uint8)
+    grey = np.zeros((z["h"], z["w"]), np.float32)
+    lib().orc_prepare_view(img.ravel(), w, h, w * 3, max_side, rgb.ravel(), grey.ravel())
+    return rgb, grey
