@@ -39,6 +39,7 @@ EXPORTS = [
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
+    "mvs_survey_set_depth",
 ]
 
 
@@ -132,6 +133,7 @@ def lib():
         L.mvs_prepared_size.argtypes = [i32, i32, i32, P(C.c_int32), P(C.c_int32),
                                         P(C.c_double)]
         L.mvs_prepare_view.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
+        L.mvs_survey_set_depth.argtypes = [vp, i32, vp]
         L.mvs_survey_set_colour.argtypes = [vp, i32, vp]
         L.mvs_survey_fuse.argtypes = [vp, i32, vp, vp, vp, i32, f64, P(C.c_int64)]
         L.mvs_survey_fused_points.argtypes = [vp, vp, vp, i64, P(C.c_int64)]
@@ -500,6 +502,13 @@ class Survey:
         self.ctx.check(lib().mvs_survey_download(self._h, view, d.ctypes.data, n.ctypes.data,
                                                  c.ctypes.data))
         return d, n, c
+
+    def set_depth(self, view, depth):
+        """A depth map computed earlier (the `.depth` cache) as the view's map."""
+        h, w = self.shapes[view]
+        d = np.ascontiguousarray(depth, np.float32)
+        assert d.shape == (h, w)
+        self.ctx.check(lib().mvs_survey_set_depth(self._h, view, d.ctypes.data))
 
     def set_colour(self, view, rgb):
         c = np.ascontiguousarray(rgb, np.uint8)

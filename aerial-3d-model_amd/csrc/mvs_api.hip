@@ -1405,6 +1405,24 @@ int mvs_survey_download(mvs_survey *s, int32_t view, float *depth, float *normal
     return download_map(c, s->maps[(size_t)view], depth, normal, cost);
 }
 
+int mvs_survey_set_depth(mvs_survey *s, int32_t view, const float *depth) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    if (view < 0 || view >= (int32_t)s->views.size() || !s->views[(size_t)view].set)
+        return fail(c, MVS_ERR_ARG, "no such view");
+    if (!depth) return fail(c, MVS_ERR_ARG, "null depth map");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const ViewDev &v = s->views[(size_t)view];
+    MapDev &m = s->maps[(size_t)view];
+    TRY(ensure_map(c, m, v.w, v.h));
+    const size_t n = (size_t)v.w * v.h;
+    HIP_TRY(c, hipMemcpy(m.depth.p, depth, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(m.normal.p, 0, 3 * n * sizeof(float)));
+    HIP_TRY(c, hipMemset(m.cost.p, 0, n * sizeof(float)));
+    m.valid = true;
+    return MVS_OK;
+}
+
 int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int32_t nn,
                            int32_t rule, int32_t *agree, int64_t *n_consistent) {
     if (!s) return MVS_ERR_ARG;

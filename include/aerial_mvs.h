@@ -188,6 +188,10 @@ int mvs_survey_depth_maps_ex(mvs_survey *sv, int32_t njobs, const int32_t *refs,
                              mvs_progress_fn progress, void *user);
 
 int mvs_survey_download(mvs_survey *sv, int32_t view, float *depth, float *normal, float *cost);
+/* A depth map computed earlier (the `.depth` cache, dense.rs:533-536) as view
+ * `view`'s map, for consistency and fusion: w*h f32 in the view's size; its
+ * normals and costs read as zero. */
+int mvs_survey_set_depth(mvs_survey *sv, int32_t view, const float *depth);
 
 /* Geometric consistency of view ref's depth map against its neighbours' maps
  * (fusion filter, dense.rs:575-615): neighbours = the first prm.neighbours+1 of
