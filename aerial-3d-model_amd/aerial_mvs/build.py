@@ -46,7 +46,8 @@ def _run(cmd):
 
 UNITS = ["mvs_api.hip", "mvs_kernels.hip", "mvs_pm_t5.hip", "mvs_pm_t7.hip", "mvs_pm_t11.hip",
          "mvs_pm_t5f.hip", "mvs_pm_t7f.hip", "mvs_pm_t11f.hip", "mvs_pm_lane.hip",
-         "mvs_refine.hip", "mvs_refine_lane.hip", "mvs_fusion.hip", "mvs_prep.hip"]
+         "mvs_refine.hip", "mvs_refine_lane.hip", "mvs_fusion.hip", "mvs_prep.hip",
+         "mvs_dsm.hip"]
 
 
 def build_mvs(extra=None):

@@ -39,7 +39,7 @@ EXPORTS = [
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
-    "mvs_survey_set_depth",
+    "mvs_survey_set_depth", "mvs_build_dsm", "mvs_save_dsm", "mvs_load_dsm", "mvs_dsm_height",
 ]
 
 
@@ -134,6 +134,10 @@ def lib():
                                         P(C.c_double)]
         L.mvs_prepare_view.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
         L.mvs_survey_set_depth.argtypes = [vp, i32, vp]
+        L.mvs_build_dsm.argtypes = [vp, vp, i64, f64, i32, P(MvsDsmInfo), vp, i64]
+        L.mvs_save_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp]
+        L.mvs_load_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp, i64]
+        L.mvs_dsm_height.argtypes = [P(MvsDsmInfo), vp, f64, f64, P(C.c_float)]
         L.mvs_survey_set_colour.argtypes = [vp, i32, vp]
         L.mvs_survey_fuse.argtypes = [vp, i32, vp, vp, vp, i32, f64, P(C.c_int64)]
         L.mvs_survey_fused_points.argtypes = [vp, vp, vp, i64, P(C.c_int64)]
@@ -238,6 +242,49 @@ def load_view(path):
                            c.ctypes.data, g.size):
         return None
     return g, c
+
+
+class MvsDsmInfo(C.Structure):
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("cell", C.c_double),
+                ("nx", C.c_int64), ("ny", C.c_int64), ("zmax", C.c_float),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return dict(x0=self.x0, y0=self.y0, cell=self.cell, nx=self.nx, ny=self.ny,
+                    zmax=self.zmax)
+
+
+def _dsm_info(d):
+    return MvsDsmInfo(d["x0"], d["y0"], d["cell"], d["nx"], d["ny"], d.get("zmax", 0.0), 0)
+
+
+def save_dsm(path, info, h):
+    """``dsm.bin`` (Dsm::save, dsm.rs:27-46)."""
+    g = np.ascontiguousarray(h, np.float32)
+    rc = lib().mvs_save_dsm(os.fsencode(path), C.byref(_dsm_info(info)), g.ctypes.data)
+    if rc:
+        raise MvsError(rc, "save_dsm failed")
+
+
+def load_dsm(path):
+    """(info, h (ny, nx) f32), or None for a missing, damaged or truncated file."""
+    d = MvsDsmInfo()
+    if lib().mvs_load_dsm(os.fsencode(path), C.byref(d), None, 0):
+        return None
+    h = np.zeros((d.ny, d.nx), np.float32)
+    if lib().mvs_load_dsm(os.fsencode(path), C.byref(d), h.ctypes.data, h.size):
+        return None
+    return d.as_dict(), h
+
+
+def dsm_height(info, h, x, y):
+    """Dsm::height (dsm.rs:82-119): the height, or None outside."""
+    g = np.ascontiguousarray(h, np.float32)
+    out = C.c_float()
+    rc = lib().mvs_dsm_height(C.byref(_dsm_info(info)), g.ctypes.data, x, y, C.byref(out))
+    if rc < 0:
+        raise MvsError(rc, "bad surface model")
+    return out.value if rc == 1 else None
 
 
 def prepared_size(w, h, max_side):
@@ -408,6 +455,17 @@ class Context:
 
     def reset_stats(self):
         self.check(lib().mvs_reset_stats(self._h))
+
+    def build_dsm(self, xyz, voxel_m, profile=1):
+        """build_dsm (dsm.rs:128-290) on the GPU: (info dict, h (ny, nx) f32)."""
+        p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        d = MvsDsmInfo()
+        self.check(lib().mvs_build_dsm(self._h, p.ctypes.data, len(p), voxel_m, profile,
+                                       C.byref(d), None, 0))
+        h = np.zeros((d.ny, d.nx), np.float32)
+        self.check(lib().mvs_build_dsm(self._h, p.ctypes.data, len(p), voxel_m, profile,
+                                       C.byref(d), h.ctypes.data, h.size))
+        return d.as_dict(), h
 
     def prepare_view(self, rgb, max_side):
         """prepare_views' per-photo work on the GPU (dense.rs:100-112): an

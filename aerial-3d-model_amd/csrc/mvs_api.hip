@@ -113,6 +113,8 @@ struct mvs_ctx {
     MapDev tmp_map;
     DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
     DevBuf prep_box, prep_rgb, prep_grey;  // view preparation
+    DevBuf dsm_xyz, dsm_h, dsm_a, dsm_b, dsm_c, dsm_d, dsm_snap, dsm_parent, dsm_la, dsm_lb,
+        dsm_ctr;  // surface model
     DevBuf pairs, grad_io, gout;  // pair-cost lists
     std::map<RayKey, DevBuf *> rays;
     // host copies that must outlive the asynchronous uploads of a call
@@ -1017,7 +1019,9 @@ void mvs_destroy(mvs_ctx *c) {
                       &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost,
                       &c->pairs,    &c->grad_io, &c->gout,
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost,
-                      &c->prep_box, &c->prep_rgb, &c->prep_grey};
+                      &c->prep_box, &c->prep_rgb, &c->prep_grey,
+                      &c->dsm_xyz, &c->dsm_h, &c->dsm_a, &c->dsm_b, &c->dsm_c, &c->dsm_d,
+                      &c->dsm_snap, &c->dsm_parent, &c->dsm_la, &c->dsm_lb, &c->dsm_ctr};
     for (DevBuf *b : bufs) b->release();
     for (auto &v : c->tmp_views) {
         v.gray.release();
@@ -1855,6 +1859,194 @@ int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64
     }
     fclose(f);
     return MVS_OK;
+}
+
+// ---- surface model (recon/dsm.rs) ----
+
+namespace {
+
+// f32::total_cmp as a signed key (finite values and infinities in order)
+int32_t total_key(float v) {
+    int32_t x;
+    memcpy(&x, &v, 4);
+    return x ^ (int32_t)(((uint32_t)(x >> 31)) >> 1);
+}
+
+float from_total_key(int32_t k) {
+    const int32_t x = k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+    float v;
+    memcpy(&v, &x, 4);
+    return v;
+}
+
+// dsm.rs:141: v[((len - 1) as f64 * f) as usize] of the sorted values
+double order_stat(std::vector<int32_t> &keys, double f) {
+    const size_t k = (size_t)((double)(keys.size() - 1) * f);
+    std::nth_element(keys.begin(), keys.begin() + (ptrdiff_t)k, keys.end());
+    return (double)from_total_key(keys[k]);
+}
+
+// dsm.rs:129-150: extent from the 0.5-99.5th percentiles, cell edge, grid
+void dsm_grid(const float *xyz, int64_t n, double voxel_m, int64_t budget, DsmGrid &g) {
+    std::vector<int32_t> kx((size_t)n), ky((size_t)n);
+    for (int64_t p = 0; p < n; ++p) {
+        kx[(size_t)p] = total_key(xyz[3 * p]);
+        ky[(size_t)p] = total_key(xyz[3 * p + 1]);
+    }
+    const double xa = order_stat(kx, 0.005), xb = order_stat(kx, 0.995);
+    const double ya = order_stat(ky, 0.005), yb = order_stat(ky, 0.995);
+    const double area = (xb - xa) * (yb - ya);
+    g.cell = std::max(voxel_m * 1.5, std::sqrt(area / (double)budget));
+    g.x0 = xa;
+    g.y0 = ya;
+    g.nx = (int64_t)std::ceil((xb - xa) / g.cell) + 1;
+    g.ny = (int64_t)std::ceil((yb - ya) / g.cell) + 1;
+}
+
+}  // namespace
+
+int mvs_build_dsm(mvs_ctx *c, const float *xyz, int64_t n, double voxel_m, int32_t profile,
+                  mvs_dsm_info *info, float *h, int64_t cap) {
+    if (!c) return MVS_ERR_ARG;
+    static const int64_t budget[3] = {3000000, 8000000, 20000000};  // dsm.rs:122-128
+    if (!info || profile < 0 || profile > 2 || !(voxel_m > 0.0) || n < 0 || (n > 0 && !xyz))
+        return fail(c, MVS_ERR_ARG, "bad surface model arguments");
+    if (n < 100) return fail(c, MVS_ERR_ARG, "the point cloud is too small to build a surface");
+    DsmGrid g;
+    dsm_grid(xyz, n, voxel_m, budget[profile], g);
+    info->x0 = g.x0;
+    info->y0 = g.y0;
+    info->cell = g.cell;
+    info->nx = g.nx;
+    info->ny = g.ny;
+    info->zmax = NAN;
+    info->reserved = 0;
+    if (!h) return MVS_OK;
+    const int64_t nc = g.nx * g.ny;
+    if (!(g.nx > 0 && g.ny > 0) || nc >= (1ll << 31))
+        return fail(c, MVS_ERR_ARG, "surface grid too large (2^31 cells)");
+    if (cap < nc) return fail(c, MVS_ERR_ARG, "output array too small");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t cb = (size_t)nc * 4;
+    HIP_TRY(c, c->dsm_xyz.ensure((size_t)n * 12));
+    HIP_TRY(c, c->dsm_h.ensure(cb));
+    DevBuf *bufs[] = {&c->dsm_a, &c->dsm_b, &c->dsm_c, &c->dsm_d, &c->dsm_snap, &c->dsm_parent,
+                      &c->dsm_la, &c->dsm_lb};
+    for (DevBuf *b : bufs) HIP_TRY(c, b->ensure(cb));
+    HIP_TRY(c, c->dsm_ctr.ensure(16));
+    HIP_TRY(c, hipMemcpyAsync(c->dsm_xyz.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, c->st));
+    DsmWork w{c->dsm_a.as<uint32_t>(),  c->dsm_b.as<uint32_t>(),  c->dsm_c.as<uint32_t>(),
+              c->dsm_d.as<uint32_t>(),  c->dsm_snap.as<float>(),  c->dsm_parent.as<int32_t>(),
+              c->dsm_la.as<int32_t>(),  c->dsm_lb.as<int32_t>(),  c->dsm_ctr.as<uint32_t>()};
+    float zmax = 0.0f;
+    if (dsm_build(g, c->dsm_xyz.as<float>(), n, c->dsm_h.as<float>(), w, c->st, &zmax))
+        return hip_fail(c, hipGetLastError(), "surface model kernels");
+    HIP_TRY(c, hipMemcpyAsync(h, c->dsm_h.p, cb, hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(c, hipStreamSynchronize(c->st));
+    info->zmax = zmax;
+    return MVS_OK;
+}
+
+int mvs_save_dsm(const char *path, const mvs_dsm_info *info, const float *h) {
+    if (!path || !info || info->nx < 0 || info->ny < 0 || (info->nx * info->ny > 0 && !h))
+        return MVS_ERR_ARG;
+    // Dsm::save (dsm.rs:27-46): "DSM1", x0 y0 cell f64, nx ny u64, zmax f32,
+    // heights f32, all little-endian; tmp + rename
+    std::string tmp = std::string(path);
+    const size_t slash = tmp.find_last_of('/');
+    const size_t dot = tmp.find_last_of('.');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash))
+        tmp = tmp.substr(0, dot);
+    tmp += ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return MVS_ERR_IO;
+    const double d3[3] = {info->x0, info->y0, info->cell};
+    const uint64_t n2[2] = {(uint64_t)info->nx, (uint64_t)info->ny};
+    const size_t nc = (size_t)(info->nx * info->ny);
+    bool ok = fwrite("DSM1", 1, 4, f) == 4;
+    ok = ok && fwrite(d3, 8, 3, f) == 3;
+    ok = ok && fwrite(n2, 8, 2, f) == 2;
+    ok = ok && fwrite(&info->zmax, 4, 1, f) == 1;
+    ok = ok && (nc == 0 || fwrite(h, 4, nc, f) == nc);
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path) != 0) return MVS_ERR_IO;
+    return MVS_OK;
+}
+
+int mvs_load_dsm(const char *path, mvs_dsm_info *info, float *h, int64_t cap) {
+    if (!path || !info) return MVS_ERR_ARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return MVS_ERR_IO;
+    unsigned char hdr[48];
+    const size_t got = fread(hdr, 1, 48, f);
+    fseek(f, 0, SEEK_END);
+    const long len = ftell(f);
+    // Dsm::load (dsm.rs:48-75): "damaged" below 48 bytes or without the tag
+    if (got < 48 || memcmp(hdr, "DSM1", 4) != 0) {
+        fclose(f);
+        return MVS_ERR_FORMAT;
+    }
+    double d3[3];
+    uint64_t n2[2];
+    memcpy(d3, hdr + 4, 24);
+    memcpy(n2, hdr + 28, 16);
+    memcpy(&info->zmax, hdr + 44, 4);
+    info->x0 = d3[0];
+    info->y0 = d3[1];
+    info->cell = d3[2];
+    info->nx = (int64_t)n2[0];
+    info->ny = (int64_t)n2[1];
+    info->reserved = 0;
+    const uint64_t nc = n2[0] * n2[1];
+    if (n2[0] > (1ull << 31) || n2[1] > (1ull << 31) || (uint64_t)len != 48 + 4 * nc) {
+        fclose(f);
+        return MVS_ERR_FORMAT;  // "truncated surface model file"
+    }
+    if (h) {
+        if (cap < (int64_t)nc) {
+            fclose(f);
+            return MVS_ERR_ARG;
+        }
+        fseek(f, 48, SEEK_SET);
+        const bool ok = fread(h, 4, (size_t)nc, f) == (size_t)nc;
+        fclose(f);
+        return ok ? MVS_OK : MVS_ERR_IO;
+    }
+    fclose(f);
+    return MVS_OK;
+}
+
+int mvs_dsm_height(const mvs_dsm_info *d, const float *h, double x, double y, float *out) {
+    if (!d || !h || !out || d->nx < 1 || d->ny < 1) return MVS_ERR_ARG;
+    // Dsm::height (dsm.rs:82-119): bilinear between cell centres over the
+    // finite samples, the nearest valid one on an empty sample, None outside
+    const double fx = (x - d->x0) / d->cell - 0.5;
+    const double fy = (y - d->y0) / d->cell - 0.5;
+    if (fx < -0.5 || fy < -0.5 || fx > (double)d->nx - 0.5 || fy > (double)d->ny - 0.5) return 0;
+    const int64_t i0 = std::min((int64_t)std::max(std::floor(fx), 0.0), d->nx - 1);
+    const int64_t j0 = std::min((int64_t)std::max(std::floor(fy), 0.0), d->ny - 1);
+    const int64_t i1 = std::min(i0 + 1, d->nx - 1), j1 = std::min(j0 + 1, d->ny - 1);
+    const float ax = (float)std::min(std::max(fx - (double)i0, 0.0), 1.0);
+    const float ay = (float)std::min(std::max(fy - (double)j0, 0.0), 1.0);
+    const float v[4] = {h[j0 * d->nx + i0], h[j0 * d->nx + i1], h[j1 * d->nx + i0],
+                        h[j1 * d->nx + i1]};
+    const float wt[4] = {(1.0f - ax) * (1.0f - ay), ax * (1.0f - ay), (1.0f - ax) * ay, ax * ay};
+    float s = 0.0f, ws = 0.0f;
+    for (int k = 0; k < 4; ++k)
+        if (std::isfinite(v[k])) {
+            s += v[k] * wt[k];
+            ws += wt[k];
+        }
+    if (ws > 1e-6f) {
+        *out = s / ws;
+        return 1;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (std::isfinite(v[k])) {
+            *out = v[k];
+            return 1;
+        }
+    return 0;
 }
 
 int mvs_prepared_size(int32_t w, int32_t h, int32_t max_side, int32_t *out_w, int32_t *out_h,

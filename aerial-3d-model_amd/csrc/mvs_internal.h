@@ -176,6 +176,22 @@ void launch_prepare_view(const uint8_t *src, int32_t w, int32_t h, int32_t strid
                          const PrepSizes &z, uint8_t *boxed, uint8_t *out_rgb, float *grey,
                          hipStream_t st);
 
+// DSM (mvs_dsm.hip): the grid of dsm.rs:129-150 and the device work buffers
+// (ncell each; list_a/list_b int32; counters 3 x u32).
+struct DsmGrid {
+    double x0, y0, cell;
+    int64_t nx, ny;
+};
+struct DsmWork {
+    uint32_t *u32a, *u32b, *u32c, *u32d;
+    float *snap;
+    int32_t *parent, *list_a, *list_b;
+    uint32_t *counters;
+};
+// Heights of the grid from n device points (xyz f32) into d_h; 0 or -1 (HIP error).
+int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWork &w,
+              hipStream_t st, float *zmax);
+
 // Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
 int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
                      const int32_t *py, const float4 *hyp, int32_t *pairs, const float *gout,

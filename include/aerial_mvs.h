@@ -315,6 +315,33 @@ int mvs_load_view(const char *path, int32_t *w, int32_t *h, float *gray, uint8_t
 int mvs_view_intrinsics(double f, double cx, double cy, int32_t full_width, int32_t w,
                         double out[3]);
 
+/* ---- surface model (recon/dsm.rs) ---- */
+typedef struct {
+    double x0, y0;   /* world coordinate of the south-west corner of cell (0, 0) */
+    double cell;     /* cell edge, m */
+    int64_t nx, ny;  /* grid; row 0 is the southernmost */
+    float zmax;      /* highest height (-FLT_MAX without surface; NaN until built) */
+    int32_t reserved;
+} mvs_dsm_info;
+/* build_dsm (dsm.rs:128-290) on the GPU from a cloud of n points (xyz f32, as
+ * read from dense.ply), the fusion's voxel edge and the profile (0 Lite, 1
+ * Standard, 2 High: a budget of 3, 8, 20 M cells, dsm.rs:122-128). With
+ * h == NULL only the grid is computed (info; nx * ny floats to allocate); else
+ * h (cap floats) receives the heights, row-major from the south, NaN = no
+ * surface. MVS_ERR_ARG with "the point cloud is too small to build a surface"
+ * below 100 points. */
+int mvs_build_dsm(mvs_ctx *ctx, const float *xyz, int64_t n, double voxel_m, int32_t profile,
+                  mvs_dsm_info *info, float *h, int64_t cap);
+/* Dsm::save / Dsm::load (dsm.rs:27-75): the `dsm.bin` layout ("DSM1", x0, y0,
+ * cell f64, nx, ny u64, zmax f32, heights f32, little-endian; tmp + rename).
+ * load: MVS_ERR_FORMAT for a damaged or truncated file; h (cap floats)
+ * nullable to read the header only. */
+int mvs_save_dsm(const char *path, const mvs_dsm_info *info, const float *h);
+int mvs_load_dsm(const char *path, mvs_dsm_info *info, float *h, int64_t cap);
+/* Dsm::height (dsm.rs:82-119): 1 with *out, 0 for None (outside), or an
+ * error code. Host-only. */
+int mvs_dsm_height(const mvs_dsm_info *info, const float *h, double x, double y, float *out);
+
 /* ---- view preparation (dense.rs:92-121, imgops.rs:89-171) ---- */
 /* shrink_to's output size for a w x h photo and working size max_side
  * (imgops.rs:145-171): unchanged when the long side fits, else an integer box

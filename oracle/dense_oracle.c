@@ -18,6 +18,7 @@
  */
 #include "dense_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -945,4 +946,292 @@ void orc_prepare_view(const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
     const int64_t n = (int64_t)ow * oh;
     for (int64_t i = 0; i < n; ++i)
         grey[i] = orc_gray_rgb(out_rgb[3 * i], out_rgb[3 * i + 1], out_rgb[3 * i + 2]);
+}
+
+
+/* ---- DSM (recon/dsm.rs) ---------------------------------------------------- */
+
+void orc_free(void *p) { free(p); }
+
+static int cmp_f32_total(const void *a, const void *b) {
+    /* f32::total_cmp on the values the DSM sorts (finite ones) */
+    int32_t x, y;
+    memcpy(&x, a, 4);
+    memcpy(&y, b, 4);
+    x ^= (int32_t)(((uint32_t)(x >> 31)) >> 1);
+    y ^= (int32_t)(((uint32_t)(y >> 31)) >> 1);
+    return (x > y) - (x < y);
+}
+
+/* dsm.rs:183-201: median of the finite values in the (2r+1)^2 window, None
+ * (returns 0) when fewer than 5. */
+static int dsm_med(const float *h, int64_t nx, int64_t ny, int64_t i, int64_t j, int64_t r,
+                   float *out) {
+    float v[25];
+    int n = 0;
+    for (int64_t dj = -r; dj <= r; ++dj)
+        for (int64_t di = -r; di <= r; ++di) {
+            const int64_t a = i + di, b = j + dj;
+            if (a >= 0 && b >= 0 && a < nx && b < ny) {
+                const float x = h[b * nx + a];
+                if (isfinite(x)) v[n++] = x;
+            }
+        }
+    if (n < 5) return 0;
+    qsort(v, (size_t)n, sizeof(float), cmp_f32_total);
+    *out = v[n / 2];
+    return 1;
+}
+
+/* dsm.rs:293-331: 4-connected components of valid cells smaller than
+ * min_cells become empty. */
+void orc_remove_islands(float *h, int64_t nx, int64_t ny, int64_t min_cells) {
+    const int64_t n = nx * ny;
+    uint8_t *seen = (uint8_t *)calloc((size_t)n, 1);
+    int64_t *stack = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t *comp = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    for (int64_t start = 0; start < n; ++start) {
+        if (!isfinite(h[start]) || seen[start]) continue;
+        int64_t sp = 0, nc = 0;
+        stack[sp++] = start;
+        seen[start] = 1;
+        while (sp) {
+            const int64_t k = stack[--sp];
+            comp[nc++] = k;
+            const int64_t i = k % nx, j = k / nx;
+            const int64_t nb[4] = {i > 0 ? k - 1 : -1, i + 1 < nx ? k + 1 : -1,
+                                   j > 0 ? k - nx : -1, j + 1 < ny ? k + nx : -1};
+            for (int q = 0; q < 4; ++q) {
+                const int64_t kk = nb[q];
+                if (kk >= 0 && isfinite(h[kk]) && !seen[kk]) {
+                    seen[kk] = 1;
+                    stack[sp++] = kk;
+                }
+            }
+        }
+        if (nc < min_cells)
+            for (int64_t q = 0; q < nc; ++q) h[comp[q]] = NAN;
+    }
+    free(seen);
+    free(stack);
+    free(comp);
+}
+
+/* dsm.rs:335-390: empty cells not reachable from the border through empty
+ * cells are filled by repeated 4-neighbour averaging of a snapshot. */
+void orc_fill_enclosed(float *h, int64_t nx, int64_t ny) {
+    const int64_t n = nx * ny;
+    uint8_t *outside = (uint8_t *)calloc((size_t)n, 1);
+    /* every cell is marked once and pushes at most 4: border + 4 n entries */
+    int64_t *stack = (int64_t *)malloc((size_t)(4 * n + 2 * nx + 2 * ny + 4) * sizeof(int64_t));
+    int64_t sp = 0;
+    for (int64_t i = 0; i < nx; ++i) {
+        stack[sp++] = i;
+        stack[sp++] = (ny - 1) * nx + i;
+    }
+    for (int64_t j = 0; j < ny; ++j) {
+        stack[sp++] = j * nx;
+        stack[sp++] = j * nx + nx - 1;
+    }
+    while (sp) {
+        const int64_t k = stack[--sp];
+        if (isfinite(h[k]) || outside[k]) continue;
+        outside[k] = 1;
+        const int64_t i = k % nx, j = k / nx;
+        if (i > 0) stack[sp++] = k - 1;
+        if (i + 1 < nx) stack[sp++] = k + 1;
+        if (j > 0) stack[sp++] = k - nx;
+        if (j + 1 < ny) stack[sp++] = k + nx;
+    }
+    int64_t *todo = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t nt = 0;
+    for (int64_t k = 0; k < n; ++k)
+        if (!isfinite(h[k]) && !outside[k]) todo[nt++] = k;
+    float *snap = (float *)malloc((size_t)n * sizeof(float));
+    for (int guard = 0; nt > 0 && guard < 10000; ++guard) {
+        memcpy(snap, h, (size_t)n * sizeof(float));
+        int64_t kept = 0;
+        for (int64_t t = 0; t < nt; ++t) {
+            const int64_t k = todo[t], i = k % nx, j = k / nx;
+            float s = 0.0f;
+            int c = 0;
+            const int64_t d[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+            for (int q = 0; q < 4; ++q) {
+                const int64_t a = i + d[q][0], b = j + d[q][1];
+                if (a >= 0 && b >= 0 && a < nx && b < ny) {
+                    const float x = snap[b * nx + a];
+                    if (isfinite(x)) {
+                        s += x;
+                        ++c;
+                    }
+                }
+            }
+            if (c > 0)
+                h[k] = s / (float)c;
+            else
+                todo[kept++] = k;
+        }
+        nt = kept;
+    }
+    free(snap);
+    free(todo);
+    free(outside);
+    free(stack);
+}
+
+static double dsm_q(float *v, int64_t n, double f) {
+    /* dsm.rs:141: v[((len - 1) as f64 * f) as usize] of the sorted values */
+    const int64_t k = (int64_t)((double)(n - 1) * f);
+    qsort(v, (size_t)n, sizeof(float), cmp_f32_total);
+    return (double)v[k];
+}
+
+int orc_build_dsm(const float *xyz, int64_t n, double voxel_m, int64_t budget, orc_dsm *out,
+                  float **hout) {
+    *hout = NULL;
+    if (n < 100) return -1;
+    float *xs = (float *)malloc((size_t)n * sizeof(float));
+    float *ys = (float *)malloc((size_t)n * sizeof(float));
+    for (int64_t p = 0; p < n; ++p) {
+        xs[p] = xyz[3 * p];
+        ys[p] = xyz[3 * p + 1];
+    }
+    const double xa = dsm_q(xs, n, 0.005), xb = dsm_q(xs, n, 0.995);
+    const double ya = dsm_q(ys, n, 0.005), yb = dsm_q(ys, n, 0.995);
+    free(xs);
+    free(ys);
+    const double area = (xb - xa) * (yb - ya);
+    double cell = voxel_m * 1.5;
+    const double c2 = sqrt(area / (double)budget);
+    if (c2 > cell) cell = c2;
+    const int64_t nx = (int64_t)ceil((xb - xa) / cell) + 1;
+    const int64_t ny = (int64_t)ceil((yb - ya) / cell) + 1;
+    const int64_t nc = nx * ny;
+    float *top = (float *)malloc((size_t)nc * 3 * sizeof(float));
+    uint16_t *count = (uint16_t *)calloc((size_t)nc, sizeof(uint16_t));
+    for (int64_t k = 0; k < 3 * nc; ++k) top[k] = -INFINITY;
+    for (int64_t p = 0; p < n; ++p) {
+        const double i = floor(((double)xyz[3 * p] - xa) / cell);
+        const double j = floor(((double)xyz[3 * p + 1] - ya) / cell);
+        if (i < 0.0 || j < 0.0 || i >= (double)nx || j >= (double)ny) continue;
+        // a NaN coordinate passes the test above; Rust's `as usize` makes it 0
+        const int64_t k = (int64_t)(isnan(j) ? 0.0 : j) * nx + (int64_t)(isnan(i) ? 0.0 : i);
+        if (count[k] < 65535) ++count[k];
+        float *t = top + 3 * k;
+        const float z = xyz[3 * p + 2];
+        if (z > t[0]) {
+            t[2] = t[1];
+            t[1] = t[0];
+            t[0] = z;
+        } else if (z > t[1]) {
+            t[2] = t[1];
+            t[1] = z;
+        } else if (z > t[2]) {
+            t[2] = z;
+        }
+    }
+    float *h = (float *)malloc((size_t)nc * sizeof(float));
+    for (int64_t k = 0; k < nc; ++k)
+        h[k] = count[k] == 0 ? NAN : (count[k] == 1 ? top[3 * k] : top[3 * k + 1]);
+    free(top);
+    /* despike (dsm.rs:202-216) */
+    float *snap = (float *)malloc((size_t)nc * sizeof(float));
+    memcpy(snap, h, (size_t)nc * sizeof(float));
+    for (int64_t j = 0; j < ny; ++j)
+        for (int64_t i = 0; i < nx; ++i) {
+            const int64_t k = j * nx + i;
+            if (!isfinite(snap[k])) continue;
+            float m;
+            if (dsm_med(snap, nx, ny, i, j, 2, &m)) {
+                if (fabsf(snap[k] - m) > 3.0f && count[k] <= 2) h[k] = m;
+            } else if (count[k] <= 2) {
+                h[k] = NAN;
+            }
+        }
+    free(count);
+    orc_remove_islands(h, nx, ny, (int64_t)(25.0 / (cell * cell)));
+    orc_fill_enclosed(h, nx, ny);
+    /* small holes (dsm.rs:226-257): 8 passes, 8 neighbours, at least 3 */
+    static const int64_t d8[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1},
+                                     {-1, -1}, {1, 1}, {-1, 1}, {1, -1}};
+    for (int pass = 0; pass < 8; ++pass) {
+        memcpy(snap, h, (size_t)nc * sizeof(float));
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                const int64_t k = j * nx + i;
+                if (isfinite(snap[k])) continue;
+                float s = 0.0f;
+                int c = 0;
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t a = i + d8[q][0], b = j + d8[q][1];
+                    if (a >= 0 && b >= 0 && a < nx && b < ny) {
+                        const float x = snap[b * nx + a];
+                        if (isfinite(x)) {
+                            s += x;
+                            ++c;
+                        }
+                    }
+                }
+                if (c >= 3) h[k] = s / (float)c;
+            }
+    }
+    /* 3x3 median (dsm.rs:259-270) */
+    memcpy(snap, h, (size_t)nc * sizeof(float));
+    for (int64_t j = 0; j < ny; ++j)
+        for (int64_t i = 0; i < nx; ++i) {
+            const int64_t k = j * nx + i;
+            float m;
+            if (isfinite(snap[k]) && dsm_med(snap, nx, ny, i, j, 1, &m)) h[k] = m;
+        }
+    free(snap);
+    float zmax = -FLT_MAX;
+    for (int64_t k = 0; k < nc; ++k)
+        if (isfinite(h[k])) zmax = fmaxf(zmax, h[k]);
+    out->x0 = xa;
+    out->y0 = ya;
+    out->cell = cell;
+    out->nx = nx;
+    out->ny = ny;
+    out->zmax = zmax;
+    *hout = h;
+    return 0;
+}
+
+/* dsm.rs:82-119 */
+int orc_dsm_height(const orc_dsm *d, const float *h, double x, double y, float *out) {
+    const double fx = (x - d->x0) / d->cell - 0.5;
+    const double fy = (y - d->y0) / d->cell - 0.5;
+    if (fx < -0.5 || fy < -0.5 || fx > (double)d->nx - 0.5 || fy > (double)d->ny - 0.5) return 0;
+    double ffx = floor(fx), ffy = floor(fy);
+    if (ffx < 0.0) ffx = 0.0;
+    if (ffy < 0.0) ffy = 0.0;
+    int64_t i0 = (int64_t)ffx, j0 = (int64_t)ffy;
+    if (i0 > d->nx - 1) i0 = d->nx - 1;
+    if (j0 > d->ny - 1) j0 = d->ny - 1;
+    const int64_t i1 = i0 + 1 < d->nx - 1 ? i0 + 1 : d->nx - 1;
+    const int64_t j1 = j0 + 1 < d->ny - 1 ? j0 + 1 : d->ny - 1;
+    double ax = fx - (double)i0, ay = fy - (double)j0;
+    ax = ax < 0.0 ? 0.0 : (ax > 1.0 ? 1.0 : ax);
+    ay = ay < 0.0 ? 0.0 : (ay > 1.0 ? 1.0 : ay);
+    const float fax = (float)ax, fay = (float)ay;
+    const float v[4] = {h[j0 * d->nx + i0], h[j0 * d->nx + i1], h[j1 * d->nx + i0],
+                        h[j1 * d->nx + i1]};
+    const float w[4] = {(1.0f - fax) * (1.0f - fay), fax * (1.0f - fay), (1.0f - fax) * fay,
+                        fax * fay};
+    float s = 0.0f, ws = 0.0f;
+    for (int k = 0; k < 4; ++k)
+        if (isfinite(v[k])) {
+            s += v[k] * w[k];
+            ws += w[k];
+        }
+    if (ws > 1e-6f) {
+        *out = s / ws;
+        return 1;
+    }
+    for (int k = 0; k < 4; ++k)
+        if (isfinite(v[k])) {
+            *out = v[k];
+            return 1;
+        }
+    return 0;
 }

@@ -142,6 +142,24 @@ void orc_resize_bilinear(const uint8_t *src, int32_t sw, int32_t sh, int32_t str
 void orc_prepare_view(const uint8_t *rgb, int32_t w, int32_t h, int32_t stride,
                       int32_t max_side, uint8_t *out_rgb, float *grey);
 
+/* DSM from a point cloud (recon/dsm.rs:128-290). Cloud: n points xyz f32 (as
+ * read from dense.ply). budget = cells of the profile (dsm.rs:122-128).
+ * Returns 0, or -1 for fewer than 100 points. h (nx*ny, row 0 south) is
+ * allocated by the oracle; free with orc_free. */
+typedef struct {
+    double x0, y0, cell;
+    int64_t nx, ny;
+    float zmax;
+} orc_dsm;
+int orc_build_dsm(const float *xyz, int64_t n, double voxel_m, int64_t budget, orc_dsm *out,
+                  float **h);
+void orc_free(void *p);
+/* pieces, for the pins: dsm.rs:293-331 and :335-390 on an nx x ny grid */
+void orc_remove_islands(float *h, int64_t nx, int64_t ny, int64_t min_cells);
+void orc_fill_enclosed(float *h, int64_t nx, int64_t ny);
+/* Dsm::height (dsm.rs:82-119): 1 and *out, or 0 for None */
+int orc_dsm_height(const orc_dsm *d, const float *h, double x, double y, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,11 @@ class OrcNbrDepth(C.Structure):
 _lib = None
 
 
+class OrcDsm(C.Structure):
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("cell", C.c_double),
+                ("nx", C.c_int64), ("ny", C.c_int64), ("zmax", C.c_float)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -107,6 +112,15 @@ def lib():
                                           C.c_int32, C.c_int32]
         L.orc_prepare_view.argtypes = [_u8p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _u8p,
                                        _f32p]
+        L.orc_build_dsm.argtypes = [_f32p, C.c_int64, C.c_double, C.c_int64,
+                                    C.POINTER(OrcDsm), C.POINTER(C.POINTER(C.c_float))]
+        L.orc_build_dsm.restype = C.c_int
+        L.orc_free.argtypes = [C.c_void_p]
+        L.orc_remove_islands.argtypes = [_f32p, C.c_int64, C.c_int64, C.c_int64]
+        L.orc_fill_enclosed.argtypes = [_f32p, C.c_int64, C.c_int64]
+        L.orc_dsm_height.argtypes = [C.POINTER(OrcDsm), _f32p, C.c_double, C.c_double,
+                                     C.POINTER(C.c_float)]
+        L.orc_dsm_height.restype = C.c_int
         _lib = L
     return _lib
 
@@ -413,3 +427,42 @@ def prepare_view(img, max_side):
     grey = np.zeros((z["h"], z["w"]), np.float32)
     lib().orc_prepare_view(img.ravel(), w, h, w * 3, max_side, rgb.ravel(), grey.ravel())
     return rgb, grey
+
+
+# ---- DSM (recon/dsm.rs) ------------------------------------------------------
+
+DSM_BUDGET = {0: 3_000_000, 1: 8_000_000, 2: 20_000_000}  # dsm.rs:122-128
+
+
+def build_dsm(xyz, voxel_m, budget):
+    """dsm.rs:128-290 -> (dict(x0, y0, cell, nx, ny, zmax), h (ny, nx) f32), or
+    None for fewer than 100 points."""
+    p = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    d = OrcDsm()
+    hp = C.POINTER(C.c_float)()
+    if lib().orc_build_dsm(p.ravel(), len(p), voxel_m, budget, C.byref(d), C.byref(hp)):
+        return None
+    n = d.nx * d.ny
+    h = np.ctypeslib.as_array(hp, shape=(n,)).copy().reshape(d.ny, d.nx)
+    lib().orc_free(hp)
+    return dict(x0=d.x0, y0=d.y0, cell=d.cell, nx=d.nx, ny=d.ny, zmax=d.zmax), h
+
+
+def remove_islands(h, min_cells):
+    g = np.ascontiguousarray(h, np.float32).copy()
+    lib().orc_remove_islands(g.ravel(), g.shape[1], g.shape[0], min_cells)
+    return g
+
+
+def fill_enclosed(h):
+    g = np.ascontiguousarray(h, np.float32).copy()
+    lib().orc_fill_enclosed(g.ravel(), g.shape[1], g.shape[0])
+    return g
+
+
+def dsm_height(info, h, x, y):
+    d = OrcDsm(info["x0"], info["y0"], info["cell"], info["nx"], info["ny"], info["zmax"])
+    out = C.c_float()
+    g = np.ascontiguousarray(h, np.float32)
+    ok = lib().orc_dsm_height(C.byref(d), g.ravel(), x, y, C.byref(out))
+    return out.value if ok else None
