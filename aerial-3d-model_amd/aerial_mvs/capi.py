@@ -39,7 +39,8 @@ EXPORTS = [
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
-    "mvs_survey_set_depth", "mvs_build_dsm", "mvs_save_dsm", "mvs_load_dsm", "mvs_dsm_height",
+    "mvs_survey_set_depth", "mvs_build_dsm", "mvs_dsm_download", "mvs_save_dsm",
+    "mvs_load_dsm", "mvs_dsm_height",
 ]
 
 
@@ -135,6 +136,7 @@ def lib():
         L.mvs_prepare_view.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
         L.mvs_survey_set_depth.argtypes = [vp, i32, vp]
         L.mvs_build_dsm.argtypes = [vp, vp, i64, f64, i32, P(MvsDsmInfo), vp, i64]
+        L.mvs_dsm_download.argtypes = [vp, vp, i64]
         L.mvs_save_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp]
         L.mvs_load_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp, i64]
         L.mvs_dsm_height.argtypes = [P(MvsDsmInfo), vp, f64, f64, P(C.c_float)]
@@ -462,9 +464,8 @@ class Context:
         d = MvsDsmInfo()
         self.check(lib().mvs_build_dsm(self._h, p.ctypes.data, len(p), voxel_m, profile,
                                        C.byref(d), None, 0))
-        h = np.zeros((d.ny, d.nx), np.float32)
-        self.check(lib().mvs_build_dsm(self._h, p.ctypes.data, len(p), voxel_m, profile,
-                                       C.byref(d), h.ctypes.data, h.size))
+        h = np.empty((d.ny, d.nx), np.float32)
+        self.check(lib().mvs_dsm_download(self._h, h.ctypes.data, h.size))
         return d.as_dict(), h
 
     def prepare_view(self, rgb, max_side):

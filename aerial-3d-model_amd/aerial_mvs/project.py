@@ -10,7 +10,9 @@ Mirrors, over the C ABI:
 - ``reconstruct_dense`` (dense.rs:516-651): prepare missing views, compute the
   missing depth maps (here all in one survey batch), fuse, drop the views;
 - ``run_dense`` (recon/mod.rs:66-87): ``dense.ply``, ``view.ply`` and
-  ``dense.voxel``.
+  ``dense.voxel``;
+- the surface-model step of ``run_ortho`` (recon/mod.rs:91-98): ``dense.ply``
+  and ``dense.voxel`` in, ``dsm.bin`` out (the orthophoto is not built here).
 
 JSON layout. ``Pose.r`` is a nalgebra ``Matrix3<f64>``; nalgebra 0.33's serde
 writes a matrix's storage, a flat array in column-major order, although the
@@ -278,3 +280,34 @@ def run_dense(project_dir, ctx, kind=1, decode=None, prm=None, order="column", p
     with open(os.path.join(odir, "dense.voxel"), "w") as f:
         f.write(rust_f64(cloud.voxel_m))
     return len(cloud.points), cloud.voxel_m
+
+
+def dense_voxel(project_dir):
+    """recon/mod.rs:102-109: the voxel edge ``run_dense`` recorded, or None
+    (missing file, or text Rust's f64 parser refuses)."""
+    try:
+        with open(os.path.join(out_dir(project_dir), "dense.voxel")) as f:
+            t = f.read().strip()
+    except OSError:
+        return None
+    if "_" in t:  # Python's float() takes digit separators, Rust's parse does not
+        return None
+    try:
+        return float(t)
+    except ValueError:
+        return None
+
+
+def run_dsm(project_dir, ctx, kind=1):
+    """The surface model of ``run_ortho`` (recon/mod.rs:91-98) on the GPU:
+    ``dense.ply`` and ``dense.voxel`` (0.25 m when absent) in, ``dsm.bin`` out.
+    kind: the profile (0 Lite, 1 Standard, 2 High: 3, 8, 20 M cells,
+    dsm.rs:121-128). Returns (info, heights)."""
+    odir = out_dir(project_dir)
+    voxel = dense_voxel(project_dir)
+    if voxel is None:
+        voxel = 0.25
+    xyz, _ = capi.read_ply(os.path.join(odir, "dense.ply"))
+    info, h = ctx.build_dsm(xyz, voxel, kind)
+    capi.save_dsm(os.path.join(odir, "dsm.bin"), info, h)
+    return info, h

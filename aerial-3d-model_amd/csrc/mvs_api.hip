@@ -114,7 +114,8 @@ struct mvs_ctx {
     DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
     DevBuf prep_box, prep_rgb, prep_grey;  // view preparation
     DevBuf dsm_xyz, dsm_h, dsm_a, dsm_b, dsm_c, dsm_d, dsm_snap, dsm_parent, dsm_la, dsm_lb,
-        dsm_ctr;  // surface model
+        dsm_ctr, dsm_sel;  // surface model
+    int64_t dsm_cells = 0;  // cells of the last grid built (kept in dsm_h)
     DevBuf pairs, grad_io, gout;  // pair-cost lists
     std::map<RayKey, DevBuf *> rays;
     // host copies that must outlive the asynchronous uploads of a call
@@ -1021,7 +1022,7 @@ void mvs_destroy(mvs_ctx *c) {
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost,
                       &c->prep_box, &c->prep_rgb, &c->prep_grey,
                       &c->dsm_xyz, &c->dsm_h, &c->dsm_a, &c->dsm_b, &c->dsm_c, &c->dsm_d,
-                      &c->dsm_snap, &c->dsm_parent, &c->dsm_la, &c->dsm_lb, &c->dsm_ctr};
+                      &c->dsm_snap, &c->dsm_parent, &c->dsm_la, &c->dsm_lb, &c->dsm_ctr, &c->dsm_sel};
     for (DevBuf *b : bufs) b->release();
     for (auto &v : c->tmp_views) {
         v.gray.release();
@@ -1621,7 +1622,7 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
     int64_t cap = 1 << 16;
     while (cap < pixels / 32 && cap < (1ll << 30)) cap <<= 1;
     std::vector<NbrCam> cams;
-    for (int attempt = 0;; ++attempt) {
+    for (;;) {  // ends: the table grows to at most 2^30 slots
         HIP_TRY(c, s->fkeys.ensure((size_t)cap * 8));
         HIP_TRY(c, s->fsum.ensure((size_t)cap * 24));
         HIP_TRY(c, s->fcol.ensure((size_t)cap * 12));
@@ -1865,42 +1866,21 @@ int mvs_load_depth(const char *path, int32_t *w, int32_t *h, float *depth, int64
 
 namespace {
 
-// f32::total_cmp as a signed key (finite values and infinities in order)
-int32_t total_key(float v) {
-    int32_t x;
-    memcpy(&x, &v, 4);
-    return x ^ (int32_t)(((uint32_t)(x >> 31)) >> 1);
-}
-
-float from_total_key(int32_t k) {
-    const int32_t x = k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
-    float v;
-    memcpy(&v, &x, 4);
-    return v;
-}
-
-// dsm.rs:141: v[((len - 1) as f64 * f) as usize] of the sorted values
-double order_stat(std::vector<int32_t> &keys, double f) {
-    const size_t k = (size_t)((double)(keys.size() - 1) * f);
-    std::nth_element(keys.begin(), keys.begin() + (ptrdiff_t)k, keys.end());
-    return (double)from_total_key(keys[k]);
-}
-
-// dsm.rs:129-150: extent from the 0.5-99.5th percentiles, cell edge, grid
-void dsm_grid(const float *xyz, int64_t n, double voxel_m, int64_t budget, DsmGrid &g) {
-    std::vector<int32_t> kx((size_t)n), ky((size_t)n);
-    for (int64_t p = 0; p < n; ++p) {
-        kx[(size_t)p] = total_key(xyz[3 * p]);
-        ky[(size_t)p] = total_key(xyz[3 * p + 1]);
-    }
-    const double xa = order_stat(kx, 0.005), xb = order_stat(kx, 0.995);
-    const double ya = order_stat(ky, 0.005), yb = order_stat(ky, 0.995);
+// dsm.rs:142-148: the grid from the percentile extent. Rust's `as usize`
+// makes a NaN edge count 0; an edge count past 2^31 is refused by the caller.
+void dsm_grid(const float ext[4], double voxel_m, int64_t budget, DsmGrid &g) {
+    const double xa = ext[0], xb = ext[1], ya = ext[2], yb = ext[3];
     const double area = (xb - xa) * (yb - ya);
     g.cell = std::max(voxel_m * 1.5, std::sqrt(area / (double)budget));
     g.x0 = xa;
     g.y0 = ya;
-    g.nx = (int64_t)std::ceil((xb - xa) / g.cell) + 1;
-    g.ny = (int64_t)std::ceil((yb - ya) / g.cell) + 1;
+    auto edge = [](double e) -> int64_t {
+        if (std::isnan(e)) return 1;
+        if (e > 2147483647.0) return INT64_MAX;
+        return (int64_t)e + 1;
+    };
+    g.nx = edge(std::ceil((xb - xa) / g.cell));
+    g.ny = edge(std::ceil((yb - ya) / g.cell));
 }
 
 }  // namespace
@@ -1908,12 +1888,24 @@ void dsm_grid(const float *xyz, int64_t n, double voxel_m, int64_t budget, DsmGr
 int mvs_build_dsm(mvs_ctx *c, const float *xyz, int64_t n, double voxel_m, int32_t profile,
                   mvs_dsm_info *info, float *h, int64_t cap) {
     if (!c) return MVS_ERR_ARG;
-    static const int64_t budget[3] = {3000000, 8000000, 20000000};  // dsm.rs:122-128
+    static const int64_t budget[3] = {3000000, 8000000, 20000000};  // dsm.rs:121-128
+    c->dsm_cells = 0;
     if (!info || profile < 0 || profile > 2 || !(voxel_m > 0.0) || n < 0 || (n > 0 && !xyz))
         return fail(c, MVS_ERR_ARG, "bad surface model arguments");
     if (n < 100) return fail(c, MVS_ERR_ARG, "the point cloud is too small to build a surface");
+    if (n >= (1ll << 31)) return fail(c, MVS_ERR_ARG, "surface model of 2^31 points or more");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, c->dsm_xyz.ensure((size_t)n * 12));
+    HIP_TRY(c, c->dsm_sel.ensure(kDsmSelBytes));
+    HIP_TRY(c, hipMemcpyAsync(c->dsm_xyz.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, c->st));
+    // dsm.rs:142: v[((len - 1) as f64 * f) as usize] of the sorted coordinates
+    const uint32_t rlo = (uint32_t)((double)(n - 1) * 0.005);
+    const uint32_t rhi = (uint32_t)((double)(n - 1) * 0.995);
+    float ext[4];
+    if (dsm_extent(c->dsm_xyz.as<float>(), n, rlo, rhi, c->dsm_sel.p, c->st, ext))
+        return hip_fail(c, hipGetLastError(), "surface model extent");
     DsmGrid g;
-    dsm_grid(xyz, n, voxel_m, budget[profile], g);
+    dsm_grid(ext, voxel_m, budget[profile], g);
     info->x0 = g.x0;
     info->y0 = g.y0;
     info->cell = g.cell;
@@ -1921,29 +1913,35 @@ int mvs_build_dsm(mvs_ctx *c, const float *xyz, int64_t n, double voxel_m, int32
     info->ny = g.ny;
     info->zmax = NAN;
     info->reserved = 0;
-    if (!h) return MVS_OK;
-    const int64_t nc = g.nx * g.ny;
-    if (!(g.nx > 0 && g.ny > 0) || nc >= (1ll << 31))
+    if (g.nx > (1ll << 31) || g.ny > (1ll << 31) || g.nx * g.ny >= (1ll << 31))
         return fail(c, MVS_ERR_ARG, "surface grid too large (2^31 cells)");
-    if (cap < nc) return fail(c, MVS_ERR_ARG, "output array too small");
-    HIP_TRY(c, hipSetDevice(c->device));
+    const int64_t nc = g.nx * g.ny;
     const size_t cb = (size_t)nc * 4;
-    HIP_TRY(c, c->dsm_xyz.ensure((size_t)n * 12));
     HIP_TRY(c, c->dsm_h.ensure(cb));
     DevBuf *bufs[] = {&c->dsm_a, &c->dsm_b, &c->dsm_c, &c->dsm_d, &c->dsm_snap, &c->dsm_parent,
                       &c->dsm_la, &c->dsm_lb};
     for (DevBuf *b : bufs) HIP_TRY(c, b->ensure(cb));
     HIP_TRY(c, c->dsm_ctr.ensure(16));
-    HIP_TRY(c, hipMemcpyAsync(c->dsm_xyz.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, c->st));
     DsmWork w{c->dsm_a.as<uint32_t>(),  c->dsm_b.as<uint32_t>(),  c->dsm_c.as<uint32_t>(),
               c->dsm_d.as<uint32_t>(),  c->dsm_snap.as<float>(),  c->dsm_parent.as<int32_t>(),
               c->dsm_la.as<int32_t>(),  c->dsm_lb.as<int32_t>(),  c->dsm_ctr.as<uint32_t>()};
     float zmax = 0.0f;
     if (dsm_build(g, c->dsm_xyz.as<float>(), n, c->dsm_h.as<float>(), w, c->st, &zmax))
         return hip_fail(c, hipGetLastError(), "surface model kernels");
-    HIP_TRY(c, hipMemcpyAsync(h, c->dsm_h.p, cb, hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(c, hipStreamSynchronize(c->st));
     info->zmax = zmax;
+    c->dsm_cells = nc;
+    if (!h) return MVS_OK;
+    return mvs_dsm_download(c, h, cap);
+}
+
+int mvs_dsm_download(mvs_ctx *c, float *h, int64_t cap) {
+    if (!c) return MVS_ERR_ARG;
+    if (!h || c->dsm_cells == 0) return fail(c, MVS_ERR_ARG, "no surface model built");
+    if (cap < c->dsm_cells) return fail(c, MVS_ERR_ARG, "output array too small");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(h, c->dsm_h.p, (size_t)c->dsm_cells * 4, hipMemcpyDeviceToHost,
+                              c->st));
+    HIP_TRY(c, hipStreamSynchronize(c->st));
     return MVS_OK;
 }
 

@@ -6,6 +6,7 @@
 // reference's sequential loops.
 #include "mvs_internal.h"
 
+#include <algorithm>
 #include <math.h>
 #include <string.h>
 
@@ -310,6 +311,107 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
     if (k < n) p[k] = v;
 }
 
+// ---- order statistics of the coordinates (dsm.rs:140-143) by radix select
+// over the total-order keys: three digit passes (11, 11 and 10 bits from the
+// top), each a histogram of the keys that match the digits found so far.
+// Targets: 0 x lo, 1 x hi, 2 y lo, 3 y hi. ----
+
+constexpr int kSelBins = 2048;
+
+struct SelState {
+    uint32_t prefix[4];  // the key's digits found so far
+    uint32_t rank[4];    // rank still to find among the keys with that prefix
+};
+
+__device__ __forceinline__ uint32_t ukey(float v) { return (uint32_t)tkey(v) ^ 0x80000000u; }
+
+__device__ __forceinline__ uint32_t sel_digit(uint32_t u, int pass) {
+    return pass == 0 ? u >> 21 : (pass == 1 ? (u >> 10) & 2047u : u & 1023u);
+}
+__device__ __forceinline__ uint32_t sel_mask(int pass) {
+    return pass == 0 ? 0u : (pass == 1 ? 0xFFE00000u : 0xFFFFFC00u);
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const float *__restrict__ xyz, int64_t n,
+                                                  int pass, const SelState *__restrict__ st,
+                                                  uint32_t *__restrict__ hist) {
+    __shared__ uint32_t lh[4 * kSelBins];
+    for (int b = threadIdx.x; b < 4 * kSelBins; b += 256) lh[b] = 0u;
+    __syncthreads();
+    const uint32_t mask = sel_mask(pass);
+    const uint32_t p0 = st->prefix[0], p1 = st->prefix[1], p2 = st->prefix[2],
+                   p3 = st->prefix[3];
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * 256) {
+        const uint32_t ux = ukey(xyz[3 * p]), uy = ukey(xyz[3 * p + 1]);
+        const uint32_t dx = sel_digit(ux, pass), dy = sel_digit(uy, pass);
+        if ((ux & mask) == p0) atomicAdd(&lh[dx], 1u);
+        if ((ux & mask) == p1) atomicAdd(&lh[kSelBins + dx], 1u);
+        if ((uy & mask) == p2) atomicAdd(&lh[2 * kSelBins + dy], 1u);
+        if ((uy & mask) == p3) atomicAdd(&lh[3 * kSelBins + dy], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 4 * kSelBins; b += 256)
+        if (lh[b]) atomicAdd(&hist[b], lh[b]);
+}
+
+// One block per target: the digit whose bin holds the rank, then the bins
+// cleared for the next pass. The rank is below the matching keys' count, so
+// the walk ends inside the histogram.
+__global__ __launch_bounds__(256) void k_sel_scan(int pass, SelState *st,
+                                                  uint32_t *__restrict__ hist) {
+    const int t = blockIdx.x;
+    uint32_t *h = hist + t * kSelBins;
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t found[2];
+    uint32_t v[8], s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[i] = h[threadIdx.x * 8 + i];
+        s += v[i];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = st->rank[t];
+        uint32_t c = 0;
+        while (c < 255 && r >= part[c]) {
+            r -= part[c];
+            ++c;
+        }
+        found[0] = c;
+        found[1] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == found[0]) {
+        // the first bin of this thread's eight that holds the rank
+        uint32_t r = found[1], d = 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (d == 8u) {
+                if (r < v[i] || i == 7)
+                    d = (uint32_t)i;
+                else
+                    r -= v[i];
+            }
+        }
+        const uint32_t digit = threadIdx.x * 8 + d;
+        const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
+        st->prefix[t] |= digit << shift;
+        st->rank[t] = r;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) h[threadIdx.x * 8 + i] = 0u;
+}
+
+__global__ void k_sel_init(SelState *st, uint32_t rank_lo, uint32_t rank_hi) {
+    const int t = threadIdx.x;
+    if (t < 4) {
+        st->prefix[t] = 0u;
+        st->rank[t] = (t & 1) ? rank_hi : rank_lo;
+    }
+}
+
 unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -325,6 +427,32 @@ float dsm_fval_host(uint32_t k) {
     float v;
     memcpy(&v, &b, 4);
     return v;
+}
+
+int dsm_extent(const float *d_xyz, int64_t n, uint32_t rank_lo, uint32_t rank_hi, void *d_sel,
+               hipStream_t st, float out[4]) {
+    uint32_t *hist = (uint32_t *)d_sel;
+    SelState *state = (SelState *)(hist + 4 * kSelBins);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(4 * kSelBins)), dim3(256), 0, st, hist,
+                       (int64_t)4 * kSelBins, 0u);
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(64), 0, st, state, rank_lo, rank_hi);
+    const unsigned blocks = (unsigned)std::min<int64_t>(nblk(n), 1024);
+    for (int pass = 0; pass < 3; ++pass) {
+        hipLaunchKernelGGL(k_sel_hist, dim3(blocks), dim3(256), 0, st, d_xyz, n, pass, state,
+                           hist);
+        hipLaunchKernelGGL(k_sel_scan, dim3(4), dim3(256), 0, st, pass, state, hist);
+    }
+    SelState h;
+    if (hipMemcpyAsync(&h, state, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -1;
+    for (int t = 0; t < 4; ++t) {
+        // invert ukey: back to the total-order key, then to the float's bits
+        const int32_t k = (int32_t)(h.prefix[t] ^ 0x80000000u);
+        const int32_t x = k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+        memcpy(&out[t], &x, 4);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWork &w,

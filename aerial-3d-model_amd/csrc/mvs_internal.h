@@ -191,6 +191,13 @@ struct DsmWork {
 // Heights of the grid from n device points (xyz f32) into d_h; 0 or -1 (HIP error).
 int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWork &w,
               hipStream_t st, float *zmax);
+// Bytes of device memory dsm_extent works in.
+constexpr size_t kDsmSelBytes = 4 * 2048 * 4 + 64;
+// The order statistics rank_lo / rank_hi (0-based, in f32::total_cmp order)
+// of the x and of the y coordinates of n device points (n < 2^31): out =
+// {x lo, x hi, y lo, y hi}. 0 or -1 (HIP error). Synchronises the stream.
+int dsm_extent(const float *d_xyz, int64_t n, uint32_t rank_lo, uint32_t rank_hi, void *d_sel,
+               hipStream_t st, float out[4]);
 
 // Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
 int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,

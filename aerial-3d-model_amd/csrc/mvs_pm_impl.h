@@ -143,7 +143,6 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
             const ff FF{S.f, S.f_lo}, FCX{S.cx, S.cx_lo}, FCY{S.cy, S.cy_lo},
                 FK1{S.k1, S.k1_lo}, FK2{S.k2, S.k2_lo};
             const float K1 = S.k1, K2 = S.k2;
-            const float *__restrict__ G = S.gray;
             const uint32_t *__restrict__ Q = S.quad;  // fast kernels only
             const float4 *__restrict__ QF = S.quadf;
             const int32_t sw = S.w;

@@ -135,6 +135,38 @@ def view_prep_stage(ctx, spec, photos=8):
                  **out), photo, outputs)
 
 
+def dsm_stage(ctx, xyz, voxel, reps=5):
+    """build_dsm (recon/dsm.rs:128-290) on the GPU from the fused cloud, at the
+    Standard profile's cell budget: upload, extent, cell binning, filters,
+    download. Returns the figures and the GPU's grid (for the CPU leg's check)."""
+    ctx.build_dsm(xyz, voxel, 1)  # warm (buffers)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        info, h = ctx.build_dsm(xyz, voxel, 1)
+    dt = (time.perf_counter() - t) / reps
+    out = dict(points=int(len(xyz)), grid=f"{info['nx']}x{info['ny']}", cell_m=info["cell"],
+               filled=float(np.isfinite(h).mean()), gpu_ms=dt * 1e3,
+               gpu_mpoints_s=len(xyz) / dt / 1e6,
+               path="mvs_build_dsm: upload, extent, binning, despike, islands, holes, "
+                    "median, download")
+    log(f"dsm: {out}")
+    return out, (info, h)
+
+
+def cpu_baseline_dsm(xyz, voxel, got):
+    """CPU-baseline leg of the surface-model stage: the oracle on the same
+    cloud (one thread), and the GPU's grid checked identical to it."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    tc = time.perf_counter()
+    info, h = orc.build_dsm(xyz, voxel, orc.DSM_BUDGET[1])
+    dc = time.perf_counter() - tc
+    ginfo, gh = got
+    return dict(cpu_ms=dc * 1e3, cpu_mpoints_s=len(xyz) / dc / 1e6, cores=1, kind="port",
+                identical=bool(info == ginfo and np.array_equal(h, gh, equal_nan=True)))
+
+
 def cpu_baseline_view_prep(photo, outputs):
     """CPU-baseline leg of the view-preparation stage: the oracle on the same
     photo (one thread), and the GPU's outputs checked identical to it."""
@@ -222,6 +254,7 @@ def main():
                     help="arithmetic of the batched path's PatchMatch kernels")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--no-dsm", action="store_true")
     ap.add_argument("--no-prep", action="store_true",
                     help="skip the view-preparation stage (RGB photo -> .view contents)")
     ap.add_argument("--cpu-crop", default="1280x960")
@@ -404,6 +437,11 @@ def main():
                       ply_bytes=int(len(xyz)) * 15)
         log(f"fusion: {len(xyz)} voxels of {voxel:.4f} m in {dtf:.2f} s")
 
+    # the surface model from the fused cloud (dsm.rs:128-290)
+    dsm = dsm_got = None
+    if fusion is not None and not args.no_dsm and len(xyz) >= 100:
+        dsm, dsm_got = dsm_stage(ctx, xyz, voxel)
+
     prep = prep_photo = prep_out = None
     if not args.no_prep:
         prep, prep_photo, prep_out = view_prep_stage(ctx, spec)
@@ -417,6 +455,9 @@ def main():
         if prep_out:
             cpu["view_prep"] = cpu_baseline_view_prep(prep_photo, prep_out)
             log(f"CPU view prep: {cpu['view_prep']}")
+        if dsm_got:
+            cpu["dsm"] = cpu_baseline_dsm(xyz, voxel, dsm_got)
+            log(f"CPU dsm: {cpu['dsm']}")
 
     line = dict(
         metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,
@@ -442,7 +483,7 @@ def main():
         refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
         graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
-        fusion=fusion, view_prep=prep,
+        fusion=fusion, view_prep=prep, dsm=dsm,
     )
     print(json.dumps(line), flush=True)
     sv.close()

@@ -324,14 +324,21 @@ typedef struct {
     int32_t reserved;
 } mvs_dsm_info;
 /* build_dsm (dsm.rs:128-290) on the GPU from a cloud of n points (xyz f32, as
- * read from dense.ply), the fusion's voxel edge and the profile (0 Lite, 1
- * Standard, 2 High: a budget of 3, 8, 20 M cells, dsm.rs:122-128). With
- * h == NULL only the grid is computed (info; nx * ny floats to allocate); else
- * h (cap floats) receives the heights, row-major from the south, NaN = no
- * surface. MVS_ERR_ARG with "the point cloud is too small to build a surface"
- * below 100 points. */
+ * read from dense.ply; 100 <= n < 2^31), the fusion's voxel edge and the
+ * profile (0 Lite, 1 Standard, 2 High: a budget of 3, 8, 20 M cells,
+ * dsm.rs:121-128). Replaces the reference's build_dsm(dense_ply, voxel_m,
+ * profile), which reads the cloud itself. The grid is built on the device and
+ * kept there (until the next build on this context); info receives its
+ * extent and zmax. With h != NULL the heights are also copied into h (cap
+ * floats, at least nx * ny), row-major from the south, NaN = no surface;
+ * with h == NULL, mvs_dsm_download copies them after the caller has sized
+ * its array from info. MVS_ERR_ARG with "the point cloud is too small to
+ * build a surface" below 100 points (dsm.rs:137-139). */
 int mvs_build_dsm(mvs_ctx *ctx, const float *xyz, int64_t n, double voxel_m, int32_t profile,
                   mvs_dsm_info *info, float *h, int64_t cap);
+/* The heights of the last grid mvs_build_dsm built on this context into h
+ * (cap floats, at least nx * ny). Part of the same replacement. */
+int mvs_dsm_download(mvs_ctx *ctx, float *h, int64_t cap);
 /* Dsm::save / Dsm::load (dsm.rs:27-75): the `dsm.bin` layout ("DSM1", x0, y0,
  * cell f64, nx, ny u64, zmax f32, heights f32, little-endian; tmp + rename).
  * load: MVS_ERR_FORMAT for a damaged or truncated file; h (cap floats)

@@ -1104,8 +1104,10 @@ int orc_build_dsm(const float *xyz, int64_t n, double voxel_m, int64_t budget, o
     double cell = voxel_m * 1.5;
     const double c2 = sqrt(area / (double)budget);
     if (c2 > cell) cell = c2;
-    const int64_t nx = (int64_t)ceil((xb - xa) / cell) + 1;
-    const int64_t ny = (int64_t)ceil((yb - ya) / cell) + 1;
+    // Rust's `as usize` makes a NaN edge count (a NaN percentile) 0
+    const double ex = ceil((xb - xa) / cell), ey = ceil((yb - ya) / cell);
+    const int64_t nx = (isnan(ex) ? 0 : (int64_t)ex) + 1;
+    const int64_t ny = (isnan(ey) ? 0 : (int64_t)ey) + 1;
     const int64_t nc = nx * ny;
     float *top = (float *)malloc((size_t)nc * 3 * sizeof(float));
     uint16_t *count = (uint16_t *)calloc((size_t)nc, sizeof(uint16_t));

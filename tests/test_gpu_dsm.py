@@ -103,6 +103,24 @@ def test_cell_edge_from_budget(gpu_ctx):
     assert info["cell"] > 0.45 and info["nx"] * info["ny"] <= 3_100_000
 
 
+def test_extent_edge_cases(gpu_ctx):
+    """The percentile extent (dsm.rs:140-148) over keys of both signs, heavy
+    duplicates, a degenerate (one-column) extent and a NaN upper percentile."""
+    rng = np.random.default_rng(17)
+    n = 30_000
+    p = np.c_[rng.uniform(-40, 40, n), rng.uniform(-1e-3, 1e-3, n) - 512.0, rng.normal(0, 1, n)]
+    p[: n // 3, 0] = -0.0  # a third of the keys equal, around zero
+    p[n // 3: n // 3 + 50, 0] = 0.0
+    _check(gpu_ctx, "signs and duplicates", p, 0.3, 0)
+    q = p.copy()
+    q[:, 0] = 7.25  # every x equal: one column
+    _check(gpu_ctx, "one column", q, 0.3, 0)
+    r = p.copy()
+    r[: n // 50, 0] = np.nan  # 2% NaN x: the 99.5th percentile is NaN, nx = 1
+    info, _ = _check(gpu_ctx, "NaN percentile", r, 0.3, 0)
+    assert info["nx"] == 1
+
+
 def test_survey_ground_truth_cloud(gpu_ctx, tiny_survey):
     pts = []
     for v in tiny_survey.views:

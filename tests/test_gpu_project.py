@@ -10,7 +10,8 @@ directly on the same views (the `.view` grey of R = G = B equals the grey
 table, so the maps must be bit-identical); dense.ply holds exactly the cloud
 of a direct fusion with the same voxel and the origin comment; dense.voxel
 is the voxel edge in Rust's float format; the `.view` files are gone; a
-second run reuses the `.depth` cache and writes the same cloud.
+second run reuses the `.depth` cache and writes the same cloud; the surface
+model built from dense.ply and dense.voxel (dsm.bin) equals the oracle's.
 """
 import os
 import sys
@@ -20,6 +21,8 @@ import pytest
 
 from aerial_mvs import capi
 from aerial_mvs import project as P
+
+import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -113,6 +116,15 @@ def test_run_dense_on_a_project(gpu_ctx, tiny_survey, tmp_path):
         assert os.path.getmtime(os.path.join(cdir, f)) == t
     pxyz2, _, _ = _read_ply_with_comment(os.path.join(odir, "dense.ply"))
     assert np.array_equal(pxyz2, pxyz)
+
+    # the surface model from dense.ply at the recorded voxel (recon/mod.rs:91-98)
+    _say("run_dsm on the project")
+    info, h = P.run_dsm(root, gpu_ctx, kind=0)
+    winfo, wh = orc.build_dsm(pxyz, float(P.rust_f64(voxel)), orc.DSM_BUDGET[0])
+    assert info == winfo and np.array_equal(h, wh, equal_nan=True)
+    linfo, lh = capi.load_dsm(os.path.join(odir, "dsm.bin"))
+    assert linfo == info and lh.tobytes() == h.tobytes()
+    assert np.isfinite(h).mean() > 0.5
 
 
 def _read_ply_with_comment(path):

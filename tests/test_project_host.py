@@ -92,3 +92,16 @@ def test_origin_comment_and_paths():
     assert P.depth_path("/p/cache/dense", "IMG_1", 960) == "/p/cache/dense/IMG_1_960.depth"
     assert P.cache_dir("/p") == os.path.join("/p", "cache", "dense")
     assert P.out_dir("/p") == os.path.join("/p", "result", "3d")
+
+
+def test_dense_voxel_file(tmp_path):
+    """recon/mod.rs:102-109: trimmed text parsed as f64, None otherwise."""
+    root = str(tmp_path)
+    assert P.dense_voxel(root) is None
+    os.makedirs(P.out_dir(root))
+    path = os.path.join(P.out_dir(root), "dense.voxel")
+    for text, want in (("0.1875", 0.1875), (" 0.3\n", 0.3), ("1e-1", 0.1), ("abc", None),
+                       ("1_0", None), ("", None)):
+        with open(path, "w") as f:
+            f.write(text)
+        assert P.dense_voxel(root) == want, text
