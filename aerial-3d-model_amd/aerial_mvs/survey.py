@@ -122,14 +122,18 @@ class SurveySpec:
 
 # The reference-window shape: 5 taps at stride 2 (dense.rs:170).
 SPECS = {
-    # 16 nadir views 1024x768, 4 neighbours, 7x7, 4 iterations
+    # c1: 16 nadir views 1024x768, 4 neighbours, 7x7, 4 iterations
     "c1": SurveySpec("c1", 1024, 768, 900.0, 60.0, 4, 4, neighbours=4, iterations=4,
                      seed=0xA3D00001),
-    # 77 nadir views 2000x1500, 8 neighbours, 7x7, 6 iterations;
-    # focal scaled with the width to keep configs[0]'s field of view; 60 m
+    # c2: 77 nadir views 2000x1500, 8 neighbours, 7x7, 6 iterations;
+    # focal scaled with the width to keep c1's field of view; 60 m
     # altitude, depth range [40, 80], the 7 x 11 block over the scene centre.
     "c2": SurveySpec("c2", 2000, 1500, 900.0 * 2000 / 1024, 60.0, 7, 11, patch=(500.0, 350.0),
                      neighbours=8, iterations=6, seed=0xA3D00002),
+    # c4: c2's survey shape (its own scene seed) with an 11x11 window
+    # (N = 121), refined by Gauss-Newton steps
+    "c4": SurveySpec("c4", 2000, 1500, 900.0 * 2000 / 1024, 60.0, 7, 11, patch=(500.0, 350.0),
+                     neighbours=8, taps=11, iterations=6, seed=0xA3D00004),
     # small surveys for tests: the c1 camera geometry with a cropped sensor
     "tiny": SurveySpec("tiny", 160, 120, 900.0, 60.0, 3, 3, patch=(60.0, 50.0), neighbours=4,
                        taps=5, stride=2, iterations=3, seed=0xA3D0000A),

@@ -167,6 +167,93 @@ def cpu_baseline_dsm(xyz, voxel, got):
                 identical=bool(info == ginfo and np.array_equal(h, gh, equal_nan=True)))
 
 
+def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
+    """CPU-baseline leg: the backward kernel at full size. At n_items random
+    valid pixels of the refined maps (spread over the views), the refined
+    plane perturbed by up to 1% in depth and ~0.05 rad in normal (at the
+    refined plane itself the gradient is near zero: it is a Gauss-Newton
+    minimum, where a relative error says nothing), the GPU's fixed-pair cost
+    and analytic gradient w.r.t. (depth, normal) at the pair it selects,
+    against the oracle's f64 pair cost and f64 central finite differences
+    (1e-4 m in depth, 1e-5 per normal component). Items whose bilinear cells
+    change within a difference step are skipped (the cost is not
+    differentiable there)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    from aerial_mvs import capi
+    rng = np.random.default_rng(seed)
+    prm = capi.params(taps=spec.taps, stride=spec.stride)
+    per = max(1, n_items // len(jobs))
+    hs = (1e-4, 1e-5, 1e-5, 1e-5)
+    rels, dcost = [], []
+    tried = 0
+    tc = time.perf_counter()
+    for (i, nb, rg) in jobs:
+        v = surv.views[i]
+        srcs = [surv.views[j] for j in nb]
+        d, n, _ = sv.download(i)
+        m = 12
+        yy, xx = np.nonzero(d[m:-m, m:-m] > 0)
+        if len(xx) == 0:
+            continue
+        pick = rng.choice(len(xx), min(per, len(xx)), replace=False)
+        xs = (xx[pick] + m).astype(np.int32)
+        ys = (yy[pick] + m).astype(np.int32)
+        nrm = n[ys, xs] + rng.normal(0, 0.05, (len(xs), 3))
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        dep = d[ys, xs] * rng.uniform(0.99, 1.01, len(xs))
+        hy = np.concatenate([dep[:, None], nrm], axis=1).astype(np.float32)
+        cost, pq, grad = ctx.ncc_pair(v, srcs, rg[0], rg[1], prm, xs, ys, hy)
+        o = orc.OracleDepth(v, srcs, rg[0], rg[1], taps=spec.taps, stride=spec.stride)
+        for k in range(len(xs)):
+            if cost[k] >= 2.0:
+                continue
+            tried += 1
+            h64 = hy[k].astype(np.float64)
+            s0, s1 = int(pq[k][0]), int(pq[k][1])
+            base = o.pair_cost64(xs[k], ys[k], h64, s0, s1)
+            if base is None:
+                continue
+            dcost.append(abs(float(cost[k]) - base[0]))
+            fd = np.zeros(4)
+            ok = True
+            for a in range(4):
+                hp, hm = h64.copy(), h64.copy()
+                hp[a] += hs[a]
+                hm[a] -= hs[a]
+                p_ = o.pair_cost64(xs[k], ys[k], hp, s0, s1)
+                m_ = o.pair_cost64(xs[k], ys[k], hm, s0, s1)
+                if p_ is None or m_ is None or p_[2] != base[2] or m_[2] != base[2]:
+                    ok = False
+                    break
+                fd[a] = (p_[0] - m_[0]) / (2 * hs[a])
+            if not ok:
+                continue
+            # error relative to the gradient's size, normal components scaled
+            # by 1/depth so both parts weigh alike
+            scale = np.array([1.0, 1 / h64[0], 1 / h64[0], 1 / h64[0]])
+            e = np.abs(grad[k].astype(np.float64) - fd) / scale
+            r = np.abs(fd) / scale
+            rels.append(e.max() / max(r.max(), 1e-12))
+        del o
+    rels = np.array(rels)
+    dcost = np.array(dcost)
+    out = dict(items=int(tried), checked=int(len(rels)), seconds=time.perf_counter() - tc,
+               hypotheses="refined plane, depth x U(0.99, 1.01), normal + N(0, 0.05)",
+               fd_steps="1e-4 m depth, 1e-5 per normal component",
+               bars="gradient median < 2e-4, p99 < 3e-3; cost p99 < 3e-5")
+    if len(rels) and len(dcost):
+        out.update(grad_rel_err_median=float(np.median(rels)),
+                   grad_rel_err_p99=float(np.quantile(rels, 0.99)),
+                   grad_rel_err_max=float(rels.max()),
+                   cost_abs_diff_median=float(np.median(dcost)),
+                   cost_abs_diff_p99=float(np.quantile(dcost, 0.99)),
+                   cost_abs_diff_max=float(dcost.max()))
+        out["pass"] = bool(out["grad_rel_err_median"] < 2e-4 and out["grad_rel_err_p99"] < 3e-3
+                           and out["cost_abs_diff_p99"] < 3e-5)
+    return out
+
+
 def cpu_baseline_view_prep(photo, outputs):
     """CPU-baseline leg of the view-preparation stage: the oracle on the same
     photo (one thread), and the GPU's outputs checked identical to it."""
@@ -255,6 +342,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--no-dsm", action="store_true")
+    ap.add_argument("--grad-items", type=int, default=-1,
+                    help="pixels of the full-size gradient check in the CPU leg "
+                         "(default 10000 for c4, 2000 otherwise; 0 skips it)")
     ap.add_argument("--no-prep", action="store_true",
                     help="skip the view-preparation stage (RGB photo -> .view contents)")
     ap.add_argument("--cpu-crop", default="1280x960")
@@ -458,6 +548,12 @@ def main():
         if dsm_got:
             cpu["dsm"] = cpu_baseline_dsm(xyz, voxel, dsm_got)
             log(f"CPU dsm: {cpu['dsm']}")
+        n_grad = args.grad_items if args.grad_items >= 0 else (10_000 if args.config == "c4"
+                                                               else 2_000)
+        if n_grad and args.refine_steps > 0:
+            log(f"gradient check on {n_grad} pixels ...")
+            cpu["gradient_check"] = gradient_check(ctx, surv, timed, spec, sv, n_grad)
+            log(f"gradient check: {cpu['gradient_check']}")
 
     line = dict(
         metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,

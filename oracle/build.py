@@ -5,7 +5,7 @@ and no fast-math, so its f32/f64 arithmetic rounds like the Rust reference.
 
 The reference itself (crates/stitch-core, Rust) cannot be built in this image:
 there is no Rust toolchain and its crates (nalgebra, rayon, image, serde) are
-not vendored. ``oracle/_ref/`` therefore stays empty; see DESIGN.md.
+not vendored. ``oracle/_ref/`` therefore stays empty.
 """
 import os
 import subprocess
