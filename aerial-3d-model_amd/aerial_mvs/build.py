@@ -2,6 +2,8 @@
 
 - ``libaerial_mvs.so``: the HIP engine behind ``include/aerial_mvs.h``.
 - ``libsynth.so``: the seeded synthetic-survey generator (test/benchmark workload).
+- ``libsynth_gpu.so``: the same generator on the GPU, for surveys too large to
+  render on the host (test/benchmark workload; not part of the engine).
 
 The oracle under ``oracle/`` is built by ``oracle/build.py``; this module does not
 touch it.
@@ -18,6 +20,7 @@ REPO = os.path.dirname(SRC_ROOT)
 
 LIB_MVS = os.path.join(PKG_DIR, "libaerial_mvs.so")
 LIB_SYNTH = os.path.join(PKG_DIR, "libsynth.so")
+LIB_SYNTH_GPU = os.path.join(PKG_DIR, "libsynth_gpu.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -80,14 +83,23 @@ def build_synth():
     return LIB_SYNTH
 
 
+def build_synth_gpu():
+    tmp = LIB_SYNTH_GPU + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "synth_gpu.hip"), "-o", tmp])
+    os.replace(tmp, LIB_SYNTH_GPU)
+    return LIB_SYNTH_GPU
+
+
 def build_all():
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found; the HIP engine cannot be built")
     build_synth()
+    build_synth_gpu()
     build_mvs()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "mvs": build_mvs, "synth": build_synth}[w]()
+        {"all": build_all, "mvs": build_mvs, "synth": build_synth,
+         "synth_gpu": build_synth_gpu}[w]()

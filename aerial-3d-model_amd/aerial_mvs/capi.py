@@ -27,6 +27,8 @@ MVS_CONS_REFERENCE = 0
 MVS_CONS_REPROJ = 1
 MVS_MODE_EXACT = 0
 MVS_MODE_FAST = 1
+MVS_SURVEY_LEAN_VIEWS = 1
+MVS_SURVEY_DEPTH_ONLY = 2
 
 EXPORTS = [
     "mvs_default_params", "mvs_dense_params_for_profile", "mvs_select_neighbours",
@@ -39,7 +41,7 @@ EXPORTS = [
     "mvs_survey_depth_maps_ex", "mvs_save_view", "mvs_load_view", "mvs_view_intrinsics",
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
-    "mvs_survey_set_depth", "mvs_build_dsm", "mvs_dsm_download", "mvs_save_dsm",
+    "mvs_survey_set_depth", "mvs_build_dsm", "mvs_dsm_download", "mvs_save_dsm", "mvs_survey_set_options",
     "mvs_load_dsm", "mvs_dsm_height",
 ]
 
@@ -137,6 +139,7 @@ def lib():
         L.mvs_survey_set_depth.argtypes = [vp, i32, vp]
         L.mvs_build_dsm.argtypes = [vp, vp, i64, f64, i32, P(MvsDsmInfo), vp, i64]
         L.mvs_dsm_download.argtypes = [vp, vp, i64]
+        L.mvs_survey_set_options.argtypes = [vp, i32]
         L.mvs_save_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp]
         L.mvs_load_dsm.argtypes = [C.c_char_p, P(MvsDsmInfo), vp, i64]
         L.mvs_dsm_height.argtypes = [P(MvsDsmInfo), vp, f64, f64, P(C.c_float)]
@@ -504,10 +507,15 @@ class Context:
 class Survey:
     """Views resident in HBM; depth maps stay on the device."""
 
-    def __init__(self, ctx, views):
+    def __init__(self, ctx, views, options=0):
+        """options: MVS_SURVEY_LEAN_VIEWS | MVS_SURVEY_DEPTH_ONLY (residency
+        for large surveys)."""
         self.ctx = ctx
         self._h = C.c_void_p()
         ctx.check(lib().mvs_survey_create(ctx.handle, len(views), C.byref(self._h)))
+        self.options = options
+        if options:
+            ctx.check(lib().mvs_survey_set_options(self._h, options))
         self.shapes = [(v.h, v.w) for v in views]
         for i, v in enumerate(views):
             keep = []
@@ -554,8 +562,13 @@ class Survey:
                                                        None))
 
     def download(self, view):
+        """(depth, normal, cost); normal and cost are None on a depth-only
+        survey."""
         h, w = self.shapes[view]
         d = np.zeros((h, w), np.float32)
+        if self.options & MVS_SURVEY_DEPTH_ONLY:
+            self.ctx.check(lib().mvs_survey_download(self._h, view, d.ctypes.data, None, None))
+            return d, None, None
         n = np.zeros((h, w, 3), np.float32)
         c = np.zeros((h, w), np.float32)
         self.ctx.check(lib().mvs_survey_download(self._h, view, d.ctypes.data, n.ctypes.data,

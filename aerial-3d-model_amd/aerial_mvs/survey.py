@@ -332,3 +332,192 @@ def make_uneven_survey(spec, nthreads=None):
                         range=tuple(r * spec.scale for r in rng_v)))
     surv = Survey(SPECS["small"], views, nb, [p["range"] for p in per], ground)
     return surv, per
+
+
+# ---- large surveys rendered on the GPU (libsynth_gpu.so) ----
+
+_glib = None
+
+
+def _synth_gpu():
+    global _glib
+    if _glib is None:
+        if not os.path.exists(_build.LIB_SYNTH_GPU):
+            raise RuntimeError("libsynth_gpu.so not built")
+        L = C.CDLL(_build.LIB_SYNTH_GPU)
+        L.sg_scene_create.argtypes = [C.c_double] * 6 + [C.c_int32, C.c_double, C.c_double,
+                                                         C.c_uint64, C.c_double, C.c_int32,
+                                                         C.POINTER(C.c_void_p)]
+        L.sg_scene_add_boxes.argtypes = [C.c_void_p, _f64p, C.c_int32]
+        L.sg_scene_info.argtypes = [C.c_void_p, _f64p]
+        L.sg_scene_info.restype = None
+        L.sg_render.argtypes = [C.c_void_p, C.c_int32, C.c_int32] + [C.c_double] * 3 + [
+            _f64p, _f64p, C.c_uint64, C.c_double, _u8p, C.c_void_p]
+        L.sg_scene_destroy.argtypes = [C.c_void_p]
+        L.sg_scene_destroy.restype = None
+        _glib = L
+    return _glib
+
+
+class GpuScene:
+    """The generator's scene on the GPU (same arguments as synth_scene_create)."""
+
+    def __init__(self, x0, y0, width, height, step, base_lambda, octaves, persistence, relief,
+                 seed, tex_lambda0, tex_octaves):
+        L = _synth_gpu()
+        h = C.c_void_p()
+        rc = L.sg_scene_create(x0, y0, width, height, step, base_lambda, octaves, persistence,
+                               relief, seed, tex_lambda0, tex_octaves, C.byref(h))
+        if rc:
+            raise RuntimeError(f"sg_scene_create failed ({rc})")
+        self._h = h
+
+    def add_boxes(self, boxes):
+        b = np.ascontiguousarray(boxes, np.float64).reshape(-1, 5)
+        if _synth_gpu().sg_scene_add_boxes(self._h, b.ravel(), len(b)):
+            raise RuntimeError("sg_scene_add_boxes failed")
+
+    def info(self):
+        out = np.zeros(6)
+        _synth_gpu().sg_scene_info(self._h, out)
+        return out
+
+    def render(self, w, h, f, cx, cy, R, Cc, noise_seed, noise_dn, with_depth):
+        img = np.zeros((h, w), np.uint8)
+        dep = np.zeros((h, w), np.float32) if with_depth else None
+        rc = _synth_gpu().sg_render(self._h, w, h, f, cx, cy,
+                                    np.ascontiguousarray(np.asarray(R, np.float64).ravel()),
+                                    np.ascontiguousarray(Cc, np.float64), noise_seed, noise_dn,
+                                    img.ravel(), None if dep is None else dep.ctypes.data)
+        if rc:
+            raise RuntimeError(f"sg_render failed ({rc})")
+        return img, dep
+
+    def close(self):
+        if self._h:
+            _synth_gpu().sg_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class LargeSpec:
+    """A large survey: nadir views on a lawnmower block and oblique stations
+    of four 45-degree views (north, east, south, west) over a fractal
+    heightfield with building-like boxes; rendered on the GPU.
+
+    c5: 2000 views 2400x1800 (1400 nadir in 28 lines of 50, 150 oblique
+    stations in 10 x 15), 2 km x 2 km block. The altitude, 183 m, is the one
+    at which 1400 nadir views of the c1 field of view cover the block with
+    ~75% forward and ~65% side overlap; the oblique stations fly at the same
+    height. Depth ranges: nadir [100, 230] m, oblique [100, 520] m. Boxes
+    stand on the highest node under them, so overlapping ones stack."""
+    name: str
+    size: tuple = (2400, 1800)
+    nadir_lines: int = 28
+    nadir_per_line: int = 50
+    oblique_grid: tuple = (10, 15)
+    block: tuple = (2000.0, 2000.0)
+    altitude: float = 183.0
+    margin: float = 640.0  # the far corner rays of an oblique view land ~575 m out
+    grid: float = 0.1
+    base_lambda: float = 400.0
+    octaves: int = 8
+    relief: float = 30.0
+    boxes: int = 1200
+    box_half: tuple = (4.0, 15.0)
+    box_height: tuple = (3.0, 30.0)
+    tex_lambda0: float = 0.2
+    tex_octaves: int = 5
+    noise_dn: float = 1.0
+    neighbours: int = 10
+    taps: int = 7
+    stride: int = 1
+    iterations: int = 6
+    nadir_range: tuple = (100.0, 230.0)
+    oblique_range: tuple = (100.0, 520.0)
+    depth_every: int = 25  # ground-truth depth for every n-th view
+    seed: int = 0xA3D00005
+
+
+LARGE = {
+    "c5": LargeSpec("c5"),
+    # the same layout scaled down, for tests
+    "large_small": LargeSpec("large_small", size=(320, 240), nadir_lines=4, nadir_per_line=6,
+                             oblique_grid=(2, 2), block=(64.0, 72.0), altitude=60.0,
+                             margin=200.0, base_lambda=60.0, octaves=6, relief=10.0, boxes=60,
+                             box_half=(2.0, 6.0), box_height=(3.0, 12.0), neighbours=6,
+                             iterations=4, nadir_range=(30.0, 80.0),
+                             oblique_range=(30.0, 220.0), depth_every=1, seed=0xA3D0000D),
+}
+
+
+def make_large_survey(spec, progress=None):
+    """Render a large survey on the GPU. Returns (Survey, per-view dict(kind,
+    range)); depth_gt (f32) only on every spec.depth_every-th view."""
+    if isinstance(spec, str):
+        spec = LARGE[spec]
+    rng = np.random.default_rng(spec.seed + 1)
+    bx, by = spec.block
+    m = spec.margin
+    scene = GpuScene(-m, -m, bx + 2 * m, by + 2 * m, spec.grid, spec.base_lambda, spec.octaves,
+                     0.5, spec.relief, spec.seed, spec.tex_lambda0, spec.tex_octaves)
+    try:
+        ground = 0.5 * spec.relief  # before the boxes: the terrain spans [0, relief]
+        half = rng.uniform(spec.box_half[0], spec.box_half[1], (spec.boxes, 2))
+        boxes = np.c_[rng.uniform(-0.25 * m, bx + 0.25 * m, spec.boxes),
+                      rng.uniform(-0.25 * m, by + 0.25 * m, spec.boxes), half,
+                      rng.uniform(spec.box_height[0], spec.box_height[1], spec.boxes)]
+        scene.add_boxes(boxes)
+        w, h = spec.size
+        f = 900.0 * w / 1024  # the c1 field of view
+        alt = ground + spec.altitude
+        cams = []
+        along = bx / (spec.nadir_per_line - 1)
+        across = by / (spec.nadir_lines - 1)
+        for r in range(spec.nadir_lines):
+            yaw = 90.0 if r % 2 == 0 else 270.0
+            cols = range(spec.nadir_per_line) if r % 2 == 0 else range(spec.nadir_per_line - 1,
+                                                                      -1, -1)
+            for cidx in cols:
+                dp = rng.normal(0.0, 0.05, 3)
+                da = rng.normal(0.0, 0.05, 3)
+                cams.append(("nadir", (cidx * along + dp[0], r * across + dp[1], alt + dp[2]),
+                             yaw + da[0], -90.0 + da[1], da[2]))
+        gx, gy = spec.oblique_grid
+        for r in range(gy):
+            for cidx in range(gx):
+                x = (cidx + 0.5) * bx / gx
+                y = (r + 0.5) * by / gy
+                for yaw in (0.0, 90.0, 180.0, 270.0):
+                    cams.append(("oblique", (x, y, alt), yaw, -45.0, 0.0))
+        views, per = [], []
+        for vi, (kind, c, yaw, pitch, roll) in enumerate(cams):
+            R, Cc = from_angles(c, yaw, pitch, roll)
+            want_depth = vi % spec.depth_every == 0
+            img, dep = scene.render(w, h, f, w / 2.0, h / 2.0, R, Cc, spec.seed + 1000 + vi,
+                                    spec.noise_dn, want_depth)
+            views.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
+            per.append(dict(kind=kind, yaw=yaw,
+                            range=spec.nadir_range if kind == "nadir" else spec.oblique_range))
+            if progress and (vi + 1) % 200 == 0:
+                progress(vi + 1, len(cams))
+    finally:
+        scene.close()
+    # neighbours: nearest by position among views of the same kind and heading
+    cent = np.array([v.C for v in views])
+    nb = []
+    for i in range(len(views)):
+        same = np.array([j for j in range(len(views)) if j != i and per[j]["kind"] ==
+                         per[i]["kind"] and (per[i]["kind"] == "nadir" or
+                                             per[j]["yaw"] == per[i]["yaw"])], np.int64)
+        d = np.hypot(cent[same, 0] - cent[i, 0], cent[same, 1] - cent[i, 1])
+        nb.append([int(j) for j in same[np.argsort(d, kind="stable")[:spec.neighbours + 2]]])
+    surv = Survey(SPECS["small"], views, nb, [p["range"] for p in per], ground)
+    surv.extra["large_spec"] = spec
+    return surv, per

@@ -67,6 +67,7 @@ struct ViewDev {
 struct MapDev {
     int32_t w = 0, h = 0;
     DevBuf depth, normal, cost;
+    bool keep_normal = true, keep_cost = true;  // false: the buffer is not kept
     bool valid = false;
 };
 
@@ -82,6 +83,7 @@ struct JobSpec {
     double range0 = 0, range1 = 0;
     mvs_params prm{};       // window, iterations, thresholds, refinement, mode
     MapDev *out = nullptr;  // thresholded outputs (nullable)
+    bool depth_only = false;  // the output keeps the depth only
 };
 
 typedef std::tuple<double, double, double, double, double, int32_t, int32_t> RayKey;
@@ -137,6 +139,7 @@ struct mvs_ctx {
 
 struct mvs_survey {
     mvs_ctx *ctx = nullptr;
+    int32_t options = 0;  // MVS_SURVEY_* flags
     std::vector<ViewDev> views;
     std::vector<MapDev> maps;
     // voxel table of the last fusion
@@ -249,7 +252,8 @@ int check_params(mvs_ctx *c, const mvs_params *p) {
 }
 
 // Upload a host view into a ViewDev (grey converted to f32 on the device).
-int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v) {
+// lean: an 8-bit view gets no f32 corner plane.
+int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v, bool lean = false) {
     TRY(check_view(c, v, "view"));
     const size_t n = (size_t)v->w * v->h;
     HIP_TRY(c, dv.gray.ensure(n * sizeof(float)));
@@ -276,9 +280,13 @@ int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v) {
                                     (size_t)v->w * sizeof(float), v->h, hipMemcpyHostToDevice,
                                     c->st));
     }
-    HIP_TRY(c, dv.quadf.ensure(n * sizeof(float4)));
-    launch_quadf(dv.gray.as<float>(), v->w, v->h, dv.quadf.as<float4>(), c->st);
-    HIP_TRY(c, hipGetLastError());
+    if (lean && dv.has_quad) {
+        dv.quadf.release();
+    } else {
+        HIP_TRY(c, dv.quadf.ensure(n * sizeof(float4)));
+        launch_quadf(dv.gray.as<float>(), v->w, v->h, dv.quadf.as<float4>(), c->st);
+        HIP_TRY(c, hipGetLastError());
+    }
     // the staging buffer and the caller's pixels are reused after return
     HIP_TRY(c, hipStreamSynchronize(c->st));
     dv.w = v->w;
@@ -360,11 +368,17 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
     return c;
 }
 
-int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h) {
+int ensure_map(mvs_ctx *c, MapDev &m, int32_t w, int32_t h, bool depth_only = false) {
     const size_t n = (size_t)w * h;
     HIP_TRY(c, m.depth.ensure(n * sizeof(float)));
-    HIP_TRY(c, m.normal.ensure(3 * n * sizeof(float)));
-    HIP_TRY(c, m.cost.ensure(n * sizeof(float)));
+    m.keep_normal = m.keep_cost = !depth_only;
+    if (depth_only) {
+        m.normal.release();
+        m.cost.release();
+    } else {
+        HIP_TRY(c, m.normal.ensure(3 * n * sizeof(float)));
+        HIP_TRY(c, m.cost.ensure(n * sizeof(float)));
+    }
     m.w = w;
     m.h = h;
     return MVS_OK;
@@ -728,6 +742,14 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         const int32_t ph =
             pinhole ? (ref.k1 == 0.0 && ref.k2 == 0.0 && ref.has_quad && lane_size ? 2 : 1) : 0;
         job.lane_ok = ph == 2 && quads && K <= 16 ? 1 : 0;
+        // a lean view (8-bit, without the f32 corner plane) serves only the
+        // fast kernels on undistorted views, which read the 8-bit and f16 planes
+        bool lean = !ref.quadf.p;
+        for (const ViewDev *sv : js.srcs) lean = lean || !sv->quadf.p;
+        if (lean && !(fast && job.lane_ok))
+            return fail(c, MVS_ERR_ARG,
+                        "a lean view (MVS_SURVEY_LEAN_VIEWS) serves only fast-mode jobs on "
+                        "undistorted 8-bit views");
         ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), ph, fast);
     }
 
@@ -867,12 +889,14 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
     for (int32_t i = 0; i < nj; ++i) {
         MapDev *m = jobs[(size_t)i].out;
         if (!m) continue;
-        TRY(ensure_map(c, *m, hj[(size_t)i].w, hj[(size_t)i].h));
-        launch_threshold(hj[(size_t)i], m->depth.as<float>(), m->normal.as<float>(), c->st);
+        TRY(ensure_map(c, *m, hj[(size_t)i].w, hj[(size_t)i].h, jobs[(size_t)i].depth_only));
+        launch_threshold(hj[(size_t)i], m->depth.as<float>(),
+                         m->keep_normal ? m->normal.as<float>() : nullptr, c->st);
         HIP_TRY(c, hipGetLastError());
-        HIP_TRY(c, hipMemcpyAsync(m->cost.p, hj[(size_t)i].cost,
-                                  sizeof(float) * (size_t)hj[(size_t)i].w * hj[(size_t)i].h,
-                                  hipMemcpyDeviceToDevice, c->st));
+        if (m->keep_cost)
+            HIP_TRY(c, hipMemcpyAsync(m->cost.p, hj[(size_t)i].cost,
+                                      sizeof(float) * (size_t)hj[(size_t)i].w * hj[(size_t)i].h,
+                                      hipMemcpyDeviceToDevice, c->st));
         m->valid = true;
     }
     if (!classes.empty()) {
@@ -885,6 +909,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
 
 int download_map(mvs_ctx *c, const MapDev &m, float *depth, float *normal, float *cost) {
     const size_t n = (size_t)m.w * m.h;
+    if ((normal && !m.keep_normal) || (cost && !m.keep_cost))
+        return fail(c, MVS_ERR_ARG, "this map keeps the depth only (MVS_SURVEY_DEPTH_ONLY)");
     if (depth)
         HIP_TRY(c, hipMemcpyAsync(depth, m.depth.p, n * sizeof(float), hipMemcpyDeviceToHost,
                                   c->st));
@@ -1299,7 +1325,15 @@ int mvs_survey_set_view(mvs_survey *s, int32_t i, const mvs_view *v) {
     if (i < 0 || i >= (int32_t)s->views.size())
         return fail(c, MVS_ERR_ARG, "view index out of range");
     HIP_TRY(c, hipSetDevice(c->device));
-    return upload_view(c, s->views[(size_t)i], v);
+    return upload_view(c, s->views[(size_t)i], v, (s->options & MVS_SURVEY_LEAN_VIEWS) != 0);
+}
+
+int mvs_survey_set_options(mvs_survey *s, int32_t options) {
+    if (!s) return MVS_ERR_ARG;
+    if (options & ~(MVS_SURVEY_LEAN_VIEWS | MVS_SURVEY_DEPTH_ONLY))
+        return fail(s->ctx, MVS_ERR_ARG, "unknown survey option");
+    s->options = options;
+    return MVS_OK;
 }
 
 }  // extern "C"
@@ -1336,6 +1370,7 @@ int survey_depth_maps(mvs_survey *s, int32_t njobs, const int32_t *refs, const i
         js.range1 = ranges[2 * j + 1];
         js.prm = prms[prm_step ? j : 0];
         js.out = &s->maps[(size_t)ref];
+        js.depth_only = (s->options & MVS_SURVEY_DEPTH_ONLY) != 0;
         if (js.srcs.empty()) {
             s->maps[(size_t)ref].valid = false;
             continue;
@@ -1419,11 +1454,11 @@ int mvs_survey_set_depth(mvs_survey *s, int32_t view, const float *depth) {
     HIP_TRY(c, hipSetDevice(c->device));
     const ViewDev &v = s->views[(size_t)view];
     MapDev &m = s->maps[(size_t)view];
-    TRY(ensure_map(c, m, v.w, v.h));
+    TRY(ensure_map(c, m, v.w, v.h, (s->options & MVS_SURVEY_DEPTH_ONLY) != 0));
     const size_t n = (size_t)v.w * v.h;
     HIP_TRY(c, hipMemcpy(m.depth.p, depth, n * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemset(m.normal.p, 0, 3 * n * sizeof(float)));
-    HIP_TRY(c, hipMemset(m.cost.p, 0, n * sizeof(float)));
+    if (m.keep_normal) HIP_TRY(c, hipMemset(m.normal.p, 0, 3 * n * sizeof(float)));
+    if (m.keep_cost) HIP_TRY(c, hipMemset(m.cost.p, 0, n * sizeof(float)));
     m.valid = true;
     return MVS_OK;
 }

@@ -331,10 +331,135 @@ def run_uneven(args):
     ctx.close()
 
 
+def run_large(args):
+    """The large survey (c5): 2000 views 2400x1800 (1400 nadir, 600 oblique)
+    rendered on the GPU, all resident in HBM as lean views (f32 grey, 8-bit
+    and f16 corner planes: 16 B per pixel) with depth-only maps; a few views
+    of each kind tune the schedules, then every view's depth map (fast
+    arithmetic, refinement) and the reprojection consistency filter are
+    timed in one survey call, which batches the views by memory."""
+    from aerial_mvs import capi, survey
+    if args.mode != "fast":
+        print("the large survey runs in fast mode only: its lean views keep no f32 plane",
+              file=sys.stderr)
+        sys.exit(2)
+    spec = survey.LARGE[args.config]
+    t = time.perf_counter()
+    surv, per = survey.make_large_survey(
+        spec, progress=lambda k, n: log(f"rendered {k}/{n} views"))
+    n_views = len(surv.views)
+    log(f"large survey {spec.name}: {n_views} views {spec.size[0]}x{spec.size[1]} rendered on "
+        f"the GPU in {time.perf_counter() - t:.1f} s")
+    centres = surv.centres()
+    jobs = []
+    for i in range(n_views):
+        rg = per[i]["range"]
+        nb = capi.select_neighbours(centres, i, surv.neighbours[i], rg[0], spec.neighbours)
+        if nb:
+            jobs.append((i, nb, rg))
+    if args.steps > 0:
+        jobs = jobs[:args.steps]
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=spec.iterations,
+                      refine_steps=args.refine_steps, fast=True)
+    ctx = capi.Context(0)
+    t = time.perf_counter()
+    sv = capi.Survey(ctx, surv.views,
+                     options=capi.MVS_SURVEY_LEAN_VIEWS | capi.MVS_SURVEY_DEPTH_ONLY)
+    ctx.synchronize()
+    gpix = sum(v.w * v.h for v in surv.views) / 1e9
+    log(f"{n_views} views resident in HBM ({gpix:.2f} Gpix, {16 * gpix:.0f} GB of planes) in "
+        f"{time.perf_counter() - t:.1f} s")
+    # tune the schedules on a few views of each kind (untimed)
+    warm = ([j for j in jobs if per[j[0]]["kind"] == "nadir"][:8] +
+            [j for j in jobs if per[j[0]]["kind"] == "oblique"][:8])
+    sv.depth_maps(warm, prm)
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    done_views = [0]
+
+    def progress(user, stage, k, n):
+        if k - done_views[0] >= 100 or k == n:
+            done_views[0] = k
+            log(f"  depth maps {k}/{n} ({time.perf_counter() - t0:.1f} s)")
+        return 0
+
+    done = set(j[0] for j in jobs)
+    t0 = time.perf_counter()
+    sv.depth_maps(jobs, prm, progress=progress)
+    n_cons = 0
+    for (i, _, _) in jobs:
+        nbrs = [j for j in surv.neighbours[i][:spec.neighbours + 1] if j in done]
+        if nbrs:
+            n_cons += sv.consistency(i, nbrs, rule=capi.MVS_CONS_REPROJ, download=False)[1]
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = ctx.stats()
+    pixels = sum(surv.views[j[0]].w * surv.views[j[0]].h for j in jobs)
+    value = pixels / elapsed / 1e6
+    log(f"timed {len(jobs)} views: {elapsed:.1f} s, {value:.2f} depth Mpix/s; stats {st}")
+    errs = {"nadir": [], "oblique": []}
+    for (i, _, _) in jobs:
+        if surv.views[i].depth_gt is not None:
+            d, _, _ = sv.download(i)
+            errs[per[i]["kind"]].append(depth_errors(d, surv.views[i].depth_gt))
+    err = {k: mean_errors(v) for k, v in errs.items() if v}
+    err["views_checked"] = {k: len(v) for k, v in errs.items()}
+    err["reference"] = "FP64 ray-cast ground truth (stored as f32) of every 25th view"
+    err["consistent_pixels"] = int(n_cons)
+    # roofline of the half-sweeps: flops per evaluation at the pixel-weighted mean K
+    kmean = sum(len(j[1]) * surv.views[j[0]].w * surv.views[j[0]].h for j in jobs) / pixels
+    fpe = spec.taps * spec.taps * (12 + 48 * kmean)
+    avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
+    evals_per_launch = st["sweep_evals"] / max(1, st["sweep_launches"])
+    achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", achieved=achieved, peak=157.3, unit="TFLOP/s",
+                    frac=achieved / 157.3, traffic=None, avg_launch_ms=avg_ms,
+                    launches=st["sweep_launches"], flops_per_eval=fpe,
+                    bound_note="FP32 peak; vector ALU kernel limited by its sample gathers")
+    cpu = None
+    if not args.no_cpu:
+        cw, ch = (int(a) for a in args.cpu_crop.split("x"))
+        log("CPU baseline (oracle) ...")
+        cpu = cpu_baseline(surv, jobs, spec, args.threads, cw, ch, ctx,
+                           capi.params(taps=spec.taps, stride=spec.stride,
+                                       iterations=spec.iterations),
+                           capi.params(taps=spec.taps, stride=spec.stride,
+                                       iterations=spec.iterations, fast=True))
+        log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
+    n_nadir = sum(1 for j in jobs if per[j[0]]["kind"] == "nadir")
+    line = dict(
+        metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=len(jobs),
+        warmup=len(warm), ms_per_step=elapsed * 1e3 / len(jobs), higher_is_better=True,
+        scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(workload=(f"{n_views} views {spec.size[0]}x{spec.size[1]} u8 grey "
+                              f"({n_nadir} nadir, {len(jobs) - n_nadir} oblique timed), "
+                              f"{spec.block[0] / 1000:g}x{spec.block[1] / 1000:g} km fractal "
+                              f"heightfield with {spec.boxes} boxes 3-30 m, "
+                              f"{spec.neighbours} neighbours, {spec.taps}x{spec.taps} NCC, "
+                              f"{spec.iterations} iterations, {args.refine_steps} refinement "
+                              "steps, reprojection consistency filter"),
+                    views=n_views, resident_gpix=gpix, layout="lean views + depth-only maps",
+                    mean_k=kmean,
+                    schedule=[dict(iteration=t_["iteration"], mode=t_["mode"], split=t_["split"],
+                                   gpc=t_["gpc"], per_cu=t_["per_cu"])
+                              for t_ in ctx.tuned_schedules()]),
+        roofline=roofline, cpu_baseline=cpu, depth_error=err,
+        gpu_vs_cpu=(value / cpu["value"]) if cpu else None,
+        stats=dict(sweep_ms=st["sweep_ms"], sweep_launches=st["sweep_launches"],
+                   init_ms=st["init_ms"], refine_ms=st["refine_ms"], evals=st["evals"],
+                   graph_replays=st["graph_replays"]),
+    )
+    print(json.dumps(line), flush=True)
+    sv.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=77)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="reference views timed (default 0: every view of the survey)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--mode", choices=("fast", "exact"), default="fast",
@@ -376,6 +501,8 @@ def main():
 
     if args.config in survey.UNEVEN:
         return run_uneven(args)
+    if args.config in survey.LARGE:
+        return run_large(args)
     spec = survey.SPECS[args.config]
     t = time.perf_counter()
     surv = survey.make_survey(spec, nthreads=args.threads)
@@ -387,7 +514,7 @@ def main():
         nb = capi.select_neighbours(centres, i, surv.neighbours[i], spec.depth_range[0],
                                     spec.neighbours)
         jobs.append((i, nb, spec.depth_range))
-    steps = max(1, min(args.steps, len(jobs)))
+    steps = len(jobs) if args.steps <= 0 else max(1, min(args.steps, len(jobs)))
     geom = dict(taps=spec.taps, stride=spec.stride, iterations=spec.iterations)
     prm = capi.params(**geom, refine_steps=args.refine_steps, fast=args.mode == "fast")
     prm_exact = capi.params(**geom)                    # the reference's algorithm

@@ -161,6 +161,18 @@ int mvs_survey_create(mvs_ctx *ctx, int32_t nviews, mvs_survey **out);
 void mvs_survey_destroy(mvs_survey *sv);
 /* Upload view i (pixels converted to device grey once). */
 int mvs_survey_set_view(mvs_survey *sv, int32_t i, const mvs_view *v);
+/* Residency options of a survey (no reference counterpart: they let a survey
+ * of thousands of views stay in one GPU's memory). They apply to views set
+ * and maps computed after the call.
+ * MVS_SURVEY_LEAN_VIEWS: an 8-bit view keeps its f32 grey and its 8-bit and
+ *   f16 corner planes only (16 bytes per pixel instead of 32); jobs that
+ *   touch such a view must run in fast mode on undistorted views (the
+ *   one-lane-per-pixel kernels), others fail with MVS_ERR_ARG.
+ * MVS_SURVEY_DEPTH_ONLY: maps keep the depth only (4 bytes per pixel instead
+ *   of 20); downloading their normals or costs fails with MVS_ERR_ARG. */
+#define MVS_SURVEY_LEAN_VIEWS 1
+#define MVS_SURVEY_DEPTH_ONLY 2
+int mvs_survey_set_options(mvs_survey *sv, int32_t options);
 
 /* One reference view of the survey (the body of the loop dense.rs:530-540,
  * without file I/O). Neighbour indices are survey view indices, already
