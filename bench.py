@@ -399,11 +399,21 @@ def run_large(args):
     value = pixels / elapsed / 1e6
     log(f"timed {len(jobs)} views: {elapsed:.1f} s, {value:.2f} depth Mpix/s; stats {st}")
     errs = {"nadir": [], "oblique": []}
+    errs_c = {"nadir": [], "oblique": []}
     for (i, _, _) in jobs:
         if surv.views[i].depth_gt is not None:
             d, _, _ = sv.download(i)
             errs[per[i]["kind"]].append(depth_errors(d, surv.views[i].depth_gt))
+            nbrs = [j for j in surv.neighbours[i][:spec.neighbours + 1] if j in done]
+            if nbrs:
+                agree, _ = sv.consistency(i, nbrs, rule=capi.MVS_CONS_REPROJ)
+                errs_c[per[i]["kind"]].append(depth_errors(np.where(agree >= 3, d, 0.0),
+                                                           surv.views[i].depth_gt))
     err = {k: mean_errors(v) for k, v in errs.items() if v}
+    for k, v in errs_c.items():
+        if v:
+            err[k + "_consistent"] = mean_errors(v)
+    err["consistency_rule"] = "reprojection <= 1 px in >= 3 of the K+1 nearest views"
     err["views_checked"] = {k: len(v) for k, v in errs.items()}
     err["reference"] = "FP64 ray-cast ground truth (stored as f32) of every 25th view"
     err["consistent_pixels"] = int(n_cons)
