@@ -172,13 +172,29 @@ __global__ void k_dsm_median3(int32_t nx, int32_t ny, const float *__restrict__ 
 // ---- 4-connected components (lock-free union-find, roots hook to smaller
 // indices, so parent chains always descend and every find ends) ----
 
-// Loads at device scope: other workgroups hook roots while this one walks,
-// and a stale cached parent could make a retry loop spin.
+// Loads and stores at device scope: other workgroups hook roots and shorten
+// paths while this one walks, and a stale cached parent could make a retry
+// loop spin.
 __device__ __forceinline__ int32_t cc_parent(const int32_t *parent, int32_t x) {
     return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int32_t cc_find(const int32_t *parent, int32_t x) {
+// The root of x, halving the path on the way: a node's parent is replaced by
+// its grandparent, an ancestor, so every chain still descends to the root.
+__device__ __forceinline__ int32_t cc_find(int32_t *parent, int32_t x) {
+    int32_t p = cc_parent(parent, x);
+    while (p != x) {
+        const int32_t g = cc_parent(parent, p);
+        if (g != p)
+            __hip_atomic_store(parent + x, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+        p = g;
+    }
+    return x;
+}
+
+// The root of x without writing (for passes where no root changes any more).
+__device__ __forceinline__ int32_t cc_root(const int32_t *parent, int32_t x) {
     int32_t p = cc_parent(parent, x);
     while (p != x) {
         x = p;
@@ -207,28 +223,45 @@ __device__ __forceinline__ bool cc_member(const float *h, int64_t k, bool want_f
     return isfinite(h[k]) == want_finite;
 }
 
-__global__ void k_cc_init(int64_t ncell, const float *__restrict__ h, bool want_finite,
+// Each member starts under its left neighbour when that is a member, else
+// under its upper one, else as a root (both have smaller indices); -1 for
+// other cells.
+__global__ void k_cc_init(int32_t nx, int32_t ny, const float *__restrict__ h, bool want_finite,
                           int32_t *__restrict__ parent) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncell) return;
-    parent[k] = cc_member(h, k, want_finite) ? (int32_t)k : -1;
+    if (k >= (int64_t)nx * ny) return;
+    // branch-free: both neighbours are read at clamped, in-bounds indices and
+    // the parent is a select (hipcc 7.2 dropped the root case of the nested
+    // if/else form for some lanes)
+    const int32_t kk = (int32_t)k, i = (int32_t)(k % nx);
+    const int32_t kl = i > 0 ? kk - 1 : kk, ku = kk >= nx ? kk - nx : kk;
+    const bool m = cc_member(h, kk, want_finite);
+    const bool left = i > 0 && cc_member(h, kl, want_finite);
+    const bool up = kk >= nx && cc_member(h, ku, want_finite);
+    const int32_t p = left ? kl : (up ? ku : kk);
+    parent[k] = m ? p : -1;
 }
 
+// The adjacencies the initial parents do not cover: a member whose left and
+// upper neighbours are both members is joined to the upper one.
 __global__ void k_cc_hook(int32_t nx, int32_t ny, const float *__restrict__ h, bool want_finite,
                           int32_t *parent) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int64_t)nx * ny || !cc_member(h, k, want_finite)) return;
-    const int32_t j = (int32_t)(k / nx), i = (int32_t)(k - (int64_t)j * nx);
-    if (i + 1 < nx && cc_member(h, k + 1, want_finite)) cc_unite(parent, (int32_t)k, (int32_t)k + 1);
-    if (j + 1 < ny && cc_member(h, k + nx, want_finite))
-        cc_unite(parent, (int32_t)k, (int32_t)(k + nx));
+    if (k >= (int64_t)nx * ny || k < nx || !cc_member(h, k, want_finite)) return;
+    const int32_t i = (int32_t)(k % nx);
+    if (i > 0 && cc_member(h, k - 1, want_finite) && cc_member(h, k - nx, want_finite))
+        cc_unite(parent, (int32_t)k, (int32_t)(k - nx));
 }
 
+// Every member points at its root afterwards; sizes per root. No root
+// changes here, and each thread writes only its own cell (the root, an
+// ancestor of every value it replaces), so concurrent walks stay valid.
 __global__ void k_cc_flatten(int64_t ncell, int32_t *parent, uint32_t *__restrict__ size) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncell || parent[k] < 0) return;
-    const int32_t r = cc_find(parent, (int32_t)k);
-    atomicAdd(&size[r], 1u);
+    if (k >= ncell || cc_parent(parent, (int32_t)k) < 0) return;
+    const int32_t r = cc_root(parent, (int32_t)k);
+    __hip_atomic_store(parent + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (size) atomicAdd(&size[r], 1u);
 }
 
 // dsm.rs:293-331: components smaller than min_cells become empty.
@@ -237,7 +270,7 @@ __global__ void k_cc_drop_small(int64_t ncell, const int32_t *parent,
                                 float *__restrict__ h) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncell || parent[k] < 0) return;
-    if (size[cc_find(parent, (int32_t)k)] < min_cells) h[k] = NAN;
+    if (size[parent[k]] < min_cells) h[k] = NAN;  // flattened: parent is the root
 }
 
 // dsm.rs:337-359: empty components touching the border are "outside".
@@ -255,7 +288,7 @@ __global__ void k_cc_mark_outside(int32_t nx, int32_t ny, const int32_t *parent,
         k = (t - 2 * (int64_t)nx) * nx;
     else
         k = (t - 2 * (int64_t)nx - ny) * nx + nx - 1;
-    if (parent[k] >= 0) outside[cc_find(parent, (int32_t)k)] = 1u;
+    if (parent[k] >= 0) outside[parent[k]] = 1u;  // flattened: parent is the root
 }
 
 // The enclosed empty cells: the first to-do list of dsm.rs:360-362.
@@ -264,7 +297,7 @@ __global__ void k_cc_enclosed(int64_t ncell, const int32_t *parent,
                               uint32_t *__restrict__ ntodo) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncell || parent[k] < 0) return;
-    if (!outside[cc_find(parent, (int32_t)k)]) todo[atomicAdd(ntodo, 1u)] = (int32_t)k;
+    if (!outside[parent[k]]) todo[atomicAdd(ntodo, 1u)] = (int32_t)k;  // flattened
 }
 
 // One pass of dsm.rs:363-389: every listed cell with a finite 4-neighbour in
@@ -483,7 +516,8 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
     int32_t *parent = w.parent;
     uint32_t *size = w.u32b;  // top1 is no longer needed
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nc)), dim3(256), 0, st, size, nc, 0u);
-    hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, nc, d_h, true, parent);
+    hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, true,
+                       parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, true,
                        parent);
     hipLaunchKernelGGL(k_cc_flatten, dim3(nblk(nc)), dim3(256), 0, st, nc, parent, size);
@@ -494,9 +528,12 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
     // enclosed holes (dsm.rs:222): components of empty cells off the border
     uint32_t *outside = w.u32c;  // eq is no longer needed
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nc)), dim3(256), 0, st, outside, nc, 0u);
-    hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, nc, d_h, false, parent);
+    hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, false,
+                       parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, false,
                        parent);
+    hipLaunchKernelGGL(k_cc_flatten, dim3(nblk(nc)), dim3(256), 0, st, nc, parent,
+                       (uint32_t *)nullptr);
     hipLaunchKernelGGL(k_cc_mark_outside, dim3(nblk(2 * (g.nx + g.ny))), dim3(256), 0, st, gg.nx,
                        gg.ny, parent, outside);
     uint32_t *counts = w.counters;  // [0] list a, [1] list b, [2] zmax key
