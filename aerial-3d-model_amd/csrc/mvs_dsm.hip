@@ -243,14 +243,26 @@ __global__ void k_cc_init(int32_t nx, int32_t ny, const float *__restrict__ h, b
 }
 
 // The adjacencies the initial parents do not cover: a member whose left and
-// upper neighbours are both members is joined to the upper one.
+// upper neighbours are both members is joined to the upper one — only at the
+// first cell of each stretch where two rows' runs touch: further along, the
+// cell's left neighbour is already joined to its own upper neighbour, and the
+// row links carry that over (k ~ k-1 ~ k-1-nx ~ k-nx).
 __global__ void k_cc_hook(int32_t nx, int32_t ny, const float *__restrict__ h, bool want_finite,
                           int32_t *parent) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int64_t)nx * ny || k < nx || !cc_member(h, k, want_finite)) return;
     const int32_t i = (int32_t)(k % nx);
-    if (i > 0 && cc_member(h, k - 1, want_finite) && cc_member(h, k - nx, want_finite))
+    if (i > 0 && cc_member(h, k - 1, want_finite) && cc_member(h, k - nx, want_finite) &&
+        !cc_member(h, k - 1 - nx, want_finite))
         cc_unite(parent, (int32_t)k, (int32_t)(k - nx));
+}
+
+// Path halving from every member (no root changes any more): the walks of
+// the flatten pass that follows become short.
+__global__ void k_cc_compress(int64_t ncell, int32_t *parent) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncell || cc_parent(parent, (int32_t)k) < 0) return;
+    (void)cc_find(parent, (int32_t)k);
 }
 
 // Every member points at its root afterwards; sizes per root. No root
@@ -261,7 +273,16 @@ __global__ void k_cc_flatten(int64_t ncell, int32_t *parent, uint32_t *__restric
     if (k >= ncell || cc_parent(parent, (int32_t)k) < 0) return;
     const int32_t r = cc_root(parent, (int32_t)k);
     __hip_atomic_store(parent + k, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (size) atomicAdd(&size[r], 1u);
+    if (size) {
+        // most lanes of a wave share a root (one large component): they add
+        // their count with one atomic, the others one each
+        const int32_t lead = __builtin_amdgcn_readfirstlane(r);
+        const unsigned long long same = __ballot(r == lead);
+        if (r != lead)
+            atomicAdd(&size[r], 1u);
+        else if ((int)__lane_id() == __ffsll((long long)same) - 1)
+            atomicAdd(&size[lead], (uint32_t)__popcll(same));
+    }
 }
 
 // dsm.rs:293-331: components smaller than min_cells become empty.
@@ -520,6 +541,7 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
                        parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, true,
                        parent);
+    hipLaunchKernelGGL(k_cc_compress, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
     hipLaunchKernelGGL(k_cc_flatten, dim3(nblk(nc)), dim3(256), 0, st, nc, parent, size);
     const double mc = 25.0 / (g.cell * g.cell);
     const uint32_t min_cells = mc >= 4294967295.0 ? 0xffffffffu : (uint32_t)mc;
@@ -532,6 +554,7 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
                        parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, false,
                        parent);
+    hipLaunchKernelGGL(k_cc_compress, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
     hipLaunchKernelGGL(k_cc_flatten, dim3(nblk(nc)), dim3(256), 0, st, nc, parent,
                        (uint32_t *)nullptr);
     hipLaunchKernelGGL(k_cc_mark_outside, dim3(nblk(2 * (g.nx + g.ny))), dim3(256), 0, st, gg.nx,
