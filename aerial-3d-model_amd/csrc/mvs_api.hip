@@ -146,6 +146,9 @@ struct mvs_survey {
     DevBuf fkeys, fsum, fcol, fcnt, fctr;
     int64_t fcap = 0;
     double fvoxel = 0.0;
+    int64_t fvox = 0;          // voxels of the last fusion
+    bool fpts_ready = false;   // its points are in fo_xyz / fo_rgb
+    DevBuf fo_kin, fo_kout, fo_sin, fo_sout, fo_tmp, fo_xyz, fo_rgb, fo_ctr;
     bool fused = false;
 };
 
@@ -1302,7 +1305,9 @@ void mvs_survey_destroy(mvs_survey *s) {
     if (!s) return;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->st);
-    DevBuf *fb[] = {&s->fkeys, &s->fsum, &s->fcol, &s->fcnt, &s->fctr};
+    DevBuf *fb[] = {&s->fkeys,  &s->fsum,   &s->fcol,  &s->fcnt,   &s->fctr,
+                    &s->fo_kin, &s->fo_kout, &s->fo_sin, &s->fo_sout, &s->fo_tmp,
+                    &s->fo_xyz, &s->fo_rgb, &s->fo_ctr};
     for (DevBuf *b : fb) b->release();
     for (auto &v : s->views) {
         v.gray.release();
@@ -1708,6 +1713,8 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
         }
         if (ctr[2] > 0) return fail(c, MVS_ERR_NOMEM, "fusion table full");
         s->fvoxel = voxel;
+        s->fvox = (int64_t)ctr[0];
+        s->fpts_ready = false;
         s->fused = true;
         if (n_voxels) *n_voxels = (int64_t)ctr[0];
         return MVS_OK;
@@ -1719,31 +1726,44 @@ int mvs_survey_fused_points(mvs_survey *s, float *xyz, uint8_t *rgb, int64_t cap
     mvs_ctx *c = s->ctx;
     if (!s->fused) return fail(c, MVS_ERR_ARG, "no fusion has run");
     HIP_TRY(c, hipSetDevice(c->device));
-    const int64_t tc = s->fcap;
-    std::vector<unsigned long long> keys((size_t)tc), sum((size_t)tc * 3);
-    std::vector<unsigned int> col((size_t)tc * 3), cnt((size_t)tc);
-    HIP_TRY(c, hipMemcpy(keys.data(), s->fkeys.p, (size_t)tc * 8, hipMemcpyDeviceToHost));
-    std::vector<std::pair<unsigned long long, int64_t>> occ;
-    for (int64_t i = 0; i < tc; ++i)
-        if (keys[(size_t)i] != ~0ull) occ.emplace_back(keys[(size_t)i], i);
-    if (n) *n = (int64_t)occ.size();
+    const int64_t nv = s->fvox;
+    if (n) *n = nv;
     if (!xyz && !rgb) return MVS_OK;
-    if (cap < (int64_t)occ.size()) return fail(c, MVS_ERR_ARG, "output arrays too small");
-    HIP_TRY(c, hipMemcpy(sum.data(), s->fsum.p, (size_t)tc * 24, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(col.data(), s->fcol.p, (size_t)tc * 12, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(cnt.data(), s->fcnt.p, (size_t)tc * 4, hipMemcpyDeviceToHost));
-    // one point per voxel, in voxel-index order (x, then y, then z)
-    std::sort(occ.begin(), occ.end());
-    for (size_t o = 0; o < occ.size(); ++o) {
-        const int64_t i = occ[o].second;
-        const unsigned int k = cnt[(size_t)i];
-        // dense.rs:637-641: the mean position and the mean colour, truncated
-        for (int r = 0; r < 3; ++r) {
-            if (xyz)
-                xyz[3 * o + r] =
-                    (float)((double)(long long)sum[(size_t)(3 * i + r)] / 1048576.0 / (double)k);
-            if (rgb) rgb[3 * o + r] = (uint8_t)((float)col[(size_t)(3 * i + r)] / (float)k);
-        }
+    if (cap < nv) return fail(c, MVS_ERR_ARG, "output arrays too small");
+    if (nv > 0xffffffffll) return fail(c, MVS_ERR_ARG, "more than 2^32 voxels");
+    if (!s->fpts_ready && nv > 0) {
+        // compaction, radix sort by voxel key and the means, on the device
+        const size_t tb = fuse_points_tmp_bytes(nv);
+        if (!tb) return fail(c, MVS_ERR_DEVICE, "radix sort size query failed");
+        HIP_TRY(c, s->fo_kin.ensure((size_t)nv * 8));
+        HIP_TRY(c, s->fo_kout.ensure((size_t)nv * 8));
+        HIP_TRY(c, s->fo_sin.ensure((size_t)nv * 4));
+        HIP_TRY(c, s->fo_sout.ensure((size_t)nv * 4));
+        HIP_TRY(c, s->fo_tmp.ensure(tb));
+        HIP_TRY(c, s->fo_xyz.ensure((size_t)nv * 12));
+        HIP_TRY(c, s->fo_rgb.ensure((size_t)nv * 3));
+        HIP_TRY(c, s->fo_ctr.ensure(8));
+        FusePointsWork w{s->fo_kin.as<unsigned long long>(), s->fo_kout.as<unsigned long long>(),
+                         s->fo_sin.as<uint32_t>(), s->fo_sout.as<uint32_t>(), s->fo_tmp.p, tb,
+                         s->fo_ctr.as<unsigned long long>()};
+        if (fuse_points(fuse_table(s), nv, w, s->fo_xyz.as<float>(), s->fo_rgb.as<uint8_t>(),
+                        c->st))
+            return hip_fail(c, hipGetLastError(), "fused points");
+        unsigned long long got = 0;
+        HIP_TRY(c, hipMemcpyAsync(&got, w.ctr, 8, hipMemcpyDeviceToHost, c->st));
+        HIP_TRY(c, hipStreamSynchronize(c->st));
+        if ((int64_t)got != nv)
+            return fail(c, MVS_ERR_DEVICE, "fusion table and voxel count disagree");
+        s->fpts_ready = true;
+    }
+    if (nv > 0) {
+        if (xyz)
+            HIP_TRY(c, hipMemcpyAsync(xyz, s->fo_xyz.p, (size_t)nv * 12, hipMemcpyDeviceToHost,
+                                      c->st));
+        if (rgb)
+            HIP_TRY(c, hipMemcpyAsync(rgb, s->fo_rgb.p, (size_t)nv * 3, hipMemcpyDeviceToHost,
+                                      c->st));
+        HIP_TRY(c, hipStreamSynchronize(c->st));
     }
     return MVS_OK;
 }

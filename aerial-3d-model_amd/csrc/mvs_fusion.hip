@@ -9,6 +9,8 @@
 #include "mvs_device.h"
 #include "mvs_geometry.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace amvs {
 namespace {
 
@@ -148,7 +150,69 @@ __global__ __launch_bounds__(kTileSlots) void k_fuse(
     if ((tid & 63) == 0 && pts) atomicAdd(&t.counters[1], pts);
 }
 
+// The occupied slots as (key, slot) pairs; one counter atomic per wavefront.
+// Writes stop at n (the voxel count); the count returned lets the caller check
+// that it saw exactly n.
+__global__ void k_fuse_compact(FuseTable t, int64_t n, unsigned long long *__restrict__ okeys,
+                               uint32_t *__restrict__ oslots, unsigned long long *ctr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long key = i < t.cap ? t.keys[i] : kEmpty;
+    const bool occ = key != kEmpty;
+    const unsigned long long m = __ballot(occ);
+    if (m == 0ull) return;
+    const int lane = (int)__lane_id();
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    unsigned long long base = 0;
+    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    if (occ && (int64_t)(base + below) < n) {
+        okeys[base + below] = key;
+        oslots[base + below] = (uint32_t)i;
+    }
+}
+
+// dense.rs:634-641: one point per voxel, the mean position (f64 from the
+// fixed-point sums, then f32) and the mean colour truncated to bytes.
+__global__ void k_fuse_points(FuseTable t, const uint32_t *__restrict__ slots, int64_t n,
+                              float *__restrict__ xyz, uint8_t *__restrict__ rgb) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int64_t i = slots[o];
+    const unsigned int k = t.cnt[i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        xyz[3 * o + r] = (float)((double)(long long)t.sum[3 * i + r] / kFix / (double)k);
+        rgb[3 * o + r] = (uint8_t)((float)t.col[3 * i + r] / (float)k);
+    }
+}
+
 }  // namespace
+
+int fuse_points(const FuseTable &t, int64_t n, FusePointsWork &w, float *xyz, uint8_t *rgb,
+                hipStream_t st) {
+    if (n <= 0) return 0;
+    if (hipMemsetAsync(w.ctr, 0, 8, st) != hipSuccess) return -1;
+    const int64_t blocks = (t.cap + 255) / 256;
+    hipLaunchKernelGGL(k_fuse_compact, dim3((unsigned)blocks), dim3(256), 0, st, t, n,
+                       w.keys_in, w.slots_in, w.ctr);
+    size_t bytes = w.tmp_bytes;
+    if (rocprim::radix_sort_pairs(w.tmp, bytes, w.keys_in, w.keys_out, w.slots_in, w.slots_out,
+                                  (size_t)n, 0, 64, st) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(k_fuse_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t,
+                       w.slots_out, n, xyz, rgb);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+size_t fuse_points_tmp_bytes(int64_t n) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, (unsigned long long *)nullptr,
+                                  (unsigned long long *)nullptr, (uint32_t *)nullptr,
+                                  (uint32_t *)nullptr, (size_t)n, 0, 64) != hipSuccess)
+        return 0;
+    return bytes;
+}
 
 void launch_fuse_clear(const FuseTable &t, hipStream_t st) {
     const int64_t blocks = std::min<int64_t>((t.cap + 255) / 256, 65536);

@@ -149,6 +149,19 @@ struct FuseTable {
     int64_t cap;                   // power of two
 };
 void launch_fuse_clear(const FuseTable &t, hipStream_t st);
+// The fused cloud on the device (dense.rs:634-641): the n occupied slots
+// compacted, sorted by voxel key (radix sort) and averaged into xyz (3n f32)
+// and rgb (3n u8), in the order of the keys. 0 or -1 (HIP error).
+struct FusePointsWork {
+    unsigned long long *keys_in, *keys_out;  // n each
+    uint32_t *slots_in, *slots_out;          // n each
+    void *tmp;                               // tmp_bytes
+    size_t tmp_bytes;
+    unsigned long long *ctr;                 // 1
+};
+size_t fuse_points_tmp_bytes(int64_t n);
+int fuse_points(const FuseTable &t, int64_t n, FusePointsWork &w, float *xyz, uint8_t *rgb,
+                hipStream_t st);
 // One reference view: its confirmed pixels into the table (dense.rs:589-632).
 // Colour from rgb (w*h*3, nullable), else the pixel's 8-bit grey (quad byte 0,
 // nullable), else the f32 grey x 255.
