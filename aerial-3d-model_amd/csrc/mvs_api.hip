@@ -2001,7 +2001,8 @@ int mvs_dsm_download(mvs_ctx *c, float *h, int64_t cap) {
 }
 
 int mvs_save_dsm(const char *path, const mvs_dsm_info *info, const float *h) {
-    if (!path || !info || info->nx < 0 || info->ny < 0 || (info->nx * info->ny > 0 && !h))
+    if (!path || !info || info->nx < 0 || info->ny < 0 || info->nx > (1ll << 31) ||
+        info->ny > (1ll << 31) || (info->nx * info->ny > 0 && !h))
         return MVS_ERR_ARG;
     // Dsm::save (dsm.rs:27-46): "DSM1", x0 y0 cell f64, nx ny u64, zmax f32,
     // heights f32, all little-endian; tmp + rename
@@ -2050,8 +2051,11 @@ int mvs_load_dsm(const char *path, mvs_dsm_info *info, float *h, int64_t cap) {
     info->nx = (int64_t)n2[0];
     info->ny = (int64_t)n2[1];
     info->reserved = 0;
-    const uint64_t nc = n2[0] * n2[1];
-    if (n2[0] > (1ull << 31) || n2[1] > (1ull << 31) || (uint64_t)len != 48 + 4 * nc) {
+    // the header's grid against the file length, without overflow
+    const uint64_t body = len >= 48 ? (uint64_t)len - 48 : ~0ull;
+    const uint64_t nc = n2[0] * n2[1];  // both at most 2^31: no wrap
+    if (n2[0] > (1ull << 31) || n2[1] > (1ull << 31) || body == ~0ull || body % 4 != 0 ||
+        nc != body / 4) {
         fclose(f);
         return MVS_ERR_FORMAT;  // "truncated surface model file"
     }
