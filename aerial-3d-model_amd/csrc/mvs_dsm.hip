@@ -83,38 +83,44 @@ __global__ void k_dsm_heights(int64_t ncell, const uint32_t *__restrict__ cnt,
 }
 
 // dsm.rs:183-201: median (total order) of the finite values in the window of
-// radius R; false when fewer than 5.
+// radius R; false when fewer than 5. The element of rank n/2 in the stable
+// total-order sort is found by counting, fully unrolled, so the window stays
+// in registers.
 template <int R>
 __device__ bool window_median(const float *__restrict__ h, int32_t nx, int32_t ny, int32_t i,
                               int32_t j, float &m) {
-    constexpr int W = (2 * R + 1) * (2 * R + 1);
+    constexpr int D = 2 * R + 1, W = D * D;
     float v[W];
     int32_t key[W];
+    bool ok[W];
     int n = 0;
 #pragma unroll
-    for (int dj = -R; dj <= R; ++dj)
-#pragma unroll
-        for (int di = -R; di <= R; ++di) {
-            const int32_t a = i + di, b = j + dj;
-            if (a >= 0 && b >= 0 && a < nx && b < ny) {
-                const float x = h[(int64_t)b * nx + a];
-                if (isfinite(x)) {
-                    // insertion into the sorted prefix
-                    const int32_t kx = tkey(x);
-                    int q = n;
-                    while (q > 0 && key[q - 1] > kx) {
-                        key[q] = key[q - 1];
-                        v[q] = v[q - 1];
-                        --q;
-                    }
-                    key[q] = kx;
-                    v[q] = x;
-                    ++n;
-                }
-            }
-        }
+    for (int q = 0; q < W; ++q) {
+        const int32_t a = i + q % D - R, b = j + q / D - R;
+        const bool inb = a >= 0 && b >= 0 && a < nx && b < ny;
+        const float x = inb ? h[(int64_t)b * nx + a] : NAN;
+        ok[q] = isfinite(x);
+        key[q] = tkey(x);
+        v[q] = x;
+        n += ok[q] ? 1 : 0;
+    }
     if (n < 5) return false;
-    m = v[n / 2];
+    const int want = n / 2;
+    float res = 0.0f;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        int rank = 0;
+#pragma unroll
+        for (int p = 0; p < W; ++p) {
+            if (p == q) continue;
+            // values ahead of q in the stable sort: smaller keys, or equal
+            // keys met earlier in the reference's row-major window order
+            const bool ahead = p < q ? key[p] <= key[q] : key[p] < key[q];
+            rank += (ok[p] && ahead) ? 1 : 0;
+        }
+        if (ok[q] && rank == want) res = v[q];
+    }
+    m = res;
     return true;
 }
 
@@ -242,6 +248,19 @@ __global__ void k_cc_init(int32_t nx, int32_t ny, const float *__restrict__ h, b
     parent[k] = m ? p : -1;
 }
 
+// Pointer jumping on the initial row chains (no root changes yet): each pass
+// replaces a parent by its parent, so after log2(width) passes every member
+// points at the start of its run or close to it, and the finds that follow
+// are short.
+__global__ void k_cc_jump(int64_t ncell, int32_t *parent) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncell) return;
+    const int32_t p = cc_parent(parent, (int32_t)k);
+    if (p < 0) return;
+    const int32_t g = cc_parent(parent, p);
+    if (g != p) __hip_atomic_store(parent + k, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The adjacencies the initial parents do not cover: a member whose left and
 // upper neighbours are both members is joined to the upper one — only at the
 // first cell of each stretch where two rows' runs touch: further along, the
@@ -351,13 +370,23 @@ __global__ void k_fill_pass(int32_t nx, int32_t ny, const float *__restrict__ sn
         next[atomicAdd(nnext, 1u)] = k;
 }
 
-__global__ void k_dsm_zmax(int64_t ncell, const float *__restrict__ h, uint32_t *__restrict__ zk) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The highest finite height: a grid-stride pass per thread, then the
+// block's maximum, one atomic per block.
+__global__ __launch_bounds__(256) void k_dsm_zmax(int64_t ncell, const float *__restrict__ h,
+                                                  uint32_t *__restrict__ zk) {
+    __shared__ float wmax[4];
     float v = -INFINITY;
-    if (k < ncell && isfinite(h[k])) v = h[k];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncell;
+         k += (int64_t)gridDim.x * blockDim.x)
+        if (isfinite(h[k])) v = fmaxf(v, h[k]);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0 && v > -INFINITY) atomicMax(zk, fkey(v));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        if (m > -INFINITY) atomicMax(zk, fkey(m));
+    }
 }
 
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
@@ -535,10 +564,17 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
                        d_h);
     // islands (dsm.rs:220): 4-connected components of finite cells
     int32_t *parent = w.parent;
+    // ceil(log2(nx + ny)) + 1 passes: the initial chains run along rows and
+    // up the first cells of runs (longer chains are only shortened, and every
+    // walk still ends)
+    int jumps = 1;
+    while ((1ll << (jumps - 1)) < g.nx + g.ny) ++jumps;
     uint32_t *size = w.u32b;  // top1 is no longer needed
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nc)), dim3(256), 0, st, size, nc, 0u);
     hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, true,
                        parent);
+    for (int pass = 0; pass < jumps; ++pass)
+        hipLaunchKernelGGL(k_cc_jump, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, true,
                        parent);
     hipLaunchKernelGGL(k_cc_compress, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
@@ -552,6 +588,8 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(nc)), dim3(256), 0, st, outside, nc, 0u);
     hipLaunchKernelGGL(k_cc_init, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, false,
                        parent);
+    for (int pass = 0; pass < jumps; ++pass)
+        hipLaunchKernelGGL(k_cc_jump, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
     hipLaunchKernelGGL(k_cc_hook, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, d_h, false,
                        parent);
     hipLaunchKernelGGL(k_cc_compress, dim3(nblk(nc)), dim3(256), 0, st, nc, parent);
@@ -594,7 +632,8 @@ int dsm_build(const DsmGrid &g, const float *d_xyz, int64_t n, float *d_h, DsmWo
     if (hipMemcpyAsync(snap, d_h, hb, hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
     hipLaunchKernelGGL(k_dsm_median3, dim3(nblk(nc)), dim3(256), 0, st, gg.nx, gg.ny, snap, d_h);
     // zmax (dsm.rs:271-275): f32::MIN when there is no surface
-    hipLaunchKernelGGL(k_dsm_zmax, dim3(nblk(nc)), dim3(256), 0, st, nc, d_h, counts + 2);
+    hipLaunchKernelGGL(k_dsm_zmax, dim3((unsigned)std::min<int64_t>(nblk(nc), 1024)), dim3(256), 0,
+                       st, nc, d_h, counts + 2);
     uint32_t zk = 0;
     if (hipMemcpyAsync(&zk, counts + 2, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
