@@ -308,6 +308,10 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     const float lo = job.dmin * 0.7f, hi = job.dmax * 1.4f;
     float lambda = 1e-3f;
     unsigned long long acc = 0;
+    // J^T J, the gradient and the cost at the current plane: after a rejected
+    // step the plane is unchanged, so they are reused, not recomputed
+    float Hm[6], g[3], cost0 = 0.0f;
+    bool have_terms = false;
     for (int it = 0; it < steps; ++it) {
         if (!live) break;
         // tangent basis of the normal
@@ -325,19 +329,24 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
             {e1[0], e1[1], e1[2]},
             {nrm[1] * e1[2] - nrm[2] * e1[1], nrm[2] * e1[0] - nrm[0] * e1[2],
              nrm[0] * e1[1] - nrm[1] * e1[0]}};
-        const LTerms<3> TA = lane_terms<TAPS, 3>(Ss[p0], pr, rstep, hp.x, nrm, dirs, Ah, lane);
-        const LTerms<3> TB = lane_terms<TAPS, 3>(Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane);
-        live = TA.ok && TB.ok;
-        if (!live) break;
-        // J^T J and the NCC gradient summed over the pair
-        float Hm[6], g[3];
+        if (!have_terms) {
+            const LTerms<3> TA =
+                lane_terms<TAPS, 3>(Ss[p0], pr, rstep, hp.x, nrm, dirs, Ah, lane);
+            const LTerms<3> TB =
+                lane_terms<TAPS, 3>(Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane);
+            live = TA.ok && TB.ok;
+            if (!live) break;
+            // J^T J and the NCC gradient summed over the pair
 #pragma unroll
-        for (int q = 0; q < 6; ++q) Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
+            for (int q = 0; q < 6; ++q)
+                Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
-            g[q] = -(TA.Dta[q] / pr.na - TA.ncc * TA.Dtb[q] / TA.nb) / TA.nb -
-                   (TB.Dta[q] / pr.na - TB.ncc * TB.Dtb[q] / TB.nb) / TB.nb;
-        const float cost0 = 1.0f - 0.5f * (TA.ncc + TB.ncc);
+            for (int q = 0; q < 3; ++q)
+                g[q] = -(TA.Dta[q] / pr.na - TA.ncc * TA.Dtb[q] / TA.nb) / TA.nb -
+                       (TB.Dta[q] / pr.na - TB.ncc * TB.Dtb[q] / TB.nb) / TB.nb;
+            cost0 = 1.0f - 0.5f * (TA.ncc + TB.ncc);
+            have_terms = true;
+        }
         const float Hd[6] = {Hm[0] * (1.0f + lambda), Hm[1], Hm[2], Hm[3] * (1.0f + lambda),
                              Hm[4], Hm[5] * (1.0f + lambda)};
         const float rhs[3] = {-g[0], -g[1], -g[2]};
@@ -373,6 +382,7 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
             nrm[2] = n1[2];
             lambda = fmaxf(lambda * 0.1f, 1e-6f);
             ++acc;
+            have_terms = false;
         } else {
             lambda = fminf(lambda * 10.0f, 1e6f);
         }
