@@ -589,6 +589,8 @@ def main():
     avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
     evals_per_launch = st["sweep_evals"] / max(1, st["sweep_launches"])
     achieved = fpe * evals_per_launch / (avg_ms * 1e-3) / 1e12
+    # bilinear samples gathered per evaluation: taps x sources
+    samples_per_eval = spec.taps * spec.taps * spec.neighbours
     rd, wr = BYTES_PER_EVAL[args.mode]
     traffic = (rd + wr) * evals_per_launch if rd is not None else None
     sched = [t for t in ctx.tuned_schedules()
@@ -613,7 +615,10 @@ def main():
                     hbm_gbs=(traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
                     kernel=kname,
                     avg_launch_ms=avg_ms, evals_per_launch=evals_per_launch, flops_per_eval=fpe,
-                    launches=st["sweep_launches"])
+                    launches=st["sweep_launches"],
+                    samples_per_s=samples_per_eval * evals_per_launch / (avg_ms * 1e-3),
+                    samples_note=("bilinear samples gathered per second by the half-sweeps "
+                                  "(taps x sources per evaluation), the quantity that bounds them"))
 
     # depth error against the FP64 ground truth, all timed views: every valid
     # pixel, and the pixels the consistency filter keeps
