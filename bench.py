@@ -531,8 +531,11 @@ def main():
     prm_fast0 = capi.params(**geom, fast=True)         # fast, no refinement
 
     ctx = capi.Context(0)
+    tu = time.perf_counter()
     sv = capi.Survey(ctx, surv.views)
-    log("views resident in HBM")
+    ctx.synchronize()
+    upload_s = time.perf_counter() - tu
+    log(f"views resident in HBM (upload and planes: {upload_s:.2f} s)")
     for w in range(args.warmup):
         j = jobs[w % len(jobs)]
         sv.depth_maps([j], prm)
@@ -722,6 +725,11 @@ def main():
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
         graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
         fusion=fusion, view_prep=prep, dsm=dsm,
+        end_to_end=dict(upload_s=upload_s,
+                        upload_gb=sum(v.w * v.h for v in surv.views) / 1e9,
+                        mpix_s_with_upload=pixels / (elapsed + upload_s) / 1e6,
+                        note="host-to-device upload of all views (8-bit, pageable) and their "
+                             "planes, before the timed region; not part of value"),
     )
     print(json.dumps(line), flush=True)
     sv.close()
