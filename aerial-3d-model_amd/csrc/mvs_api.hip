@@ -1176,6 +1176,8 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     a.n_items = n;
     rc = pm_launch(c, PM_COST_LIST, prm->taps, k, a, n);
     if (rc) return abort_call(c, rc);
+    c->stats.split = k.sch.split;  // the layout that scored the list
+    c->stats.gpc = k.sch.gpc;
     HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost,
                               c->st));
     return sync_ctx(c);

@@ -106,14 +106,42 @@ __global__ void k_rays(double f, double cx, double cy, double k1, double k2, int
     rays[i] = make_float2((float)px, (float)py);
 }
 
+// The threads of k_init walk the view in tiles of kListTileW x kListTileH
+// pixels (tiles row-major, pixels row-major inside a tile), so the parity
+// lists come out tile by tile and the sweeps, which take list items in order,
+// have a compact square of pixels in flight on each XCD instead of a band of
+// full rows: while the hypotheses are still scattered, the source lines a
+// square reaches along its epipolar segments are a fraction of a band's. Each
+// wavefront covers 64 pixels of one tile row (kListTileW is a multiple of 64),
+// so a list run is still 32 pixels of one row. The order changes no result:
+// every list reader takes (x, y) from the stored index.
+#ifndef AMVS_LIST_TILE_W
+#define AMVS_LIST_TILE_W 128
+#endif
+#ifndef AMVS_LIST_TILE_H
+#define AMVS_LIST_TILE_H 256
+#endif
+constexpr int32_t kListTileW = AMVS_LIST_TILE_W;
+constexpr int32_t kListTileH = AMVS_LIST_TILE_H;
+static_assert(kListTileW % 64 == 0 && kListTileH > 0, "list tiles hold whole wavefront rows");
+
 // Initial hypotheses (dense.rs:365-381), texture mask (dense.rs:383-402) and
-// the per-parity lists of textured pixels the sweeps iterate over.
+// the per-parity lists of textured pixels the sweeps iterate over; thread t
+// takes pixel (x, y) of the tiled walk above.
 __global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t write_hyp) {
     const int32_t w = job.w, h = job.h;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < (int64_t)w * h;
-    const int32_t y = valid ? (int32_t)(i / w) : 0;
-    const int32_t x = valid ? (int32_t)(i - (int64_t)y * w) : 0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t ntx = (w + kListTileW - 1) / kListTileW;
+    const int64_t tile = t / (kListTileW * kListTileH);
+    const int32_t r = (int32_t)(t - tile * (kListTileW * kListTileH));
+    const int32_t ty = (int32_t)(tile / ntx);
+    const int32_t tx = (int32_t)(tile - (int64_t)ty * ntx);
+    const int32_t yt = ty * kListTileH + r / kListTileW;
+    const int32_t xt = tx * kListTileW + r % kListTileW;
+    const bool valid = xt < w && yt < h;
+    const int32_t y = valid ? yt : 0;
+    const int32_t x = valid ? xt : 0;
+    const int64_t i = (int64_t)y * w + x;
     bool tex = false;
     if (valid) {
         if (write_hyp) {
@@ -297,7 +325,9 @@ __global__ void k_zero_i32(int32_t *p, int32_t n) {
 }
 
 void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t st) {
-    const int64_t n = (int64_t)job.w * job.h;
+    const int64_t ntx = (job.w + kListTileW - 1) / kListTileW;
+    const int64_t nty = (job.h + kListTileH - 1) / kListTileH;
+    const int64_t n = ntx * nty * kListTileW * kListTileH;  // tiles cover the view
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, job.counts, 4);
     hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, job, taps,
                        write_hyp);

@@ -258,8 +258,9 @@ typedef struct {
     int64_t pixels;           /* reference pixels processed */
     int64_t init_evals;       /* of which in the initialisation */
     int64_t sweep_evals;      /* of which in the half-sweeps */
-    int32_t split;            /* schedule of the last batch: lanes per source
-                               * (-1: one lane per pixel) */
+    int32_t split;            /* schedule of the last batch or cost list: lanes
+                               * per source (-1: one lane per pixel, f16 corner
+                               * plane; -2: one lane per pixel, 8-bit plane) */
     int32_t gpc;              /* and pixel groups per work chunk */
     int64_t refine_launches;  /* Gauss-Newton refinement kernel */
     double refine_ms;

@@ -14,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from aerial_mvs import capi
+from aerial_mvs import capi, survey
 
 import oracle as orc
 from conftest import survey_cached
@@ -134,6 +134,91 @@ def test_pixel_lane_depth_maps_within_tolerance(gpu_ctx, name, k, iters, split):
     assert both.sum() > 1000
     assert within >= 0.99
     assert valid_agree >= 0.98
+
+
+def _oblique():
+    """An oblique 45-degree view of the small uneven survey and its nearest
+    views (nadir and oblique): window scales far from one in the sources."""
+    if "oblique" not in _OBL:
+        s, per = survey.make_uneven_survey("uneven_small")
+        i = max(j for j, p in enumerate(per) if p["kind"] == "oblique")
+        _OBL["oblique"] = (s, i, per[i]["range"])
+    return _OBL["oblique"]
+
+
+_OBL = {}
+
+
+@pytest.mark.parametrize("name,taps,stride,k", [("tiny", 5, 2, 4), ("small", 7, 1, 8),
+                                                ("small", 11, 1, 8), ("oblique", 7, 1, 6),
+                                                ("oblique", 11, 1, 6)])
+def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k):
+    """The 8-bit and the f16 corner planes hold the same corner values, so the
+    one-lane-per-pixel kernels give the same costs, bit for bit, on either
+    (split -2: 8-bit plane, bytes unpacked and differenced per sample; split
+    -1: f16 plane with the column differences stored). Hypotheses near the
+    surface and scattered, on nadir views and on an oblique view whose sources
+    see its windows at scales far from one."""
+    if name == "oblique":
+        s, ref, rng_d = _oblique()
+        nb = list(s.neighbours[ref][1:k + 1]) if s.neighbours[ref][0] == ref else \
+            list(s.neighbours[ref][:k])
+    else:
+        s = survey_cached(name)
+        ref = s.reference_index()
+        nb = s.neighbours[ref][:k]
+        rng_d = s.spec.depth_range
+    v = s.views[ref]
+    srcs = [s.views[j] for j in nb]
+    rng = np.random.default_rng(700 + taps + k)
+    n = 30000  # the oblique view's costs are defined for ~3% of random items
+    xs = rng.integers(0, v.w, n).astype(np.int32)
+    ys = rng.integers(0, v.h, n).astype(np.int32)
+    hy = np.zeros((n, 4), np.float32)
+    close = rng.uniform(size=n) < 0.5
+    gt = v.depth_gt[ys, xs]
+    hy[:, 0] = np.where(close & (gt > 0), gt * rng.uniform(0.98, 1.02, n),
+                        rng.uniform(rng_d[0], rng_d[1], n))
+    up = np.asarray(v.R, np.float64).reshape(3, 3) @ np.array([0.0, 0.0, 1.0])  # camera frame
+    nrm = rng.normal(up, np.where(close, 0.05, 0.5)[:, None], (n, 3))
+    hy[:, 1:] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    prm = capi.params(taps=taps, stride=stride, fast=True)
+    got = {}
+    _say(f"planes {name} {taps}x{taps} K={k}: view {ref} {v.w}x{v.h}, sources {nb}")
+    try:
+        for split in (-1, -2):
+            gpu_ctx.set_schedule(split, 1, 16, False)
+            gpu_ctx.reset_stats()
+            got[split] = gpu_ctx.ncc_cost(v, srcs, *rng_d, prm, xs, ys, hy)
+            assert gpu_ctx.stats()["split"] == split
+    finally:
+        gpu_ctx.set_schedule(0, 0, 16, True)
+    defined = got[-1] < 2.0
+    _say(f"  defined {defined.mean():.3f}, identical {np.mean(got[-1] == got[-2]):.6f}")
+    assert defined.sum() >= 800
+    assert np.array_equal(got[-1], got[-2])
+
+
+def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx):
+    """Whole depth maps (initial costs, every half-sweep) on the 8-bit plane
+    equal those on the f16 plane."""
+    s = survey_cached("small")
+    spec = s.spec
+    ref = s.reference_index()
+    v = s.views[ref]
+    srcs = [s.views[j] for j in s.neighbours[ref][:8]]
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=4, fast=True)
+    maps = {}
+    try:
+        for split in (-1, -2):
+            _say(f"planes: depth map small split={split}")
+            gpu_ctx.set_schedule(split, 1, 16, False)
+            maps[split] = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+    finally:
+        gpu_ctx.set_schedule(0, 0, 16, True)
+    for a, b in zip(maps[-1], maps[-2]):
+        assert np.array_equal(a, b)
+    assert (maps[-1][0] > 0).mean() > 0.5
 
 
 def test_pixel_lane_schedule_leaves_exact_mode_exact(gpu_ctx, tiny_survey):
