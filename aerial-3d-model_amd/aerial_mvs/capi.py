@@ -68,7 +68,8 @@ class MvsStats(C.Structure):
                 ("split", C.c_int32), ("gpc", C.c_int32),
                 ("refine_launches", C.c_int64), ("refine_ms", C.c_double),
                 ("refine_pixels", C.c_int64), ("refine_accepted", C.c_int64),
-                ("graph_captures", C.c_int64), ("graph_replays", C.c_int64)]
+                ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
+                ("fuse_passes", C.c_int64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)
@@ -103,7 +104,7 @@ def lib():
         L.mvs_last_error.restype = C.c_char_p
         L.mvs_synchronize.argtypes = [vp]
         L.mvs_depth_map.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams),
-                                    vp, vp, vp, vp]
+                                    vp, vp, vp, vp, PROGRESS_FN, vp]
         L.mvs_ncc_cost.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
                                    vp, vp, vp, vp]
         L.mvs_ncc_pair.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
@@ -390,16 +391,22 @@ class Context:
             arr[i] = make_view(s, keep)
         return r, arr, keep
 
-    def depth_map(self, ref, srcs, range0, range1, prm, cancel=None):
-        """Returns (depth, normal, cost) or None when there are no neighbours."""
+    def depth_map(self, ref, srcs, range0, range1, prm, cancel=None, progress=None):
+        """Returns (depth, normal, cost) or None when there are no neighbours.
+        progress(stage, done, total) -> truthy to cancel."""
         r, arr, keep = self._views(ref, srcs)
         h, w = ref.h, ref.w
         d = np.zeros((h, w), np.float32)
         n = np.zeros((h, w, 3), np.float32)
         c = np.zeros((h, w), np.float32)
         cflag = None if cancel is None else C.cast(C.pointer(cancel), C.c_void_p)
+        cb = PROGRESS_FN()
+        if progress is not None:
+            cb = PROGRESS_FN(lambda user, stage, done, total:
+                             1 if progress(stage.decode(), done, total) else 0)
         rc = lib().mvs_depth_map(self._h, C.byref(r), arr, len(srcs), range0, range1,
-                                 C.byref(prm), d.ctypes.data, n.ctypes.data, c.ctypes.data, cflag)
+                                 C.byref(prm), d.ctypes.data, n.ctypes.data, c.ctypes.data, cflag,
+                                 cb, None)
         if self.check(rc, (MVS_OK, MVS_NO_NEIGHBOURS)) == MVS_NO_NEIGHBOURS:
             return None
         return d, n, c

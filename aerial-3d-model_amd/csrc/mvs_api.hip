@@ -653,9 +653,17 @@ int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
 // first_step on the jobs' current state. n_steps < 0: all 2 x iterations
 // half-sweeps of each job, then refinement and thresholded outputs for jobs
 // that have an output map.
+// step_report (nullable) is called before each half-sweep is enqueued with
+// (half-sweeps enqueued, half-sweeps in all) under stage "patchmatch", where
+// cancel is polled; a nonzero return cancels.
+struct StepReport {
+    mvs_progress_fn fn;
+    void *user;
+};
+
 int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
               int32_t n_steps, const volatile int32_t *cancel,
-              std::vector<KClass> *classes_out) {
+              std::vector<KClass> *classes_out, const StepReport *step_report = nullptr) {
     const int32_t nj = (int32_t)jobs.size();
     if (nj == 0) return MVS_OK;
     for (const JobSpec &j : jobs) {
@@ -826,6 +834,9 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
     std::vector<const KClass *> active;
     for (int32_t s = s0; s < s0 + max_steps; ++s) {
         if (cancel && *cancel) return fail(c, MVS_ERR_CANCELLED, "cancelled");
+        if (step_report && step_report->fn &&
+            step_report->fn(step_report->user, "patchmatch", s - s0, max_steps))
+            return fail(c, MVS_ERR_CANCELLED, "cancelled");
         const int32_t it = s / 2, parity = s % 2;
         bool known = true;
         active.clear();
@@ -1113,7 +1124,8 @@ int32_t mvs_tuned_schedules(mvs_ctx *c, int32_t *out, int32_t cap) {
 
 int mvs_depth_map(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
                   double range0, double range1, const mvs_params *prm, float *out_depth,
-                  float *out_normal, float *out_cost, const volatile int32_t *cancel) {
+                  float *out_normal, float *out_cost, const volatile int32_t *cancel,
+                  mvs_progress_fn progress, void *user) {
     if (!c) return MVS_ERR_ARG;
     TRY(check_params(c, prm));
     if (nsrc < 1) return fail(c, MVS_NO_NEIGHBOURS, "no neighbours");
@@ -1128,9 +1140,14 @@ int mvs_depth_map(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t
     jobs[0].range1 = range1;
     jobs[0].out = &c->tmp_map;
     jobs[0].prm = *prm;
-    int rc = run_batch(c, jobs, -1, -1, cancel, nullptr);
+    const StepReport rep{progress, user};
+    int rc = run_batch(c, jobs, -1, -1, cancel, nullptr, &rep);
     if (rc) return abort_call(c, rc);
-    return download_map(c, c->tmp_map, out_depth, out_normal, out_cost);
+    TRY(download_map(c, c->tmp_map, out_depth, out_normal, out_cost));
+    const int32_t total = 2 * prm->iterations;
+    if (progress && progress(user, "patchmatch", total, total))
+        return fail(c, MVS_ERR_CANCELLED, "cancelled");
+    return MVS_OK;
 }
 
 int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
@@ -1636,6 +1653,7 @@ FuseTable fuse_table(mvs_survey *s) {
     t.cnt = s->fcnt.as<unsigned int>();
     t.counters = s->fctr.as<unsigned long long>();
     t.cap = s->fcap;
+    t.limit = s->fcap / 10 * 7;
     return t;
 }
 
@@ -1660,11 +1678,14 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
     }
     HIP_TRY(c, hipSetDevice(c->device));
     // a table of about one slot per 32 pixels (a voxel gathers tens of
-    // points); grown 4x and refilled when it passes 70% occupancy
+    // points); a pass that fills it past 70% stops inserting, and the pass is
+    // repeated with a table sized from the voxels per point it saw
+    constexpr int64_t kMaxCap = 1ll << 30;
     int64_t cap = 1 << 16;
-    while (cap < pixels / 32 && cap < (1ll << 30)) cap <<= 1;
+    while (cap < pixels / 32 && cap < kMaxCap) cap <<= 1;
     std::vector<NbrCam> cams;
-    for (;;) {  // ends: the table grows to at most 2^30 slots
+    for (;;) {  // ends: every repeat at least quadruples the table, up to 2^30 slots
+        c->stats.fuse_passes += 1;
         HIP_TRY(c, s->fkeys.ensure((size_t)cap * 8));
         HIP_TRY(c, s->fsum.ensure((size_t)cap * 24));
         HIP_TRY(c, s->fcol.ensure((size_t)cap * 12));
@@ -1706,11 +1727,19 @@ int mvs_survey_fuse(mvs_survey *s, int32_t nrefs, const int32_t *refs, const int
         HIP_TRY(c, hipStreamSynchronize(c->st));
         unsigned long long ctr[4];
         HIP_TRY(c, hipMemcpy(ctr, s->fctr.p, sizeof ctr, hipMemcpyDeviceToHost));
-        const bool grow = (ctr[2] > 0 || (double)ctr[0] > 0.7 * (double)cap) && cap < (1ll << 30);
+        const bool grow = (ctr[2] > 0 || (double)ctr[0] > 0.7 * (double)cap) && cap < kMaxCap;
         if (ctr[3] > 0)
             return fail(c, MVS_ERR_ARG, "voxel index beyond +-2^20: voxel too small for the scene");
         if (grow) {
-            cap <<= 2;  // start again with a larger table
+            // every confirmed point was counted, stored or not: scale the
+            // voxels seen by the points still missing (an over-estimate, as
+            // later views revisit voxels of earlier ones) and start again with
+            // the table at most half full
+            const double pts = (double)ctr[1] + (double)ctr[2];
+            const double est = ctr[1] ? (double)ctr[0] * pts / (double)ctr[1] : 4.0 * (double)cap;
+            int64_t next = cap << 2;
+            while ((double)next < 2.0 * est && next < kMaxCap) next <<= 1;
+            cap = next;
             continue;
         }
         if (ctr[2] > 0) return fail(c, MVS_ERR_NOMEM, "fusion table full");

@@ -33,14 +33,22 @@ __global__ void k_fuse_clear(FuseTable t) {
     if (blockIdx.x == 0 && threadIdx.x < 4) t.counters[threadIdx.x] = 0;
 }
 
-// Global insert of one voxel's aggregated sums (open addressing, linear
-// probing bounded by the capacity). Returns false when the table is full.
+// Longest probe sequence of a global insert. Below the table's occupancy
+// limit (70%) linear probing under mix64 ends within a few tens of slots; an
+// insert that finds no slot within this many is counted as dropped, which
+// makes the host rebuild the table larger, so a full table costs each insert
+// at most this many probes instead of a walk over the whole capacity.
+constexpr int64_t kMaxProbe = 1024;
+
+// Global insert of one voxel's aggregated sums (open addressing, bounded
+// linear probing). Returns false when no slot was found.
 __device__ bool fuse_insert(const FuseTable &t, unsigned long long key,
                             const unsigned long long s3[3], const unsigned int c3[3],
                             unsigned int n) {
     const unsigned long long mask = (unsigned long long)t.cap - 1ull;
     unsigned long long slot = dev::mix64(key) & mask;
-    for (int64_t probe = 0; probe < t.cap; ++probe) {
+    const int64_t probes = t.cap < kMaxProbe ? t.cap : kMaxProbe;
+    for (int64_t probe = 0; probe < probes; ++probe) {
         const unsigned long long prev = atomicCAS(&t.keys[slot], kEmpty, key);
         if (prev == kEmpty) atomicAdd(&t.counters[0], 1ull);
         if (prev == kEmpty || prev == key) {
@@ -73,6 +81,7 @@ __global__ __launch_bounds__(kTileSlots) void k_fuse(
     __shared__ unsigned int lcol[kTileSlots][3];
     __shared__ unsigned int lcnt[kTileSlots];
     __shared__ unsigned int lrange;
+    __shared__ unsigned int lfull;
     const int tid = threadIdx.y * kTile + threadIdx.x;
     lkey[tid] = kEmpty;
 #pragma unroll
@@ -81,7 +90,20 @@ __global__ __launch_bounds__(kTileSlots) void k_fuse(
         lcol[tid][r] = 0;
     }
     lcnt[tid] = 0;
-    if (tid == 0) lrange = 0;
+    if (tid == 0) {
+        lrange = 0;
+        // Once the table holds `limit` voxels, or an insert has been dropped,
+        // this pass will be repeated with a larger table: the workgroup still
+        // aggregates its points (their count sizes the next table) but makes
+        // no global inserts. Read once per workgroup, so a few workgroups in
+        // flight may still insert past the limit; the bounded probe covers
+        // them.
+        const unsigned long long vox =
+            __hip_atomic_load(&t.counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long drop =
+            __hip_atomic_load(&t.counters[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lfull = (vox >= (unsigned long long)t.limit || drop != 0ull) ? 1u : 0u;
+    }
     __syncthreads();
     const int32_t x = blockIdx.x * kTile + threadIdx.x, y = blockIdx.y * kTile + threadIdx.y;
     if (x < w && y < h) {
@@ -140,7 +162,7 @@ __global__ __launch_bounds__(kTileSlots) void k_fuse(
     if (n) {
         const unsigned long long s3[3] = {lsum[tid][0], lsum[tid][1], lsum[tid][2]};
         const unsigned int c3[3] = {lcol[tid][0], lcol[tid][1], lcol[tid][2]};
-        stored = fuse_insert(t, lkey[tid], s3, c3, n);
+        stored = lfull ? false : fuse_insert(t, lkey[tid], s3, c3, n);
         if (!stored) atomicAdd(&t.counters[2], (unsigned long long)n);  // points dropped
     }
     // points accumulated: one atomic per wavefront

@@ -145,8 +145,10 @@ struct FuseTable {
     unsigned long long *sum;       // 3 per slot
     unsigned int *col;             // 3 per slot: colour sums
     unsigned int *cnt;             // 1 per slot
-    unsigned long long *counters;  // [0] voxels, [1] points, [2] table full, [3] key range
+    unsigned long long *counters;  // [0] voxels, [1] points stored, [2] points dropped
+                                   // (table full), [3] points beyond the key range
     int64_t cap;                   // power of two
+    int64_t limit;                 // occupancy past which a pass stops inserting
 };
 void launch_fuse_clear(const FuseTable &t, hipStream_t st);
 // The fused cloud on the device (dense.rs:634-641): the n occupied slots

@@ -31,6 +31,46 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(REPO, "aerial-3d-model_amd")]
 
 METRIC = ("depth Mpix/s (7x7 NCC, 8 nbrs) batched+refined GPU vs plain GPU vs CPU; depth err")
+
+
+def metric_for(taps, k):
+    """The metric string of a survey run with a taps x taps window and k
+    neighbours (the 7x7, 8-neighbour survey's is METRIC)."""
+    return (f"depth Mpix/s ({taps}x{taps} NCC, {k:g} nbrs) batched+refined GPU vs plain GPU vs "
+            "CPU; depth err")
+
+
+def host_cpus():
+    """The host's CPUs as this process sees them: the logical CPUs of the
+    machine, the CPUs this process may run on, the thread budget the
+    environment sets (OMP_NUM_THREADS), and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return dict(nproc=os.cpu_count(), affinity=affinity,
+                omp_num_threads=int(omp) if omp and omp.isdigit() else None, model=model)
+
+
+def default_threads():
+    """Every CPU this process may run on, capped by the environment's thread
+    budget (OMP_NUM_THREADS) where one is set: on a shared host that budget
+    is this job's share of the cores."""
+    h = host_cpus()
+    n = h["affinity"]
+    if h["omp_num_threads"]:
+        n = min(n, h["omp_num_threads"])
+    return max(1, n)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
@@ -40,6 +80,15 @@ PEAK_HBM_GBS = 8000.0
 # one lane per pixel) and profiles/k_pm_sweep_exact_8x2000x1500_k8_7x7_counters.md
 # hold the per-launch figures.
 BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (969.8, 4.99)}
+
+
+# The roofline object's "bound" names the compute roofline ("mfma": the FP32
+# peak of 157.3 TFLOP/s, MI355X's figure for vector and matrix FP32 alike);
+# the half-sweep kernel itself runs on the vector ALUs.
+LIMITER = "valu"
+BOUND_NOTE = ("compute roofline at MI355X's FP32 peak (157.3 TFLOP/s); the kernel issues no "
+              "matrix instructions: it runs on the vector ALUs (limiter 'valu'), and its "
+              "counters show the texture-address unit of the sample gathers busy beside them")
 
 
 def log(*a):
@@ -95,6 +144,11 @@ def cpu_baseline(surv, jobs, spec, nthreads, crop_w, crop_h, gpu_ctx, prm_exact,
     gf, _, _ = gpu_ctx.depth_map(cv, sv, rng[0], rng[1], prm_fast)
     mpix = crop_w * crop_h / 1e6
     return dict(value=mpix / dt, unit="depth Mpix/s", cores=nthreads, kind="port",
+                host=dict(host_cpus(), threads_used=nthreads,
+                          note="threads = the CPUs this process may run on, capped by "
+                               "OMP_NUM_THREADS (the job's share of a shared host); rows of "
+                               "each half-sweep split over them (the rayon model, "
+                               "dense.rs:417)"),
                 sample=(f"oracle C restatement of recon/dense.rs, {crop_w}x{crop_h} centre crop "
                         f"of view {ref} vs its {len(srcs)} full-size neighbours, "
                         f"{spec.taps}x{spec.taps} window, {spec.iterations} iterations: "
@@ -426,7 +480,7 @@ def run_large(args):
     roofline = dict(bound="mfma", achieved=achieved, peak=157.3, unit="TFLOP/s",
                     frac=achieved / 157.3, traffic=None, avg_launch_ms=avg_ms,
                     launches=st["sweep_launches"], flops_per_eval=fpe,
-                    bound_note="FP32 peak; vector ALU kernel limited by its sample gathers")
+                    limiter=LIMITER, bound_note=BOUND_NOTE)
     cpu = None
     if not args.no_cpu:
         cw, ch = (int(a) for a in args.cpu_crop.split("x"))
@@ -439,7 +493,8 @@ def run_large(args):
         log(f"CPU {cpu['value']:.4f} Mpix/s on {cpu['cores']} threads")
     n_nadir = sum(1 for j in jobs if per[j[0]]["kind"] == "nadir")
     line = dict(
-        metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=len(jobs),
+        metric=metric_for(spec.taps, spec.neighbours), value=value, unit="depth Mpix/s",
+        n_gpus=1, steps=len(jobs),
         warmup=len(warm), ms_per_step=elapsed * 1e3 / len(jobs), higher_is_better=True,
         scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(workload=(f"{n_views} views {spec.size[0]}x{spec.size[1]} u8 grey "
@@ -483,7 +538,9 @@ def main():
     ap.add_argument("--no-prep", action="store_true",
                     help="skip the view-preparation stage (RGB photo -> .view contents)")
     ap.add_argument("--cpu-crop", default="1280x960")
-    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--threads", type=int, default=default_threads(),
+                    help="CPU threads of the oracle leg (default: the CPUs this process may "
+                         "run on, capped by OMP_NUM_THREADS)")
     ap.add_argument("--plain-views", type=int, default=8)
     ap.add_argument("--refine-steps", type=int, default=5)
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -609,10 +666,7 @@ def main():
                  f"{'fast' if args.mode == 'fast' else 'exact'}>")
     roofline = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic,
-                    bound_note=("compute roofline at MI355X's FP32 peak (157.3 TFLOP/s, its figure "
-                                "for vector and matrix FP32 alike); the kernel computes on the "
-                                "vector ALUs, no matrix instructions, and its measured limiter is "
-                                "the texture-address unit of the sample gathers"),
+                    limiter=LIMITER, bound_note=BOUND_NOTE,
                     traffic_note=("HBM-side bytes per launch: rocprofv3 FETCH_SIZE/WRITE_SIZE per "
                                   "evaluation (copy-calibrated) x evaluations per launch"),
                     hbm_gbs=(traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
@@ -637,6 +691,21 @@ def main():
     err = mean_errors(errs)
     if errs_c:
         err["consistent"] = mean_errors(errs_c)
+        # the refined-depth bar (within 0.2% of the FP64 truth on >= 99% of
+        # valid pixels) is declared on the pixels the consistency filter
+        # keeps: the set fusion takes into the cloud. Raw maps keep every
+        # pixel under the cost threshold (dense.rs:468-477), including the
+        # PatchMatch mismatches at occlusions and image borders that the
+        # reference's algorithm makes too (the plain exact maps, identical to
+        # the oracle, carry the same share).
+        c = err["consistent"]
+        err["refined_bar"] = dict(
+            valid_set="pixels kept by the reprojection consistency filter (>= 3 of the K+1 "
+                      "nearest views agree within 1 px)",
+            valid_fraction=c["valid"], within_0p2=c["within_0p2"],
+            bar="within 0.2% of the FP64 ground truth on >= 99% of valid pixels",
+            passed=bool(c["within_0p2"] >= 0.99),
+            raw_maps_within_0p2=err["within_0p2"], raw_maps_valid_fraction=err["valid"])
     err["consistent_pixels"] = int(n_cons)
     err["consistency_rule"] = "reprojection <= 1 px in >= 3 of the K+1 nearest views"
     err["reference"] = "FP64 ray-cast ground truth of the synthetic scene"
@@ -700,8 +769,10 @@ def main():
             cpu["gradient_check"] = gradient_check(ctx, surv, timed, spec, sv, n_grad)
             log(f"gradient check: {cpu['gradient_check']}")
 
+    metric = (METRIC if (spec.taps, spec.neighbours) == (7, 8)
+              else metric_for(spec.taps, spec.neighbours))
     line = dict(
-        metric=METRIC, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,
+        metric=metric, value=value, unit="depth Mpix/s", n_gpus=1, steps=steps,
         warmup=args.warmup, ms_per_step=elapsed * 1e3 / steps, higher_is_better=True,
         scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(workload=(f"{len(surv.views)} nadir views {spec.width}x{spec.height} u8 grey, "

@@ -117,10 +117,15 @@ typedef int (*mvs_progress_fn)(void *user, const char *stage, int32_t done, int3
  * depth range = cam.depth_range (dense.rs:362). Outputs are caller-allocated
  * host arrays of w*h (normal: 3*w*h, xyz per pixel); out_normal / out_cost may
  * be NULL. out_depth follows dense.rs:468-477 (0 = invalid). cancel (nullable)
- * is polled between half-sweeps. */
+ * is polled between half-sweeps. progress (nullable) is called with stage
+ * "patchmatch" before each half-sweep is enqueued (done = half-sweeps enqueued,
+ * total = 2 x iterations) and once more with done = total when the maps are in
+ * the output arrays; a nonzero return cancels (MVS_ERR_CANCELLED), like the
+ * reference's per-view Ctx::check / Ctx::report pair (dense.rs:531, 539). */
 int mvs_depth_map(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
                   double range0, double range1, const mvs_params *prm, float *out_depth,
-                  float *out_normal, float *out_cost, const volatile int32_t *cancel);
+                  float *out_normal, float *out_cost, const volatile int32_t *cancel,
+                  mvs_progress_fn progress, void *user);
 
 /* ---- parity entry points (lock-step driver) ----
  * Cost of explicit (pixel, hypothesis) pairs: Ctx2::cost (dense.rs:181-263).
@@ -268,6 +273,9 @@ typedef struct {
     int64_t refine_accepted;  /* accepted refinement steps */
     int64_t graph_captures;   /* half-sweep graphs captured */
     int64_t graph_replays;    /* half-sweeps run as a graph launch */
+    int64_t fuse_passes;      /* passes of mvs_survey_fuse over its views (a
+                               * pass that overflows the voxel table is
+                               * repeated with a larger one) */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);

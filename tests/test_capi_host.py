@@ -37,6 +37,34 @@ def test_library_exports_every_declared_symbol():
         assert getattr(L, name) is not None
 
 
+def _header_arity():
+    """{function: argument count} of the header's declarations."""
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"\b(mvs_[a-z0-9_]+)\s*\(([^;]*?)\)\s*;", txt, flags=re.S):
+        if name.endswith("_fn"):
+            continue
+        a = args.strip()
+        out[name] = 0 if a in ("", "void") else a.count(",") + 1
+    return out
+
+
+def test_integration_binding_covers_the_header():
+    """The Rust extern block a maintainer would add declares every function of
+    the C ABI with the header's argument count."""
+    md = open(os.path.join(REPO, "INTEGRATION.md")).read()
+    block = md[md.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = {}
+    for name, args in re.findall(r"pub fn (mvs_[a-z0-9_]+)\s*\(([^;]*?)\)\s*(?:->[^;]*)?;",
+                                 block, flags=re.S):
+        a = args.strip().rstrip(",")
+        rust[name] = 0 if not a else a.count(",") + 1
+    want = _header_arity()
+    assert set(want) == _declared()
+    assert rust == want
+
+
 def test_default_params_are_the_reference_constants():
     p = capi.params()
     q = capi.MvsParams()

@@ -92,6 +92,39 @@ def test_fusion_matches_oracle(gpu_ctx, tiny_survey, rule):
         sv.close()
 
 
+def test_fusion_table_regrows_to_oracle_cloud(gpu_ctx, tiny_survey):
+    """A voxel far below the ground sampling distance gives about one voxel
+    per confirmed pixel: more than the first table (2^16 slots for the tiny
+    survey, 70% limit) holds. The pass that overflows stops inserting, counts
+    its points, and the repeat with a table sized from them must give the
+    oracle's cloud."""
+    s = tiny_survey
+    sv, maps = _maps(gpu_ctx, s)
+    try:
+        refs = list(range(len(s.views)))
+        nbrs = [list(s.neighbours[i][:s.spec.neighbours + 1]) for i in refs]
+        voxel = 0.01
+        rgb = {i: np.repeat(s.views[i].gray[:, :, None], 3, axis=2) for i in refs}
+        gpu_ctx.reset_stats()
+        _say(f"fusion regrow: {len(refs)} views, voxel {voxel}")
+        xyz, col = sv.fuse(refs, nbrs, voxel)
+        passes = gpu_ctx.stats()["fuse_passes"]
+        keys, mean, ocol, npts = _oracle_cloud(s, maps, refs, nbrs, voxel,
+                                               capi.MVS_CONS_REFERENCE, rgb)
+        _say(f"  voxels gpu {len(xyz)} oracle {len(keys)} from {npts} points, passes {passes}")
+        assert len(keys) > 0.7 * 65536
+        assert passes == 2
+        assert len(xyz) == len(keys)
+        assert np.abs(xyz.astype(np.float64) - mean).max() <= 2e-5
+        assert np.array_equal(col, ocol)
+        # a larger voxel afterwards starts from the small table again: one pass
+        gpu_ctx.reset_stats()
+        xyz_c, _ = sv.fuse(refs, nbrs, 0.2)
+        assert gpu_ctx.stats()["fuse_passes"] == 1 and 0 < len(xyz_c) < len(xyz)
+    finally:
+        sv.close()
+
+
 def test_fused_cloud_to_ply(gpu_ctx, tiny_survey, tmp_path):
     s = tiny_survey
     sv, maps = _maps(gpu_ctx, s)

@@ -237,6 +237,22 @@ def test_edge_cases(gpu_ctx, tiny_survey):
     with pytest.raises(capi.MvsError) as e:
         gpu_ctx.depth_map(rv, [s.views[0]], 40, 80, prm, cancel=flag)
     assert e.value.code == -3
+    _say("case edge: progress reports")
+    nb = [s.views[j] for j in s.neighbours[ref][:4]]
+    seen = []
+    d0, _, _ = gpu_ctx.depth_map(rv, nb, 40, 80, prm)
+    d1, _, _ = gpu_ctx.depth_map(rv, nb, 40, 80, prm,
+                                 progress=lambda st, k, n: seen.append((st, k, n)) and False)
+    total = 2 * prm.iterations
+    assert seen == [("patchmatch", k, total) for k in range(total + 1)]
+    assert np.array_equal(d0, d1)
+    _say("case edge: cancel from the progress callback")
+    with pytest.raises(capi.MvsError) as e:
+        gpu_ctx.depth_map(rv, nb, 40, 80, prm, progress=lambda st, k, n: k >= 3)
+    assert e.value.code == -3
+    # the context stays usable after a cancelled call
+    d2, _, _ = gpu_ctx.depth_map(rv, nb, 40, 80, prm)
+    assert np.array_equal(d0, d2)
     _say("case edge: untextured reference")
     flat = copy.copy(rv)
     flat.gray = np.full_like(rv.gray, 90)
