@@ -42,7 +42,7 @@ EXPORTS = [
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
     "mvs_survey_set_depth", "mvs_build_dsm", "mvs_dsm_download", "mvs_save_dsm", "mvs_survey_set_options",
-    "mvs_load_dsm", "mvs_dsm_height",
+    "mvs_load_dsm", "mvs_dsm_height", "mvs_refine_terms",
 ]
 
 
@@ -109,6 +109,8 @@ def lib():
                                    vp, vp, vp, vp]
         L.mvs_ncc_pair.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64, P(MvsParams), i64,
                                    vp, vp, vp, vp, vp, vp, vp]
+        L.mvs_refine_terms.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64,
+                                       P(MvsParams), i64, vp, vp, vp, vp, vp, vp, vp]
         L.mvs_patchmatch_steps.argtypes = [vp, P(MvsView), P(MvsView), i32, f64, f64,
                                            P(MvsParams), i32, i32, vp, vp, P(C.c_int64)]
         L.mvs_survey_create.argtypes = [vp, i32, P(vp)]
@@ -443,6 +445,30 @@ class Context:
                                       hy.ctypes.data, pr.ctypes.data, _ptr(go), out.ctypes.data,
                                       _ptr(grad)))
         return out, pr, grad
+
+    def refine_terms(self, ref, srcs, range0, range1, prm, xs, ys, hyps, pairs=None):
+        """The refinement kernels' own Gauss-Newton terms of (pixel, plane,
+        pair) items: (cost, pairs, grad (n, 4) w.r.t. (d, n0, n1, n2), J^T J
+        (n, 4, 4)). pairs: None or rows of -1 select the best pair."""
+        r, arr, keep = self._views(ref, srcs)
+        xs = np.ascontiguousarray(xs, np.int32)
+        ys = np.ascontiguousarray(ys, np.int32)
+        hy = np.ascontiguousarray(hyps, np.float32).reshape(-1, 4)
+        n = len(xs)
+        pr = (np.full((n, 2), -1, np.int32) if pairs is None
+              else np.array(pairs, np.int32).reshape(n, 2).copy())
+        out = np.zeros(n, np.float32)
+        grad = np.zeros((n, 4), np.float32)
+        j10 = np.zeros((n, 10), np.float32)
+        self.check(lib().mvs_refine_terms(self._h, C.byref(r), arr, len(srcs), range0, range1,
+                                          C.byref(prm), n, xs.ctypes.data, ys.ctypes.data,
+                                          hy.ctypes.data, pr.ctypes.data, out.ctypes.data,
+                                          grad.ctypes.data, j10.ctypes.data))
+        iu = np.triu_indices(4)
+        jtj = np.zeros((n, 4, 4), np.float32)
+        jtj[:, iu[0], iu[1]] = j10
+        jtj[:, iu[1], iu[0]] = j10
+        return out, pr, grad, jtj
 
     def patchmatch_steps(self, ref, srcs, range0, range1, prm, first_step, n_steps, state=None):
         """Returns (state (h,w,5), textured, evals). first_step = -1 initialises."""

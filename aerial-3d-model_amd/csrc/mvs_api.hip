@@ -118,7 +118,7 @@ struct mvs_ctx {
     DevBuf dsm_xyz, dsm_h, dsm_a, dsm_b, dsm_c, dsm_d, dsm_snap, dsm_parent, dsm_la, dsm_lb,
         dsm_ctr, dsm_sel;  // surface model
     int64_t dsm_cells = 0;  // cells of the last grid built (kept in dsm_h)
-    DevBuf pairs, grad_io, gout;  // pair-cost lists
+    DevBuf pairs, grad_io, gout, jtj_out;  // pair-cost lists
     std::map<RayKey, DevBuf *> rays;
     // host copies that must outlive the asynchronous uploads of a call
     std::vector<RefJob> host_jobs;
@@ -1200,10 +1200,18 @@ int mvs_ncc_cost(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     return sync_ctx(c);
 }
 
-int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
-                 double range0, double range1, const mvs_params *prm, int64_t n,
-                 const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
-                 const float *grad_out, float *out_cost, float *grad4) {
+}  // extern "C"
+
+namespace {
+
+// The explicit (pixel, plane, pair) list calls: mvs_ncc_pair (terms = false:
+// the autograd-style pair kernel) and mvs_refine_terms (terms = true: the
+// refinement kernels' own Gauss-Newton terms).
+int pair_list_call(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                   double range0, double range1, const mvs_params *prm, int64_t n,
+                   const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                   const float *grad_out, float *out_cost, float *grad4, float *jtj10,
+                   bool terms) {
     if (!c) return MVS_ERR_ARG;
     TRY(check_params(c, prm));
     if (nsrc < 2) return fail(c, MVS_NO_NEIGHBOURS, "a source pair needs two neighbours");
@@ -1229,6 +1237,7 @@ int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     HIP_TRY(c, c->pairs.ensure((size_t)n * 8));
     HIP_TRY(c, c->grad_io.ensure((size_t)n * 16));
     HIP_TRY(c, c->gout.ensure((size_t)n * 4));
+    if (jtj10) HIP_TRY(c, c->jtj_out.ensure((size_t)n * 40));
     HIP_TRY(c, hipMemcpyAsync(c->px.p, px, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
     HIP_TRY(c, hipMemcpyAsync(c->py.p, py, (size_t)n * 4, hipMemcpyHostToDevice, c->st));
     HIP_TRY(c, hipMemcpyAsync(c->hyp_in.p, hyp4, (size_t)n * 16, hipMemcpyHostToDevice, c->st));
@@ -1245,10 +1254,23 @@ int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     jobs[0].prm = *prm;
     int rc = run_batch(c, jobs, 0, 0, nullptr, nullptr);
     if (rc) return abort_call(c, rc);
-    rc = launch_pair_list(prm->taps, c->host_jobs.back(), n, c->px.as<int32_t>(),
-                          c->py.as<int32_t>(), c->hyp_in.as<float4>(), c->pairs.as<int32_t>(),
-                          grad_out ? c->gout.as<float>() : nullptr, c->cost_out.as<float>(),
-                          grad4 ? c->grad_io.as<float4>() : nullptr, c->st);
+    const RefJob &job = c->host_jobs.back();
+    if (terms) {
+        rc = launch_refine_terms(prm->taps, job, n, c->px.as<int32_t>(), c->py.as<int32_t>(),
+                                 c->hyp_in.as<float4>(), c->pairs.as<int32_t>(),
+                                 c->cost_out.as<float>(),
+                                 grad4 ? c->grad_io.as<float4>() : nullptr,
+                                 jtj10 ? c->jtj_out.as<float>() : nullptr, c->st);
+        // the layout that scored the list: one lane per pixel (-1) or one
+        // lane per source of the pair (1)
+        c->stats.split = job.lane_ok ? -1 : 1;
+        c->stats.gpc = 1;
+    } else {
+        rc = launch_pair_list(prm->taps, job, n, c->px.as<int32_t>(), c->py.as<int32_t>(),
+                              c->hyp_in.as<float4>(), c->pairs.as<int32_t>(),
+                              grad_out ? c->gout.as<float>() : nullptr, c->cost_out.as<float>(),
+                              grad4 ? c->grad_io.as<float4>() : nullptr, c->st);
+    }
     if (rc) return abort_call(c, fail(c, MVS_ERR_ARG, "no pair kernel for this window"));
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(out_cost, c->cost_out.p, (size_t)n * 4, hipMemcpyDeviceToHost,
@@ -1257,7 +1279,30 @@ int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t 
     if (grad4)
         HIP_TRY(c, hipMemcpyAsync(grad4, c->grad_io.p, (size_t)n * 16, hipMemcpyDeviceToHost,
                                   c->st));
+    if (jtj10)
+        HIP_TRY(c, hipMemcpyAsync(jtj10, c->jtj_out.p, (size_t)n * 40, hipMemcpyDeviceToHost,
+                                  c->st));
     return sync_ctx(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvs_ncc_pair(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                 double range0, double range1, const mvs_params *prm, int64_t n,
+                 const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                 const float *grad_out, float *out_cost, float *grad4) {
+    return pair_list_call(c, ref, srcs, nsrc, range0, range1, prm, n, px, py, hyp4, pairs,
+                          grad_out, out_cost, grad4, nullptr, false);
+}
+
+int mvs_refine_terms(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                     double range0, double range1, const mvs_params *prm, int64_t n,
+                     const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                     float *out_cost, float *grad4, float *jtj10) {
+    return pair_list_call(c, ref, srcs, nsrc, range0, range1, prm, n, px, py, hyp4, pairs,
+                          nullptr, out_cost, grad4, jtj10, true);
 }
 
 int mvs_patchmatch_steps(mvs_ctx *c, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,

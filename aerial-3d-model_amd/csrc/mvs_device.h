@@ -170,6 +170,69 @@ __device__ __forceinline__ void project_general(const float q0, const float q1, 
     v = vy.hi + vy.lo;
 }
 
+// ---------------------------------------------------------------------------
+// refinement geometry (no reference counterpart)
+// ---------------------------------------------------------------------------
+
+// Tangent basis of a unit normal n: two unit directions orthogonal to n and
+// to each other, the axes of the refinement's two normal angles.
+__device__ __forceinline__ void tangent_dirs(const float n[3], float dirs[2][3]) {
+    float e1[3] = {1.0f - n[0] * n[0], -n[0] * n[1], -n[0] * n[2]};
+    float l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    if (!(l1 > 0.5f)) {
+        e1[0] = -n[1] * n[0];
+        e1[1] = 1.0f - n[1] * n[1];
+        e1[2] = -n[1] * n[2];
+        l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e1[c] /= l1;
+    dirs[0][0] = e1[0];
+    dirs[0][1] = e1[1];
+    dirs[0][2] = e1[2];
+    dirs[1][0] = n[1] * e1[2] - n[2] * e1[1];
+    dirs[1][1] = n[2] * e1[0] - n[0] * e1[2];
+    dirs[1][2] = n[0] * e1[1] - n[1] * e1[0];
+}
+
+// The refinement's Gauss-Newton terms in (d, n0, n1, n2). H (packed upper:
+// 00 01 02 11 12 22) and g are J^T J and J^T r of the pair's stacked
+// residuals w.r.t. theta = (depth, angle along dirs[0], angle along dirs[1]);
+// sum_s (1 - NCC_s) = |r|^2 / 2 is twice the pair cost. With B = [e_d, (0,
+// dirs[0]), (0, dirs[1])] (orthonormal columns) and the residuals invariant
+// under scaling n, J_(d,n) = J_theta B^T, so the cost gradient is B g / 2 and
+// J^T J is B H B^T (out: packed upper of the 4 x 4, row-major order 00 01 02
+// 03 11 12 13 22 23 33).
+__device__ __forceinline__ void terms_to_dn(const float H[6], const float g[3],
+                                            const float dirs[2][3], float grad[4],
+                                            float jtj[10]) {
+    const float M[3][3] = {{H[0], H[1], H[2]}, {H[1], H[3], H[4]}, {H[2], H[4], H[5]}};
+    float B[4][3];  // rows: d, n0, n1, n2; columns: theta
+    B[0][0] = 1.0f;
+    B[0][1] = 0.0f;
+    B[0][2] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        B[1 + c][0] = 0.0f;
+        B[1 + c][1] = dirs[0][c];
+        B[1 + c][2] = dirs[1][c];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) grad[a] = 0.5f * (B[a][0] * g[0] + B[a][1] * g[1] + B[a][2] * g[2]);
+    int m = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b, ++m) {
+            float v = 0.0f;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) v += B[a][p] * M[p][q] * B[b][q];
+            jtj[m] = v;
+        }
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;

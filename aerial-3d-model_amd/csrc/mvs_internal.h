@@ -214,6 +214,17 @@ constexpr size_t kDsmSelBytes = 4 * 2048 * 4 + 64;
 int dsm_extent(const float *d_xyz, int64_t n, uint32_t rank_lo, uint32_t rank_hi, void *d_sel,
                hipStream_t st, float out[4]);
 
+// The refinement's Gauss-Newton terms for an explicit (pixel, plane, pair)
+// list, by the kernel launch_refine would use for the job (one lane per pixel
+// when lane_ok): cost, cost gradient w.r.t. (d, n0, n1, n2) and the 4 x 4
+// J^T J of the pair's stacked residuals (packed upper, 10 per item); grad and
+// jtj nullable. 0, or -1 for an unsupported window.
+int launch_refine_terms(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
+                        const int32_t *py, const float4 *hyp, int32_t *pairs, float *cost,
+                        float4 *grad, float *jtj, hipStream_t st);
+int launch_terms_lane(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
+                      const int32_t *py, const float4 *hyp, int32_t *pairs, float *cost,
+                      float4 *grad, float *jtj, hipStream_t st);
 // Fixed-pair cost and gradient of an explicit (pixel, plane, pair) list.
 int launch_pair_list(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
                      const int32_t *py, const float4 *hyp, int32_t *pairs, const float *gout,

@@ -301,6 +301,78 @@ __global__ __launch_bounds__(64) void k_pair_list(RefJob job, int64_t n, const i
     }
 }
 
+// This lane's source terms at the plane (d, n), summed with its partner
+// lane's (the other source of the pair): J^T J (packed upper) and J^T r of the
+// pair's stacked residuals w.r.t. depth and the angles along dirs, and the
+// pair cost. Both lanes of a pair must call it. true: both sources defined.
+template <int TAPS, bool PINHOLE>
+__device__ bool pair_gn(const RefJob &job, const SrcCam &S, const PixelRef &pr, bool live, int x,
+                        int y, float d, const float n[3], const float dirs[2][3], float H[6],
+                        float g[3], float &cost) {
+    Terms<3> T;
+    if (live) T = source_terms<TAPS, PINHOLE, 3, true>(job, S, pr, x, y, d, n, dirs);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) H[q] = T.ok ? T.DD[q] / (T.nb * T.nb) : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        g[q] = T.ok ? -(T.Dta[q] / pr.na - T.ncc * T.Dtb[q] / T.nb) / T.nb : 0.0f;
+    const float ncc = T.ok ? T.ncc : 0.0f;
+    const bool ok_o = __shfl_xor((int)T.ok, 1) != 0;
+    const float ncc_o = __shfl_xor(ncc, 1);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) H[q] += __shfl_xor(H[q], 1);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g[q] += __shfl_xor(g[q], 1);
+    cost = 1.0f - 0.5f * (ncc + ncc_o);
+    return T.ok && ok_o;
+}
+
+// The Gauss-Newton terms k_refine forms, for an explicit list of (pixel,
+// plane, source pair) items: two lanes per item, the refinement's own pair
+// search and terms, mapped to (d, n0, n1, n2). pairs: an item given (-1, -1)
+// gets its best pair at that plane, written back. Undefined items: cost 2,
+// gradient and J^T J 0.
+template <int TAPS, bool PINHOLE>
+__global__ __launch_bounds__(64) void k_terms_pair(RefJob job, int64_t n, const int32_t *px,
+                                                   const int32_t *py, const float4 *hyp,
+                                                   int32_t *pairs, float *cost, float4 *grad,
+                                                   float *jtj) {
+    const int lane = threadIdx.x, side = lane & 1;
+    const int64_t item = (int64_t)blockIdx.x * 32 + (lane >> 1);
+    const bool have = item < n;
+    const int x = have ? px[item] : 0, y = have ? py[item] : 0;
+    const float4 hp = have ? hyp[item] : make_float4(1, 0, 0, -1);
+    const float nn[3] = {hp.y, hp.z, hp.w};
+    PixelRef pr;
+    if (have) pr = pixel_ref<TAPS>(job, x, y);
+    int2 pq = have ? make_int2(pairs[2 * item], pairs[2 * item + 1]) : make_int2(0, 0);
+    const bool choose = pq.x < 0;  // the same for both lanes of an item
+    const int2 best =
+        best_pair<TAPS, PINHOLE>(job, pr, have && pr.ok && choose, x, y, hp.x, nn, side);
+    if (choose) pq = best;
+    const int s = side ? pq.y : pq.x;
+    const bool live = have && pr.ok && s >= 0 && s < job.nsrc && pq.x != pq.y;
+    float dirs[2][3];
+    tangent_dirs(nn, dirs);
+    float H[6], g[3], c;
+    const bool ok = pair_gn<TAPS, PINHOLE>(job, job.srcs[live ? s : 0], pr, live, x, y, hp.x, nn,
+                                           dirs, H, g, c);
+    if (!have || side != 0) return;
+    if (choose) {
+        pairs[2 * item] = pq.x;
+        pairs[2 * item + 1] = pq.y;
+    }
+    float g4[4] = {0, 0, 0, 0}, h10[10];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) h10[m] = 0.0f;
+    if (ok) terms_to_dn(H, g, dirs, g4, h10);
+    cost[item] = ok ? c : 2.0f;
+    if (grad) grad[item] = make_float4(g4[0], g4[1], g4[2], g4[3]);
+    if (jtj)
+#pragma unroll
+        for (int m = 0; m < 10; ++m) jtj[10 * item + m] = h10[m];
+}
+
 // Solve the symmetric positive 3x3 system H x = b (H packed upper) by
 // cofactors.
 __device__ bool solve3(const float H[6], const float b[3], float x[3]) {
@@ -350,39 +422,11 @@ __global__ __launch_bounds__(64) void k_refine(RefJob job, int32_t steps,
     float lambda = 1e-3f;
     unsigned long long acc = 0;
     for (int it = 0; it < steps; ++it) {
-        // tangent basis of the normal
-        float e1[3] = {1.0f - nrm[0] * nrm[0], -nrm[0] * nrm[1], -nrm[0] * nrm[2]};
-        float l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        if (!(l1 > 0.5f)) {
-            e1[0] = -nrm[1] * nrm[0];
-            e1[1] = 1.0f - nrm[1] * nrm[1];
-            e1[2] = -nrm[1] * nrm[2];
-            l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        }
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3) e1[c3] /= l1;
-        const float dirs[2][3] = {
-            {e1[0], e1[1], e1[2]},
-            {nrm[1] * e1[2] - nrm[2] * e1[1], nrm[2] * e1[0] - nrm[0] * e1[2],
-             nrm[0] * e1[1] - nrm[1] * e1[0]}};
-        Terms<3> T;
-        if (live) T = source_terms<TAPS, PINHOLE, 3, true>(job, S, pr, x, y, hp.x, nrm, dirs);
+        float dirs[2][3];
+        tangent_dirs(nrm, dirs);
         // sum of J^T J and of the NCC gradients over the pair
-        float H[6], g[3];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) H[q] = T.ok ? T.DD[q] / (T.nb * T.nb) : 0.0f;
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            g[q] = T.ok ? -(T.Dta[q] / pr.na - T.ncc * T.Dtb[q] / T.nb) / T.nb : 0.0f;
-        const float ncc = T.ok ? T.ncc : 0.0f;
-        const bool ok_o = __shfl_xor((int)T.ok, 1) != 0;
-        const float ncc_o = __shfl_xor(ncc, 1);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) H[q] += __shfl_xor(H[q], 1);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) g[q] += __shfl_xor(g[q], 1);
-        live = live && T.ok && ok_o;
-        const float cost0 = 1.0f - 0.5f * (ncc + ncc_o);
+        float H[6], g[3], cost0;
+        live = pair_gn<TAPS, PINHOLE>(job, S, pr, live, x, y, hp.x, nrm, dirs, H, g, cost0);
         const float Hd[6] = {H[0] * (1.0f + lambda), H[1], H[2], H[3] * (1.0f + lambda), H[4],
                              H[5] * (1.0f + lambda)};
         const float rhs[3] = {-g[0], -g[1], -g[2]};
@@ -466,7 +510,33 @@ void pair_list_taps(const RefJob &job, int64_t n, const int32_t *px, const int32
                            py, hyp, pairs, gout, cost, grad);
 }
 
+template <int TAPS>
+void terms_pair_taps(const RefJob &job, int64_t n, const int32_t *px, const int32_t *py,
+                     const float4 *hyp, int32_t *pairs, float *cost, float4 *grad, float *jtj,
+                     hipStream_t st) {
+    const unsigned blocks = (unsigned)((n + 31) / 32);
+    if (blocks == 0) return;
+    if (job.pinhole)
+        hipLaunchKernelGGL((k_terms_pair<TAPS, true>), dim3(blocks), dim3(64), 0, st, job, n, px,
+                           py, hyp, pairs, cost, grad, jtj);
+    else
+        hipLaunchKernelGGL((k_terms_pair<TAPS, false>), dim3(blocks), dim3(64), 0, st, job, n, px,
+                           py, hyp, pairs, cost, grad, jtj);
+}
+
 }  // namespace
+
+int launch_refine_terms(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
+                        const int32_t *py, const float4 *hyp, int32_t *pairs, float *cost,
+                        float4 *grad, float *jtj, hipStream_t st) {
+    if (job.lane_ok) return launch_terms_lane(taps, job, n, px, py, hyp, pairs, cost, grad, jtj, st);
+    switch (taps) {
+        case 5: terms_pair_taps<5>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        case 7: terms_pair_taps<7>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        case 11: terms_pair_taps<11>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        default: return -1;
+    }
+}
 
 int launch_refine(int32_t taps, const RefJob &job, int32_t steps, unsigned long long *counters,
                   hipStream_t st) {

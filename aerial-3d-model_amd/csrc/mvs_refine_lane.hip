@@ -221,13 +221,101 @@ __device__ bool lane_solve3(const float H[6], const float b[3], float x[3]) {
     return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
 }
 
+// This lane's reference window at pixel pix: the bytes b = rint(255 g) of the
+// 8-bit view, a = b - mi into LDS as f16 integers (exact), the correction
+// delta = mean(b) - mi, |a| and the centre ray. The caller checks that the
+// window lies inside the view.
+template <int TAPS>
+__device__ LaneRef lane_ref(const RefJob &job, int32_t pix, _Float16 (*Ah)[64], int lane) {
+    constexpr int N = TAPS * TAPS, H = TAPS / 2;
+    LaneRef pr;
+    const int st = job.stride, w = job.w;
+    float sb = 0.0f;
+#pragma unroll 1
+    for (int ty = 0; ty < TAPS; ++ty) {
+        const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
+#pragma unroll
+        for (int tx = 0; tx < TAPS; ++tx) sb += rintf(row[(tx - H) * st] * 255.0f);
+    }
+    const float mi = rintf(sb * (1.0f / (float)N));
+    float s2 = 0.0f;
+#pragma unroll 1
+    for (int ty = 0; ty < TAPS; ++ty) {
+        const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
+#pragma unroll
+        for (int tx = 0; tx < TAPS; ++tx) {
+            const float av = rintf(row[(tx - H) * st] * 255.0f) - mi;
+            Ah[ty * TAPS + tx][lane] = (_Float16)av;
+            s2 = fmaf(av, av, s2);
+        }
+    }
+    pr.delta = (sb - (float)N * mi) * (1.0f / (float)N);
+    // |a|^2 = sum (b - mi)^2 - N delta^2
+    pr.na = sqrtf(fmaxf(fmaf(-(float)N * pr.delta, pr.delta, s2), 0.0f));
+    const float2 rc = job.rays[pix];
+    pr.rcx = rc.x;
+    pr.rcy = rc.y;
+    pr.ok = pr.na > 255e-6f;
+    return pr;
+}
+
+// The two sources best at the plane (d, n) (highest NCC first, ties to the
+// lower index, dense.rs:255-256), every source in step across the wavefront;
+// i1 >= nsrc when fewer than two are defined.
+template <int TAPS>
+__device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, float rstep,
+                               float d, const float n[3], const _Float16 (*Ah)[64], int lane,
+                               bool search, int &i0, int &i1) {
+    float b0 = -INFINITY, b1 = -INFINITY;
+    i0 = 1 << 20;
+    i1 = 1 << 20;
+    for (int k = 0; k < nsrc; ++k) {
+        if (search) {
+            const LTerms<0> T = lane_terms<TAPS, 0>(Ss[k], pr, rstep, d, n, nullptr, Ah, lane);
+            if (T.ok) {
+                if (lane_before(T.ncc, k, b0, i0)) {
+                    b1 = b0;
+                    i1 = i0;
+                    b0 = T.ncc;
+                    i0 = k;
+                } else if (lane_before(T.ncc, k, b1, i1)) {
+                    b1 = T.ncc;
+                    i1 = k;
+                }
+            }
+        }
+    }
+}
+
+// Gauss-Newton terms of the pair (A, B) at the plane (d, n): J^T J (packed
+// upper, w.r.t. depth and the angles along dirs) and J^T r of the stacked
+// residuals, and the pair cost 1 - (NCC_A + NCC_B) / 2. false where either
+// source is undefined.
+template <int TAPS>
+__device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr, float rstep,
+                             float d, const float n[3], const float dirs[2][3],
+                             const _Float16 (*Ah)[64], int lane, float Hm[6], float g[3],
+                             float &cost) {
+    const LTerms<3> TA = lane_terms<TAPS, 3>(A, pr, rstep, d, n, dirs, Ah, lane);
+    const LTerms<3> TB = lane_terms<TAPS, 3>(B, pr, rstep, d, n, dirs, Ah, lane);
+    if (!(TA.ok && TB.ok)) return false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        g[q] = -(TA.Dta[q] / pr.na - TA.ncc * TA.Dtb[q] / TA.nb) / TA.nb -
+               (TB.Dta[q] / pr.na - TB.ncc * TB.Dtb[q] / TB.nb) / TB.nb;
+    cost = 1.0f - 0.5f * (TA.ncc + TB.ncc);
+    return true;
+}
+
 // One lane per pixel of the job's textured-pixel lists (64 per wavefront);
 // pixels whose PatchMatch cost is below cost_max are refined in place.
 // counters (nullable): [0] += accepted steps, [1] += pixels refined.
 template <int TAPS>
 __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
                                                     unsigned long long *counters) {
-    constexpr int N = TAPS * TAPS, H = TAPS / 2;
+    constexpr int N = TAPS * TAPS;
     __shared__ _Float16 Ah[N][64];
     __shared__ SrcCam Ss[kMaxSrc];
     const int lane = threadIdx.x;
@@ -248,60 +336,14 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     float4 hp = have ? job.hyp[pix] : make_float4(1, 0, 0, -1);
     have = have && job.cost[pix] < job.cost_max && x >= job.half && y >= job.half &&
            x < w - job.half && y < job.h - job.half;
-    // reference window: bytes b = rint(255 g) (8-bit views), a = b - mi in LDS
     LaneRef pr;
-    if (have) {
-        const int st = job.stride;
-        float sb = 0.0f;
-#pragma unroll 1
-        for (int ty = 0; ty < TAPS; ++ty) {
-            const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
-#pragma unroll
-            for (int tx = 0; tx < TAPS; ++tx) sb += rintf(row[(tx - H) * st] * 255.0f);
-        }
-        const float mi = rintf(sb * (1.0f / (float)N));
-        float s2 = 0.0f;
-#pragma unroll 1
-        for (int ty = 0; ty < TAPS; ++ty) {
-            const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
-#pragma unroll
-            for (int tx = 0; tx < TAPS; ++tx) {
-                const float av = rintf(row[(tx - H) * st] * 255.0f) - mi;
-                Ah[ty * TAPS + tx][lane] = (_Float16)av;
-                s2 = fmaf(av, av, s2);
-            }
-        }
-        pr.delta = (sb - (float)N * mi) * (1.0f / (float)N);
-        // |a|^2 = sum (b - mi)^2 - N delta^2
-        pr.na = sqrtf(fmaxf(fmaf(-(float)N * pr.delta, pr.delta, s2), 0.0f));
-        const float2 rc = job.rays[pix];
-        pr.rcx = rc.x;
-        pr.rcy = rc.y;
-        pr.ok = pr.na > 255e-6f;
-    }
+    if (have) pr = lane_ref<TAPS>(job, pix, Ah, lane);
     const float rstep = job.ray_step;
     float nrm[3] = {hp.y, hp.z, hp.w};
     // best pair at the PatchMatch plane: every source in step
-    float b0 = -INFINITY, b1 = -INFINITY;
-    int i0 = 1 << 20, i1 = 1 << 20;
+    int i0, i1;
     const bool search = have && pr.ok;
-    for (int k = 0; k < job.nsrc; ++k) {
-        if (search) {
-            const LTerms<0> T =
-                lane_terms<TAPS, 0>(Ss[k], pr, rstep, hp.x, nrm, nullptr, Ah, lane);
-            if (T.ok) {
-                if (lane_before(T.ncc, k, b0, i0)) {
-                    b1 = b0;
-                    i1 = i0;
-                    b0 = T.ncc;
-                    i0 = k;
-                } else if (lane_before(T.ncc, k, b1, i1)) {
-                    b1 = T.ncc;
-                    i1 = k;
-                }
-            }
-        }
-    }
+    lane_best_pair<TAPS>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, search, i0, i1);
     bool live = search && i1 < job.nsrc;
     const bool refined = live;
     const int p0 = live ? i0 : 0, p1 = live ? i1 : 0;
@@ -314,37 +356,13 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     bool have_terms = false;
     for (int it = 0; it < steps; ++it) {
         if (!live) break;
-        // tangent basis of the normal
-        float e1[3] = {1.0f - nrm[0] * nrm[0], -nrm[0] * nrm[1], -nrm[0] * nrm[2]};
-        float l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        if (!(l1 > 0.5f)) {
-            e1[0] = -nrm[1] * nrm[0];
-            e1[1] = 1.0f - nrm[1] * nrm[1];
-            e1[2] = -nrm[1] * nrm[2];
-            l1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
-        }
-#pragma unroll
-        for (int c3 = 0; c3 < 3; ++c3) e1[c3] /= l1;
-        const float dirs[2][3] = {
-            {e1[0], e1[1], e1[2]},
-            {nrm[1] * e1[2] - nrm[2] * e1[1], nrm[2] * e1[0] - nrm[0] * e1[2],
-             nrm[0] * e1[1] - nrm[1] * e1[0]}};
+        float dirs[2][3];
+        tangent_dirs(nrm, dirs);
         if (!have_terms) {
-            const LTerms<3> TA =
-                lane_terms<TAPS, 3>(Ss[p0], pr, rstep, hp.x, nrm, dirs, Ah, lane);
-            const LTerms<3> TB =
-                lane_terms<TAPS, 3>(Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane);
-            live = TA.ok && TB.ok;
+            // J^T J, J^T r and the cost of the pair at the current plane
+            live = lane_pair_gn<TAPS>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g,
+                                      cost0);
             if (!live) break;
-            // J^T J and the NCC gradient summed over the pair
-#pragma unroll
-            for (int q = 0; q < 6; ++q)
-                Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                g[q] = -(TA.Dta[q] / pr.na - TA.ncc * TA.Dtb[q] / TA.nb) / TA.nb -
-                       (TB.Dta[q] / pr.na - TB.ncc * TB.Dtb[q] / TB.nb) / TB.nb;
-            cost0 = 1.0f - 0.5f * (TA.ncc + TB.ncc);
             have_terms = true;
         }
         const float Hd[6] = {Hm[0] * (1.0f + lambda), Hm[1], Hm[2], Hm[3] * (1.0f + lambda),
@@ -400,6 +418,80 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     }
 }
 
+// The Gauss-Newton terms k_refine_lane forms, for an explicit list of (pixel,
+// plane, source pair) items, one lane per item: the refinement's own window,
+// pair search and terms, mapped to (d, n0, n1, n2). pairs: an item given
+// (-1, -1) gets its best pair at that plane, written back. Undefined items:
+// cost 2, gradient and J^T J 0.
+template <int TAPS>
+__global__ __launch_bounds__(64) void k_terms_lane(RefJob job, int64_t n,
+                                                   const int32_t *__restrict__ px,
+                                                   const int32_t *__restrict__ py,
+                                                   const float4 *__restrict__ hyp, int32_t *pairs,
+                                                   float *cost, float4 *grad, float *jtj) {
+    constexpr int N = TAPS * TAPS;
+    __shared__ _Float16 Ah[N][64];
+    __shared__ SrcCam Ss[kMaxSrc];
+    const int lane = threadIdx.x;
+    {
+        const int words = job.nsrc * (int)(sizeof(SrcCam) / 4);
+        const uint32_t *src = (const uint32_t *)job.srcs;
+        uint32_t *dst = (uint32_t *)Ss;
+        for (int i = lane; i < words; i += 64) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int64_t item = (int64_t)blockIdx.x * 64 + lane;
+    const bool inlist = item < n;
+    const int x = inlist ? px[item] : 0, y = inlist ? py[item] : 0;
+    const float4 hp = inlist ? hyp[item] : make_float4(1, 0, 0, -1);
+    const bool have = inlist && x >= job.half && y >= job.half && x < job.w - job.half &&
+                      y < job.h - job.half;
+    const int32_t pix = y * job.w + x;
+    LaneRef pr;
+    if (have) pr = lane_ref<TAPS>(job, pix, Ah, lane);
+    const float rstep = job.ray_step;
+    const float nrm[3] = {hp.y, hp.z, hp.w};
+    int p0 = inlist ? pairs[2 * item] : 0, p1 = inlist ? pairs[2 * item + 1] : 0;
+    const bool choose = p0 < 0;
+    int i0, i1;
+    lane_best_pair<TAPS>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, have && pr.ok && choose,
+                         i0, i1);
+    if (choose) {
+        p0 = i1 < job.nsrc ? i0 : -1;
+        p1 = i1 < job.nsrc ? i1 : -1;
+    }
+    float Hm[6], g[3], c = 2.0f;
+    float dirs[2][3];
+    tangent_dirs(nrm, dirs);
+    bool ok = false;
+    if (have && pr.ok && p0 >= 0 && p1 >= 0 && p0 != p1)
+        ok = lane_pair_gn<TAPS>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g, c);
+    if (!inlist) return;
+    if (choose) {
+        pairs[2 * item] = p0;
+        pairs[2 * item + 1] = p1;
+    }
+    float g4[4] = {0, 0, 0, 0}, h10[10];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) h10[m] = 0.0f;
+    if (ok) terms_to_dn(Hm, g, dirs, g4, h10);
+    cost[item] = ok ? c : 2.0f;
+    if (grad) grad[item] = make_float4(g4[0], g4[1], g4[2], g4[3]);
+    if (jtj)
+#pragma unroll
+        for (int m = 0; m < 10; ++m) jtj[10 * item + m] = h10[m];
+}
+
+template <int TAPS>
+void terms_lane_taps(const RefJob &job, int64_t n, const int32_t *px, const int32_t *py,
+                     const float4 *hyp, int32_t *pairs, float *cost, float4 *grad, float *jtj,
+                     hipStream_t st) {
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((k_terms_lane<TAPS>), dim3(blocks), dim3(64), 0, st, job, n, px, py, hyp,
+                       pairs, cost, grad, jtj);
+}
+
 template <int TAPS>
 void refine_lane_taps(const RefJob &job, int32_t steps, unsigned long long *counters,
                       hipStream_t st) {
@@ -409,6 +501,18 @@ void refine_lane_taps(const RefJob &job, int32_t steps, unsigned long long *coun
 }
 
 }  // namespace
+
+int launch_terms_lane(int32_t taps, const RefJob &job, int64_t n, const int32_t *px,
+                      const int32_t *py, const float4 *hyp, int32_t *pairs, float *cost,
+                      float4 *grad, float *jtj, hipStream_t st) {
+    if (job.nsrc > kMaxSrc) return -1;
+    switch (taps) {
+        case 5: terms_lane_taps<5>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        case 7: terms_lane_taps<7>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        case 11: terms_lane_taps<11>(job, n, px, py, hyp, pairs, cost, grad, jtj, st); return 0;
+        default: return -1;
+    }
+}
 
 int launch_refine_lane(int32_t taps, const RefJob &job, int32_t steps,
                        unsigned long long *counters, hipStream_t st) {

@@ -222,16 +222,18 @@ def cpu_baseline_dsm(xyz, voxel, got):
 
 
 def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
-    """CPU-baseline leg: the backward kernel at full size. At n_items random
-    valid pixels of the refined maps (spread over the views), the refined
-    plane perturbed by up to 1% in depth and ~0.05 rad in normal (at the
-    refined plane itself the gradient is near zero: it is a Gauss-Newton
-    minimum, where a relative error says nothing), the GPU's fixed-pair cost
-    and analytic gradient w.r.t. (depth, normal) at the pair it selects,
-    against the oracle's f64 pair cost and f64 central finite differences
-    (1e-4 m in depth, 1e-5 per normal component). Items whose bilinear cells
-    change within a difference step are skipped (the cost is not
-    differentiable there)."""
+    """CPU-baseline leg: the refinement's backward kernel at full size. At
+    n_items random valid pixels of the refined maps (spread over the views),
+    the refined plane perturbed by up to 1% in depth and ~0.05 rad in normal
+    (at the refined plane itself the gradient is near zero: it is a
+    Gauss-Newton minimum, where a relative error says nothing), the
+    Gauss-Newton terms the refinement kernel itself forms (mvs_refine_terms:
+    the one-lane-per-pixel kernel on these 8-bit views) at the pair it
+    selects: the pair cost against the oracle's f64 pair cost, the gradient
+    w.r.t. (depth, normal) against f64 central finite differences (1e-4 m in
+    depth, 1e-5 per normal component) and J^T J against the oracle's f64
+    J^T J. Items whose bilinear cells change within a difference step are
+    skipped (the cost is not differentiable there)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
     from aerial_mvs import capi
@@ -239,8 +241,9 @@ def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
     prm = capi.params(taps=spec.taps, stride=spec.stride)
     per = max(1, n_items // len(jobs))
     hs = (1e-4, 1e-5, 1e-5, 1e-5)
-    rels, dcost = [], []
+    rels, dcost, jrels = [], [], []
     tried = 0
+    kernels = set()
     tc = time.perf_counter()
     for (i, nb, rg) in jobs:
         v = surv.views[i]
@@ -257,7 +260,9 @@ def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
         nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
         dep = d[ys, xs] * rng.uniform(0.99, 1.01, len(xs))
         hy = np.concatenate([dep[:, None], nrm], axis=1).astype(np.float32)
-        cost, pq, grad = ctx.ncc_pair(v, srcs, rg[0], rg[1], prm, xs, ys, hy)
+        cost, pq, grad, jtj = ctx.refine_terms(v, srcs, rg[0], rg[1], prm, xs, ys, hy)
+        kernels.add("k_refine_lane (one lane per pixel)" if ctx.stats()["split"] == -1
+                    else "k_refine (one lane per source)")
         o = orc.OracleDepth(v, srcs, rg[0], rg[1], taps=spec.taps, stride=spec.stride)
         for k in range(len(xs)):
             if cost[k] >= 2.0:
@@ -265,10 +270,16 @@ def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
             tried += 1
             h64 = hy[k].astype(np.float64)
             s0, s1 = int(pq[k][0]), int(pq[k][1])
-            base = o.pair_cost64(xs[k], ys[k], h64, s0, s1)
+            base = o.pair_terms64(xs[k], ys[k], h64, s0, s1)
             if base is None:
                 continue
             dcost.append(abs(float(cost[k]) - base[0]))
+            # J^T J against the oracle's f64 matrix, entries weighed like the
+            # gradient (normal rows and columns scaled by 1/depth)
+            sc = np.array([1.0, 1 / h64[0], 1 / h64[0], 1 / h64[0]])
+            S = np.outer(sc, sc)
+            jrels.append(np.abs((jtj[k].astype(np.float64) - base[2]) * S).max()
+                         / max(np.abs(base[2] * S).max(), 1e-30))
             fd = np.zeros(4)
             ok = True
             for a in range(4):
@@ -277,7 +288,7 @@ def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
                 hm[a] -= hs[a]
                 p_ = o.pair_cost64(xs[k], ys[k], hp, s0, s1)
                 m_ = o.pair_cost64(xs[k], ys[k], hm, s0, s1)
-                if p_ is None or m_ is None or p_[2] != base[2] or m_[2] != base[2]:
+                if p_ is None or m_ is None or p_[2] != base[3] or m_[2] != base[3]:
                     ok = False
                     break
                 fd[a] = (p_[0] - m_[0]) / (2 * hs[a])
@@ -295,15 +306,20 @@ def gradient_check(ctx, surv, jobs, spec, sv, n_items, seed=0xA3D0C4):
     out = dict(items=int(tried), checked=int(len(rels)), seconds=time.perf_counter() - tc,
                hypotheses="refined plane, depth x U(0.99, 1.01), normal + N(0, 0.05)",
                fd_steps="1e-4 m depth, 1e-5 per normal component",
-               bars="gradient median < 2e-4, p99 < 3e-3; cost p99 < 3e-5")
+               kernel=", ".join(sorted(kernels)),
+               bars="gradient and J^T J median < 2e-4, p99 < 3e-3; cost p99 < 3e-5")
     if len(rels) and len(dcost):
         out.update(grad_rel_err_median=float(np.median(rels)),
                    grad_rel_err_p99=float(np.quantile(rels, 0.99)),
                    grad_rel_err_max=float(rels.max()),
                    cost_abs_diff_median=float(np.median(dcost)),
                    cost_abs_diff_p99=float(np.quantile(dcost, 0.99)),
-                   cost_abs_diff_max=float(dcost.max()))
+                   cost_abs_diff_max=float(dcost.max()),
+                   jtj_rel_err_median=float(np.median(jrels)),
+                   jtj_rel_err_p99=float(np.quantile(jrels, 0.99)))
         out["pass"] = bool(out["grad_rel_err_median"] < 2e-4 and out["grad_rel_err_p99"] < 3e-3
+                           and out["jtj_rel_err_median"] < 2e-4
+                           and out["jtj_rel_err_p99"] < 3e-3
                            and out["cost_abs_diff_p99"] < 3e-5)
     return out
 

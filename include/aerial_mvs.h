@@ -149,6 +149,25 @@ int mvs_ncc_pair(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_
                  const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
                  const float *grad_out, float *out_cost, float *grad4);
 
+/* The Gauss-Newton terms of the refinement (no reference counterpart; the
+ * cost is Ctx2::cost's best-two average, dense.rs:255-262, with the two
+ * sources given) for explicit items, computed by the device code the
+ * refinement itself runs on this view class: one lane per pixel for 8-bit
+ * views on undistorted cameras, one lane per source otherwise (mvs_get_stats
+ * reports which as split -1 / 1). Items and pairs as mvs_ncc_pair. Per item:
+ * out_cost = 1 - (NCC_s0 + NCC_s1) / 2; grad4 (nullable, n x 4) = d cost /
+ * d(d, n0, n1, n2); jtj10 (nullable, n x 10) = J^T J of the pair's stacked
+ * residuals r_s = a/|a| - b_s/|b_s| (a, b_s the centred reference and sample
+ * windows, 1 - NCC_s = |r_s|^2 / 2) w.r.t. (d, n0, n1, n2), packed upper
+ * row-major (00 01 02 03 11 12 13 22 23 33): the matrix the Gauss-Newton
+ * step solves with. The kernels work in depth and two tangent angles of the
+ * normal; both outputs are mapped to (d, n) through that basis (the residuals
+ * do not change along n). Undefined items: cost 2, zeros. */
+int mvs_refine_terms(mvs_ctx *ctx, const mvs_view *ref, const mvs_view *srcs, int32_t nsrc,
+                     double range0, double range1, const mvs_params *prm, int64_t n,
+                     const int32_t *px, const int32_t *py, const float *hyp4, int32_t *pairs,
+                     float *out_cost, float *grad4, float *jtj10);
+
 /* Run PatchMatch in steps on a host state of w*h x {d, n0, n1, n2, cost}
  * (Hyp, dense.rs:156-161). first_step = -1 runs the initialisation
  * (dense.rs:365-402) before the sweeps; half-sweep s covers iteration s/2,

@@ -744,103 +744,177 @@ static int gray_at64(const float *g, int32_t w, int32_t h, double x, double y, d
     return 1;
 }
 
-int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
-                    int32_t s1, double *cost, double grad[4], uint64_t *cells) {
+/* The window of pixel (x, y) under plane hyp = (d, n0, n1, n2): reference
+ * grey, plane depth dq and its derivatives d dq / d(d, n0, n1, n2) per tap,
+ * and the tap rays. 0 where the window leaves the image or the plane is
+ * degenerate. */
+typedef struct {
+    double refv[ORC_MAX_TAPS], dq[ORC_MAX_TAPS], rk[ORC_MAX_TAPS][3];
+    double ddq[ORC_MAX_TAPS][4];
+    double mr, na;
+} pair_window64;
+
+static int pair_window(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4],
+                       pair_window64 *pw) {
     const int32_t w = c->rf.w, h = c->rf.h, N = c->n;
-    if (s0 < 0 || s1 < 0 || s0 >= c->nsrc || s1 >= c->nsrc) return 0;
     const double d = hyp[0], n[3] = {hyp[1], hyp[2], hyp[3]};
     const float *rc = c->rays + 2 * ((size_t)y * (size_t)w + (size_t)x);
     const double ch[3] = {rc[0], rc[1], 1.0};
     const double nc = n[0] * ch[0] + n[1] * ch[1] + n[2] * ch[2];
     const double cc = d * nc;
-    double refv[ORC_MAX_TAPS], dq[ORC_MAX_TAPS], rk[ORC_MAX_TAPS][3];
-    double ddq[ORC_MAX_TAPS][4]; /* d dq / d(d, n0, n1, n2) */
     for (int32_t k = 0; k < N; ++k) {
         int32_t qx = x + c->wdx[k], qy = y + c->wdy[k];
         if (qx < 0 || qy < 0 || qx >= w || qy >= h) return 0;
         size_t qi = (size_t)qy * (size_t)w + (size_t)qx;
-        refv[k] = c->rf.gray[qi];
-        rk[k][0] = c->rays[2 * qi];
-        rk[k][1] = c->rays[2 * qi + 1];
-        rk[k][2] = 1.0;
-        double den = n[0] * rk[k][0] + n[1] * rk[k][1] + n[2];
+        pw->refv[k] = c->rf.gray[qi];
+        pw->rk[k][0] = c->rays[2 * qi];
+        pw->rk[k][1] = c->rays[2 * qi + 1];
+        pw->rk[k][2] = 1.0;
+        double den = n[0] * pw->rk[k][0] + n[1] * pw->rk[k][1] + n[2];
         if (fabs(den) < 1e-6) return 0;
-        dq[k] = cc / den;
-        if (dq[k] <= 0.0) return 0;
-        ddq[k][0] = nc / den;
-        for (int a = 0; a < 3; ++a) ddq[k][1 + a] = (d * ch[a] - dq[k] * rk[k][a]) / den;
+        pw->dq[k] = cc / den;
+        if (pw->dq[k] <= 0.0) return 0;
+        pw->ddq[k][0] = nc / den;
+        for (int a = 0; a < 3; ++a) pw->ddq[k][1 + a] = (d * ch[a] - pw->dq[k] * pw->rk[k][a]) / den;
     }
     double mr = 0.0;
-    for (int32_t k = 0; k < N; ++k) mr += refv[k];
+    for (int32_t k = 0; k < N; ++k) mr += pw->refv[k];
     mr /= N;
     double na = 0.0;
-    for (int32_t k = 0; k < N; ++k) na += (refv[k] - mr) * (refv[k] - mr);
-    na = sqrt(na);
-    if (na < 1e-12) return 0;
+    for (int32_t k = 0; k < N; ++k) na += (pw->refv[k] - mr) * (pw->refv[k] - mr);
+    pw->mr = mr;
+    pw->na = sqrt(na);
+    return pw->na >= 1e-12;
+}
+
+/* Samples of source si over the window and their derivatives w.r.t.
+ * (d, n0, n1, n2): the projective warp, the radial model (geometry.rs:26-32)
+ * and the bilinear grey, all in f64. 0 where a sample fails. */
+static int pair_source(const orc_ctx *c, const pair_window64 *pw, int32_t si, double *sv,
+                       double (*dsv)[4], uint64_t *cell) {
+    const int32_t N = c->n;
+    const orc_view *v = &c->srcs[si];
+    double Rr[9], tt[3];
+    /* relative pose in f64 (the f32 cast is the kernels' business) */
+    {
+        const double *Rr_ = c->rf.R, *Cr = c->rf.C, *Rs = v->R, *Cs = v->C;
+        double dc[3] = {Cr[0] - Cs[0], Cr[1] - Cs[1], Cr[2] - Cs[2]};
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j)
+                Rr[i * 3 + j] = Rs[i * 3] * Rr_[j * 3] + Rs[i * 3 + 1] * Rr_[j * 3 + 1] +
+                                Rs[i * 3 + 2] * Rr_[j * 3 + 2];
+            tt[i] = Rs[i * 3] * dc[0] + Rs[i * 3 + 1] * dc[1] + Rs[i * 3 + 2] * dc[2];
+        }
+    }
+    for (int32_t k = 0; k < N; ++k) {
+        const double *rk = pw->rk[k];
+        double p[3] = {pw->dq[k] * rk[0], pw->dq[k] * rk[1], pw->dq[k]};
+        double xs[3], wk[3];
+        for (int i = 0; i < 3; ++i) {
+            xs[i] = Rr[i * 3] * p[0] + Rr[i * 3 + 1] * p[1] + Rr[i * 3 + 2] * p[2] + tt[i];
+            wk[i] = Rr[i * 3] * rk[0] + Rr[i * 3 + 1] * rk[1] + Rr[i * 3 + 2];
+        }
+        if (xs[2] <= 0.0) return 0;
+        const double qx = xs[0] / xs[2], qy = xs[1] / xs[2];
+        const double r2 = qx * qx + qy * qy;
+        const double D = 1.0 + v->k1 * r2 + v->k2 * r2 * r2;
+        const double dD = 2.0 * v->k1 + 4.0 * v->k2 * r2; /* dD/dx = dD * x */
+        const double u = v->f * D * qx + v->cx, vv = v->f * D * qy + v->cy;
+        double val, gu, gv;
+        if (!gray_at64(v->gray, v->w, v->h, u, vv, &val, &gu, &gv, cell)) return 0;
+        sv[k] = val;
+        /* d(u, v)/d(qx, qy) */
+        const double ux = v->f * (D + qx * dD * qx), uy = v->f * qx * dD * qy;
+        const double vx = v->f * qy * dD * qx, vy = v->f * (D + qy * dD * qy);
+        /* d(qx, qy)/d dq = (w0 - qx w2, w1 - qy w2) / xs2 */
+        const double dqx = (wk[0] - qx * wk[2]) / xs[2], dqy = (wk[1] - qy * wk[2]) / xs[2];
+        const double dI = gu * (ux * dqx + uy * dqy) + gv * (vx * dqx + vy * dqy);
+        for (int a = 0; a < 4; ++a) dsv[k][a] = dI * pw->ddq[k][a];
+    }
+    return 1;
+}
+
+int orc_pair_terms64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
+                     int32_t s1, double *cost, double grad[4], double jtj[16], uint64_t *cells) {
+    const int32_t N = c->n;
+    if (s0 < 0 || s1 < 0 || s0 >= c->nsrc || s1 >= c->nsrc) return 0;
+    pair_window64 pw;
+    if (!pair_window(c, x, y, hyp, &pw)) return 0;
     uint64_t cell = 0;
-    double total = 0.0, gsum[4] = {0, 0, 0, 0};
+    double total = 0.0, gsum[4] = {0, 0, 0, 0}, H[16];
+    for (int a = 0; a < 16; ++a) H[a] = 0.0;
     const int32_t pair[2] = {s0, s1};
     for (int ps = 0; ps < 2; ++ps) {
-        const int32_t si = pair[ps];
-        const orc_view *v = &c->srcs[si];
-        double Rr[9], tt[3];
-        /* relative pose in f64 (the f32 cast is the kernels' business) */
-        {
-            const double *Rr_ = c->rf.R, *Cr = c->rf.C, *Rs = v->R, *Cs = v->C;
-            double dc[3] = {Cr[0] - Cs[0], Cr[1] - Cs[1], Cr[2] - Cs[2]};
-            for (int i = 0; i < 3; ++i) {
-                for (int j = 0; j < 3; ++j)
-                    Rr[i * 3 + j] = Rs[i * 3] * Rr_[j * 3] + Rs[i * 3 + 1] * Rr_[j * 3 + 1] +
-                                    Rs[i * 3 + 2] * Rr_[j * 3 + 2];
-                tt[i] = Rs[i * 3] * dc[0] + Rs[i * 3 + 1] * dc[1] + Rs[i * 3 + 2] * dc[2];
-            }
-        }
         double sv[ORC_MAX_TAPS], dsv[ORC_MAX_TAPS][4];
-        for (int32_t k = 0; k < N; ++k) {
-            double p[3] = {dq[k] * rk[k][0], dq[k] * rk[k][1], dq[k]};
-            double xs[3], wk[3];
-            for (int i = 0; i < 3; ++i) {
-                xs[i] = Rr[i * 3] * p[0] + Rr[i * 3 + 1] * p[1] + Rr[i * 3 + 2] * p[2] + tt[i];
-                wk[i] = Rr[i * 3] * rk[k][0] + Rr[i * 3 + 1] * rk[k][1] + Rr[i * 3 + 2];
-            }
-            if (xs[2] <= 0.0) return 0;
-            const double qx = xs[0] / xs[2], qy = xs[1] / xs[2];
-            const double r2 = qx * qx + qy * qy;
-            const double D = 1.0 + v->k1 * r2 + v->k2 * r2 * r2;
-            const double dD = 2.0 * v->k1 + 4.0 * v->k2 * r2; /* dD/dx = dD * x */
-            const double u = v->f * D * qx + v->cx, vv = v->f * D * qy + v->cy;
-            double val, gu, gv;
-            if (!gray_at64(v->gray, v->w, v->h, u, vv, &val, &gu, &gv, &cell)) return 0;
-            sv[k] = val;
-            /* d(u, v)/d(qx, qy) */
-            const double ux = v->f * (D + qx * dD * qx), uy = v->f * qx * dD * qy;
-            const double vx = v->f * qy * dD * qx, vy = v->f * (D + qy * dD * qy);
-            /* d(qx, qy)/d dq = (w0 - qx w2, w1 - qy w2) / xs2 */
-            const double dqx = (wk[0] - qx * wk[2]) / xs[2], dqy = (wk[1] - qy * wk[2]) / xs[2];
-            const double dI = gu * (ux * dqx + uy * dqy) + gv * (vx * dqx + vy * dqy);
-            for (int a = 0; a < 4; ++a) dsv[k][a] = dI * ddq[k][a];
-        }
+        if (!pair_source(c, &pw, pair[ps], sv, dsv, &cell)) return 0;
         double ms = 0.0;
         for (int32_t k = 0; k < N; ++k) ms += sv[k];
         ms /= N;
         double nb = 0.0, ab = 0.0;
         for (int32_t k = 0; k < N; ++k) {
             nb += (sv[k] - ms) * (sv[k] - ms);
-            ab += (refv[k] - mr) * (sv[k] - ms);
+            ab += (pw.refv[k] - pw.mr) * (sv[k] - ms);
         }
         nb = sqrt(nb);
         if (nb < 1e-12) return 0;
-        const double ncc = ab / (na * nb);
+        const double ncc = ab / (pw.na * nb);
         total += ncc;
         /* d ncc / d sv_k = (a_k / |a| - ncc b_k / |b|) / |b| */
         for (int32_t k = 0; k < N; ++k) {
-            const double g = ((refv[k] - mr) / na - ncc * (sv[k] - ms) / nb) / nb;
+            const double g = ((pw.refv[k] - pw.mr) / pw.na - ncc * (sv[k] - ms) / nb) / nb;
             for (int a = 0; a < 4; ++a) gsum[a] += g * dsv[k][a];
+        }
+        if (jtj) {
+            /* residual r = a/|a| - b/|b| (1 - ncc = |r|^2 / 2); its Jacobian
+             * is -(I - u u^T) Dc / |b| with u = b/|b| and Dc the derivatives
+             * minus their window means, so J^T J = (Dc^T Dc - (Dc^T u)(u^T Dc)) / |b|^2 */
+            double dm[4] = {0, 0, 0, 0}, du[4] = {0, 0, 0, 0};
+            for (int32_t k = 0; k < N; ++k)
+                for (int a = 0; a < 4; ++a) dm[a] += dsv[k][a];
+            for (int a = 0; a < 4; ++a) dm[a] /= N;
+            for (int32_t k = 0; k < N; ++k) {
+                const double u = (sv[k] - ms) / nb;
+                for (int a = 0; a < 4; ++a) du[a] += u * (dsv[k][a] - dm[a]);
+            }
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    double dd = 0.0;
+                    for (int32_t k = 0; k < N; ++k) dd += (dsv[k][a] - dm[a]) * (dsv[k][b] - dm[b]);
+                    H[4 * a + b] += (dd - du[a] * du[b]) / (nb * nb);
+                }
         }
     }
     *cost = 1.0 - 0.5 * total;
     if (grad)
         for (int a = 0; a < 4; ++a) grad[a] = -0.5 * gsum[a];
+    if (jtj)
+        for (int a = 0; a < 16; ++a) jtj[a] = H[a];
+    if (cells) *cells = cell;
+    return 1;
+}
+
+int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
+                    int32_t s1, double *cost, double grad[4], uint64_t *cells) {
+    return orc_pair_terms64(c, x, y, hyp, s0, s1, cost, grad, NULL, cells);
+}
+
+int orc_pair_unit64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s,
+                    double *unit, uint64_t *cells) {
+    const int32_t N = c->n;
+    if (s < 0 || s >= c->nsrc) return 0;
+    pair_window64 pw;
+    if (!pair_window(c, x, y, hyp, &pw)) return 0;
+    uint64_t cell = 0;
+    double sv[ORC_MAX_TAPS], dsv[ORC_MAX_TAPS][4];
+    if (!pair_source(c, &pw, s, sv, dsv, &cell)) return 0;
+    double ms = 0.0;
+    for (int32_t k = 0; k < N; ++k) ms += sv[k];
+    ms /= N;
+    double nb = 0.0;
+    for (int32_t k = 0; k < N; ++k) nb += (sv[k] - ms) * (sv[k] - ms);
+    nb = sqrt(nb);
+    if (nb < 1e-12) return 0;
+    for (int32_t k = 0; k < N; ++k) unit[k] = (sv[k] - ms) / nb;
     if (cells) *cells = cell;
     return 1;
 }

@@ -97,6 +97,17 @@ int64_t orc_depth_map(const orc_ctx *c, int32_t iterations, int32_t nthreads, fl
  * can skip steps that cross a cell edge. */
 int orc_pair_cost64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
                     int32_t s1, double *cost, double grad[4], uint64_t *cells);
+/* As orc_pair_cost64, and jtj (may be NULL) receives the 4 x 4 Gauss-Newton
+ * matrix J^T J (row-major, w.r.t. d, n0, n1, n2) of the pair's stacked
+ * residuals r_s = a/|a| - b_s/|b_s| (a, b_s the centred reference and sample
+ * windows; 1 - NCC_s = |r_s|^2 / 2). No reference counterpart. */
+int orc_pair_terms64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s0,
+                     int32_t s1, double *cost, double grad[4], double jtj[16], uint64_t *cells);
+/* The unit centred sample window b_s / |b_s| of source s (N values, tap
+ * order), the part of r_s that depends on the plane: the finite-difference
+ * check of J^T J differentiates it. 0 where undefined. */
+int orc_pair_unit64(const orc_ctx *c, int32_t x, int32_t y, const double hyp[4], int32_t s,
+                    double *unit, uint64_t *cells);
 
 /* Geometric consistency of one reference depth map against neighbour maps
  * (dense.rs:585-615). Writes the agree count per pixel (-1 where depth <= 0)

@@ -97,6 +97,13 @@ def lib():
                                       C.c_int32, C.POINTER(C.c_double), _f64p,
                                       C.POINTER(C.c_uint64)]
         L.orc_pair_cost64.restype = C.c_int
+        L.orc_pair_terms64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int32,
+                                       C.c_int32, C.POINTER(C.c_double), _f64p, _f64p,
+                                       C.POINTER(C.c_uint64)]
+        L.orc_pair_terms64.restype = C.c_int
+        L.orc_pair_unit64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int32, _f64p,
+                                      C.POINTER(C.c_uint64)]
+        L.orc_pair_unit64.restype = C.c_int
         L.orc_fuse_points.argtypes = [C.POINTER(OrcView), _f32p, C.POINTER(OrcNbrDepth),
                                       C.c_int32, C.c_int32, C.c_int32, C.c_double, _i32p,
                                       _f64p, _i32p]
@@ -288,6 +295,31 @@ class OracleDepth:
         if not ok:
             return None
         return c.value, g, cell.value
+
+    def pair_terms64(self, x, y, hyp, s0, s1):
+        """f64 fixed-pair cost, gradient w.r.t. (d, n0, n1, n2), the 4 x 4
+        Gauss-Newton matrix J^T J of the pair's stacked residuals
+        a/|a| - b_s/|b_s| and the cell signature; None where undefined."""
+        h = np.ascontiguousarray(hyp, np.float64).reshape(4)
+        c = C.c_double(0.0)
+        g = np.zeros(4, np.float64)
+        jtj = np.zeros(16, np.float64)
+        cell = C.c_uint64(0)
+        ok = lib().orc_pair_terms64(self.ctx, int(x), int(y), h, int(s0), int(s1), C.byref(c),
+                                    g, jtj, C.byref(cell))
+        if not ok:
+            return None
+        return c.value, g, jtj.reshape(4, 4), cell.value
+
+    def pair_unit64(self, x, y, hyp, s):
+        """Unit centred sample window b_s/|b_s| of source s (f64, tap order)
+        and the cell signature; None where undefined."""
+        h = np.ascontiguousarray(hyp, np.float64).reshape(4)
+        u = np.zeros(self.params.taps * self.params.taps, np.float64)
+        cell = C.c_uint64(0)
+        if not lib().orc_pair_unit64(self.ctx, int(x), int(y), h, int(s), u, C.byref(cell)):
+            return None
+        return u, cell.value
 
     def pair_costs64(self, xs, ys, hyps, pairs):
         """Vector form of pair_cost64: (cost (nan if undefined), grad (n, 4))."""

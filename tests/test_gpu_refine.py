@@ -187,6 +187,137 @@ def test_pair_argument_errors(gpu_ctx):
     assert len(c) == 0
 
 
+def _jtj_rel_err(H, want, depth):
+    """Max-abs error of J^T J relative to its largest entry, depth and normal
+    entries weighed alike (normal rows and columns scaled by 1/depth)."""
+    sc = np.stack([np.ones_like(depth), 1 / depth, 1 / depth, 1 / depth], axis=1)
+    S = sc[:, :, None] * sc[:, None, :]
+    e = np.abs((H - want) * S).reshape(len(H), -1).max(axis=1)
+    r = np.abs(want * S).reshape(len(H), -1).max(axis=1)
+    return e / np.maximum(r, 1e-30)
+
+
+TERMS_CASES = [("tiny", 4, 5, 2, False), ("small", 4, 7, 1, False), ("small", 8, 7, 1, False),
+               ("small", 4, 11, 1, False), ("small", 4, 7, 1, True), ("small", 8, 11, 1, True)]
+
+
+@pytest.mark.parametrize("name,k,taps,stride,f32", TERMS_CASES)
+def test_refinement_terms_match_oracle(gpu_ctx, name, k, taps, stride, f32):
+    """The Gauss-Newton terms the refinement kernels form — one lane per pixel
+    on 8-bit undistorted views (k_refine_lane), two lanes per pixel on f32
+    views (k_refine) — against the f64 oracle: pair cost within 3e-5, cost
+    gradient and J^T J within 0.02% at the median and 0.3% on 99% of items
+    (max-abs relative, depth and normal weighed alike), pairs chosen at the
+    plane costing no more than the f64 best pair."""
+    s, v, srcs, o = _setup(name, k, taps, stride)
+    if f32:
+        v, srcs = _as_f32([v])[0], _as_f32(srcs)
+    prm = capi.params(taps=taps, stride=stride)
+    rng = np.random.default_rng(61 + taps + k + 7 * f32)
+    xs, ys, hy = _items(v, 1500, rng)
+    pairs = np.stack([rng.integers(0, k, len(xs)), rng.integers(0, k, len(xs))], 1)
+    pairs[:, 1] = np.where(pairs[:, 0] == pairs[:, 1], (pairs[:, 1] + 1) % k, pairs[:, 1])
+    pairs[: len(xs) // 4] = -1  # a quarter of the items choose their pair
+    gpu_ctx.reset_stats()
+    _say(f"refine terms {name} k={k} taps={taps} {'f32' if f32 else '8-bit'}: {len(xs)} items")
+    cost, pq, grad, jtj = gpu_ctx.refine_terms(v, srcs, *s.spec.depth_range, prm, xs, ys, hy,
+                                               pairs=pairs)
+    assert gpu_ctx.stats()["split"] == (1 if f32 else -1)  # the kernel under test
+    given = pairs[:, 0] >= 0
+    assert np.array_equal(pq[given], pairs[given])
+    chose = ~given & (pq[:, 0] >= 0)
+    want = np.full(len(xs), np.nan)
+    wgrad = np.full((len(xs), 4), np.nan)
+    wjtj = np.full((len(xs), 4, 4), np.nan)
+    for i in range(len(xs)):
+        if pq[i, 0] < 0:
+            continue
+        t = o.pair_terms64(xs[i], ys[i], hy[i].astype(np.float64), pq[i, 0], pq[i, 1])
+        if t is not None:
+            want[i], wgrad[i], wjtj[i] = t[0], t[1], t[2]
+    undefined = np.isnan(want)
+    mismatch = np.mean(undefined[given] != (cost[given] >= 2.0))
+    _say(f"  defined {np.mean(~undefined):.3f}, definedness mismatch {mismatch:.4f}")
+    assert mismatch <= 0.002
+    both = ~undefined & (cost < 2.0)
+    assert both.sum() > 300
+    d = np.abs(cost[both] - want[both])
+    _say(f"  cost max|diff| {d.max():.3g}")
+    assert d.max() <= COST_TOL
+    assert np.all(grad[cost >= 2.0] == 0) and np.all(jtj[cost >= 2.0] == 0)
+    dep = hy[both, 0].astype(np.float64)
+    rel = _grad_rel_err(grad[both].astype(np.float64), wgrad[both], dep)
+    relj = _jtj_rel_err(jtj[both].astype(np.float64), wjtj[both], dep)
+    _say(f"  grad rel err median {np.median(rel):.3g} p99 {np.quantile(rel, 0.99):.3g}; "
+         f"J^T J rel err median {np.median(relj):.3g} p99 {np.quantile(relj, 0.99):.3g}")
+    assert np.median(rel) < 2e-4 and np.quantile(rel, 0.99) < 3e-3
+    assert np.median(relj) < 2e-4 and np.quantile(relj, 0.99) < 3e-3
+    # chosen pairs: no worse (in f64) than the f64 best pair at the plane
+    n_chose = 0
+    for i in np.nonzero(chose)[0][:150]:
+        single = []
+        for sidx in range(k):
+            r = o.pair_cost64(xs[i], ys[i], hy[i].astype(np.float64), sidx, sidx)
+            if r is not None:
+                single.append(r[0])
+        if len(single) < 2 or np.isnan(want[i]):
+            continue
+        single.sort()
+        assert want[i] <= 0.5 * (single[0] + single[1]) + COST_TOL
+        n_chose += 1
+    assert n_chose > 50
+
+
+def test_lane_refinement_terms_match_finite_differences(gpu_ctx):
+    """The one-lane-per-pixel refinement kernel's gradient and J^T J against
+    f64 central differences (1e-4 m in depth, 1e-5 per normal component) of
+    the oracle's pair cost and of its residual vectors, skipping items whose
+    bilinear cells change within a step."""
+    from test_oracle_gradient import _jtj_fd
+    s, v, srcs, o = _setup("small", 4, 7, 1)
+    prm = capi.params(taps=7, stride=1)
+    rng = np.random.default_rng(67)
+    xs, ys, hy = _items(v, 600, rng, spread=0.02, tilt=0.1)
+    pairs = np.tile(np.array([[0, 1]], np.int32), (len(xs), 1))
+    gpu_ctx.reset_stats()
+    cost, _, grad, jtj = gpu_ctx.refine_terms(v, srcs, *s.spec.depth_range, prm, xs, ys, hy,
+                                              pairs=pairs)
+    assert gpu_ctx.stats()["split"] == -1
+    hs = (1e-4, 1e-5, 1e-5, 1e-5)
+    rg, rj = [], []
+    for i in range(len(xs)):
+        if cost[i] >= 2.0:
+            continue
+        h64 = hy[i].astype(np.float64)
+        base = o.pair_cost64(xs[i], ys[i], h64, 0, 1)
+        if base is None:
+            continue
+        fd = np.zeros(4)
+        ok = True
+        for a in range(4):
+            hp, hm = h64.copy(), h64.copy()
+            hp[a] += hs[a]
+            hm[a] -= hs[a]
+            p, m = o.pair_cost64(xs[i], ys[i], hp, 0, 1), o.pair_cost64(xs[i], ys[i], hm, 0, 1)
+            if p is None or m is None or p[2] != base[2] or m[2] != base[2]:
+                ok = False
+                break
+            fd[a] = (p[0] - m[0]) / (2 * hs[a])
+        hfd = _jtj_fd(o, xs[i], ys[i], h64, 0, 1) if ok else None
+        if hfd is None:
+            continue
+        dep = np.array([h64[0]])
+        rg.append(_grad_rel_err(grad[i:i + 1].astype(np.float64), fd[None], dep)[0])
+        rj.append(_jtj_rel_err(jtj[i:i + 1].astype(np.float64), hfd[None], dep)[0])
+    rg, rj = np.array(rg), np.array(rj)
+    _say(f"lane FD check: {len(rg)} items; grad median {np.median(rg):.3g} "
+         f"p99 {np.quantile(rg, 0.99):.3g}; J^T J median {np.median(rj):.3g} "
+         f"p99 {np.quantile(rj, 0.99):.3g}")
+    assert len(rg) > 150
+    assert np.median(rg) < 2e-4 and np.quantile(rg, 0.99) < 3e-3
+    assert np.median(rj) < 2e-4 and np.quantile(rj, 0.99) < 3e-3
+
+
 def _as_f32(views):
     """The same views given as f32 grey (the table values of their bytes): the
     refinement then runs the two-lanes-per-pixel kernel instead of the

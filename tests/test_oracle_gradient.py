@@ -1,5 +1,6 @@
 """Pins of the oracle's f64 fixed-pair cost and its analytic gradient
-(orc_pair_cost64), the checker of the GPU backward pass and refinement.
+(orc_pair_cost64, orc_pair_terms64), the checker of the GPU backward pass and
+refinement.
 
 Each check is chosen so that a plausible mistake fails it:
   - forward: with the pair of the two best sources, the f64 pair cost equals
@@ -11,7 +12,10 @@ Each check is chosen so that a plausible mistake fails it:
     signature changes within the step — a dropped or mis-signed term fails it;
   - the plane is invariant under scaling the normal, so n . dcost/dn = 0 — a
     missing d(dq)/dn term fails it;
-  - along the true surface the depth gradient points toward the truth.
+  - along the true surface the depth gradient points toward the truth;
+  - the Gauss-Newton matrix J^T J against J^T J formed from f64 central
+    differences of the residual vectors themselves, symmetric, positive
+    semi-definite, with the normal as a null direction.
 """
 import numpy as np
 import pytest
@@ -179,3 +183,62 @@ def test_undefined_cases():
     assert o.pair_cost64(1, 1, h, 0, 1) is None        # window off the image
     assert o.pair_cost64(80, 60, np.array([h[0], 1.0, 0.0, 0.0]), 0, 1) is None  # edge-on plane
     assert o.pair_cost64(80, 60, np.array([-h[0], 0.0, 0.0, -1.0]), 0, 1) is None  # behind
+
+
+def _jtj_fd(o, x, y, h, s0, s1):
+    """J^T J of the pair's residuals by f64 central differences of the unit
+    centred sample windows (r_s = a/|a| - u_s, so J_r = -J_u); None where a
+    step leaves the definedness region or crosses a bilinear cell edge."""
+    steps = (H_DEPTH, H_NORMAL, H_NORMAL, H_NORMAL)
+    H = np.zeros((4, 4))
+    for s in (s0, s1):
+        base = o.pair_unit64(x, y, h, s)
+        if base is None:
+            return None
+        J = np.zeros((len(base[0]), 4))
+        for a in range(4):
+            hp, hm = h.copy(), h.copy()
+            hp[a] += steps[a]
+            hm[a] -= steps[a]
+            p, m = o.pair_unit64(x, y, hp, s), o.pair_unit64(x, y, hm, s)
+            if p is None or m is None or p[1] != base[1] or m[1] != base[1]:
+                return None
+            J[:, a] = (p[0] - m[0]) / (2 * steps[a])
+        H += J.T @ J
+    return H
+
+
+def test_pair_jtj_matches_finite_differences_of_the_residuals():
+    """The oracle's analytic Gauss-Newton matrix (the checker of both GPU
+    refinement kernels' J^T J) against J^T J formed from f64 central
+    differences of the residual vectors themselves: a dropped centring or
+    projection term, or a wrong source pose, fails it. The matrix is also
+    symmetric positive semi-definite with the normal as a null direction (the
+    plane, hence every residual, is invariant under scaling n)."""
+    s, v, o = _ctx("small", 4, 7, 1)
+    rng = np.random.default_rng(17)
+    xs, ys, hy = _items(v, 300, rng, spread=0.02, tilt=0.1)
+    rels, nulls = [], []
+    for i in range(len(xs)):
+        t = o.pair_terms64(xs[i], ys[i], hy[i], 0, 1)
+        if t is None:
+            continue
+        H = t[2]
+        assert np.allclose(H, H.T, rtol=0, atol=1e-12 * np.abs(H).max())
+        assert np.linalg.eigvalsh(H).min() >= -1e-9 * np.abs(H).max()
+        n = np.concatenate([[0.0], hy[i, 1:]])
+        nulls.append(np.linalg.norm(H @ n) / np.linalg.norm(H))
+        fd = _jtj_fd(o, xs[i], ys[i], hy[i], 0, 1)
+        if fd is None:
+            continue
+        # depth and normal entries weighed alike: normal columns scaled by 1/depth
+        sc = np.diag([1.0, 1 / hy[i, 0], 1 / hy[i, 0], 1 / hy[i, 0]])
+        e = np.abs(sc @ (H - fd) @ sc).max() / np.abs(sc @ fd @ sc).max()
+        rels.append(e)
+    rels = np.array(rels)
+    assert len(rels) > 100
+    print(f"J^T J vs finite differences: {len(rels)} items, median {np.median(rels):.3g}, "
+          f"max {rels.max():.3g}; |H n|/|H| max {max(nulls):.3g}")
+    assert np.median(rels) < 1e-6 and rels.max() < 1e-4
+    assert max(nulls) < 1e-6
+
