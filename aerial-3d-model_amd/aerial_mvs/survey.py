@@ -267,11 +267,13 @@ UNEVEN = {
 }
 
 
-def make_uneven_survey(spec, nthreads=None):
+def make_uneven_survey(spec, nthreads=None, views=None):
     """Render an uneven survey; returns (Survey, per-view dict(k, taps,
     stride, range)). Nadir views fly a lawnmower block (C1 field of view,
     75% / 65% overlap); oblique views sit on a ring around the block at the
-    same altitude, pitched 45 degrees down and facing the patch centre."""
+    same altitude, pitched 45 degrees down and facing the patch centre.
+    `views` (optional) restricts which views get pixels and depth (others
+    keep an empty image); cameras and per-view draws are the same."""
     if isinstance(spec, str):
         spec = UNEVEN[spec]
     nthreads = nthreads or min(16, os.cpu_count() or 1)
@@ -283,7 +285,7 @@ def make_uneven_survey(spec, nthreads=None):
     scene = L.synth_scene_create(-margin, -margin, px + 2 * margin, py + 2 * margin,
                                  0.1 * max(spec.scale, 0.2), 50.0 * spec.scale, 6, 0.5,
                                  15.0 * spec.scale, spec.seed, 0.2, 5, nthreads)
-    views, cams = [], []
+    rendered, cams = [], []
     try:
         info = np.zeros(6)
         L.synth_scene_info(scene, info)
@@ -315,13 +317,14 @@ def make_uneven_survey(spec, nthreads=None):
             R, Cc = from_angles((c[0], c[1], ground + c[2]), yaw, pitch, roll)
             img = np.zeros((h, w), np.uint8)
             dep = np.zeros((h, w), np.float64)
-            L.synth_render(scene, w, h, f, w / 2.0, h / 2.0, 0.0, 0.0,
-                           np.ascontiguousarray(R.ravel()), Cc, spec.seed + 1000 + vi, 1.0,
-                           img.ravel(), dep.ctypes.data, nthreads)
-            views.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
+            if views is None or vi in views:
+                L.synth_render(scene, w, h, f, w / 2.0, h / 2.0, 0.0, 0.0,
+                               np.ascontiguousarray(R.ravel()), Cc, spec.seed + 1000 + vi, 1.0,
+                               img.ravel(), dep.ctypes.data, nthreads)
+            rendered.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
     finally:
         L.synth_scene_destroy(scene)
-    cent = np.array([v.C for v in views])
+    cent = np.array([v.C for v in rendered])
     nb = nearest_neighbours(cent, spec.k_range[1])
     per = []
     for vi, (kind, *_rest) in enumerate(cams):
@@ -330,7 +333,7 @@ def make_uneven_survey(spec, nthreads=None):
         rng_v = spec.nadir_range if kind == "nadir" else spec.oblique_range
         per.append(dict(kind=kind, k=k, taps=taps, stride=1 if taps > 5 else 2,
                         range=tuple(r * spec.scale for r in rng_v)))
-    surv = Survey(SPECS["small"], views, nb, [p["range"] for p in per], ground)
+    surv = Survey(SPECS["small"], rendered, nb, [p["range"] for p in per], ground)
     return surv, per
 
 
@@ -457,9 +460,11 @@ LARGE = {
 }
 
 
-def make_large_survey(spec, progress=None):
+def make_large_survey(spec, progress=None, views=None):
     """Render a large survey on the GPU. Returns (Survey, per-view dict(kind,
-    range)); depth_gt (f32) only on every spec.depth_every-th view."""
+    range)); depth_gt (f32) only on every spec.depth_every-th view. `views`
+    (optional) restricts which views get pixels (others keep an empty image
+    and no depth); cameras, boxes and neighbours are the same."""
     if isinstance(spec, str):
         spec = LARGE[spec]
     rng = np.random.default_rng(spec.seed + 1)
@@ -496,13 +501,16 @@ def make_large_survey(spec, progress=None):
                 y = (r + 0.5) * by / gy
                 for yaw in (0.0, 90.0, 180.0, 270.0):
                     cams.append(("oblique", (x, y, alt), yaw, -45.0, 0.0))
-        views, per = [], []
+        rendered, per = [], []
         for vi, (kind, c, yaw, pitch, roll) in enumerate(cams):
             R, Cc = from_angles(c, yaw, pitch, roll)
             want_depth = vi % spec.depth_every == 0
-            img, dep = scene.render(w, h, f, w / 2.0, h / 2.0, R, Cc, spec.seed + 1000 + vi,
-                                    spec.noise_dn, want_depth)
-            views.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
+            if views is None or vi in views:
+                img, dep = scene.render(w, h, f, w / 2.0, h / 2.0, R, Cc, spec.seed + 1000 + vi,
+                                        spec.noise_dn, want_depth)
+            else:
+                img, dep = np.zeros((h, w), np.uint8), None
+            rendered.append(View(img, f, w / 2.0, h / 2.0, 0.0, 0.0, R, Cc, dep))
             per.append(dict(kind=kind, yaw=yaw,
                             range=spec.nadir_range if kind == "nadir" else spec.oblique_range))
             if progress and (vi + 1) % 200 == 0:
@@ -510,14 +518,14 @@ def make_large_survey(spec, progress=None):
     finally:
         scene.close()
     # neighbours: nearest by position among views of the same kind and heading
-    cent = np.array([v.C for v in views])
+    cent = np.array([v.C for v in rendered])
     nb = []
-    for i in range(len(views)):
-        same = np.array([j for j in range(len(views)) if j != i and per[j]["kind"] ==
+    for i in range(len(rendered)):
+        same = np.array([j for j in range(len(rendered)) if j != i and per[j]["kind"] ==
                          per[i]["kind"] and (per[i]["kind"] == "nadir" or
                                              per[j]["yaw"] == per[i]["yaw"])], np.int64)
         d = np.hypot(cent[same, 0] - cent[i, 0], cent[same, 1] - cent[i, 1])
         nb.append([int(j) for j in same[np.argsort(d, kind="stable")[:spec.neighbours + 2]]])
-    surv = Survey(SPECS["small"], views, nb, [p["range"] for p in per], ground)
+    surv = Survey(SPECS["small"], rendered, nb, [p["range"] for p in per], ground)
     surv.extra["large_spec"] = spec
     return surv, per
