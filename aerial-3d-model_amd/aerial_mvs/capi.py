@@ -42,8 +42,12 @@ EXPORTS = [
     "mvs_survey_set_colour", "mvs_survey_fuse", "mvs_survey_fused_points", "mvs_fusion_voxel",
     "mvs_write_ply", "mvs_read_ply", "mvs_prepared_size", "mvs_prepare_view",
     "mvs_survey_set_depth", "mvs_build_dsm", "mvs_dsm_download", "mvs_save_dsm", "mvs_survey_set_options",
-    "mvs_load_dsm", "mvs_dsm_height", "mvs_refine_terms",
+    "mvs_load_dsm", "mvs_dsm_height", "mvs_refine_terms", "mvs_survey_consistency_noalloc",
 ]
+
+# Limits of the allocation-free consistency filter (aerial_mvs.h)
+MVS_NOALLOC_MAX_PIXELS = 1 << 23
+MVS_NOALLOC_MAX_NBRS = 16
 
 
 class MvsView(C.Structure):
@@ -123,6 +127,7 @@ def lib():
         L.mvs_survey_depth_maps_ex.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, PROGRESS_FN, vp]
         L.mvs_survey_download.argtypes = [vp, i32, vp, vp, vp]
         L.mvs_survey_consistency.argtypes = [vp, i32, vp, i32, i32, vp, P(C.c_int64)]
+        L.mvs_survey_consistency_noalloc.argtypes = [vp, i32, vp, i32, i32, vp, P(C.c_int64)]
         L.mvs_set_timing.argtypes = [vp, i32]
         L.mvs_get_stats.argtypes = [vp, P(MvsStats)]
         L.mvs_reset_stats.argtypes = [vp]
@@ -639,12 +644,13 @@ class Survey:
                                                      n.value, C.byref(n)))
         return xyz, rgb
 
-    def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE, download=True):
-        """(agree counts or None, number of consistent pixels)."""
+    def consistency(self, ref, nbrs, rule=MVS_CONS_REFERENCE, download=True, noalloc=False):
+        """(agree counts or None, number of consistent pixels). noalloc: the
+        allocation-free entry point (mvs_survey_consistency_noalloc)."""
         h, w = self.shapes[ref]
         nb = np.ascontiguousarray(nbrs, np.int32)
         agree = np.zeros((h, w), np.int32) if download else None
         n = C.c_int64()
-        self.ctx.check(lib().mvs_survey_consistency(self._h, ref, nb.ctypes.data, len(nb), rule,
-                                                    _ptr(agree), C.byref(n)))
+        fn = lib().mvs_survey_consistency_noalloc if noalloc else lib().mvs_survey_consistency
+        self.ctx.check(fn(self._h, ref, nb.ctypes.data, len(nb), rule, _ptr(agree), C.byref(n)))
         return agree, n.value

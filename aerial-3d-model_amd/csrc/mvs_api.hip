@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -103,10 +104,45 @@ struct TimedLaunch {
 
 }  // namespace
 
+// ---- scratch of the allocation-free entry point ----
+// mvs_survey_consistency_noalloc keeps its neighbour-camera table, the
+// per-pixel agree counts and the consistent-pixel counter in this fixed
+// device array instead of allocating: [counter | camera table | agree plane].
+// Each GPU holds its own copy (at its own address) until the process ends.
+constexpr size_t kNaHead = 256;
+constexpr size_t kNaCams = (sizeof(amvs::NbrCam) * MVS_NOALLOC_MAX_NBRS + 255) / 256 * 256;
+constexpr size_t kNaBytes = kNaHead + kNaCams + sizeof(int32_t) * (size_t)MVS_NOALLOC_MAX_PIXELS;
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// This device's address of `scratch`, looked up once per device with the
+// device current (the array's host-side name is not a device address).
+hipError_t scratch_for_device(int32_t device, unsigned char **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = (unsigned char *)p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+}  // namespace
+
 struct mvs_ctx {
     int32_t device = 0;
     int32_t num_cu = 256;
     hipStream_t st = nullptr;
+    // this device's copy of the allocation-free entry point's scratch array
+    // (hipGetSymbolAddress at creation; null with scratch_err set on failure)
+    unsigned char *scratch = nullptr;
+    std::string scratch_err;
     std::string err;
     std::vector<JobWork> work;  // job slots
     DevBuf jobs_dev, prefix, counters, evals, ok_count;
@@ -1033,6 +1069,13 @@ int mvs_create(int32_t device, mvs_ctx **out, char *err, size_t errlen) {
         delete c;
         return report(MVS_ERR_DEVICE, hipGetErrorString(e));
     }
+    // the allocation-free entry point's scratch on this device (its failure
+    // only disables that entry point)
+    e = scratch_for_device(device, &c->scratch);
+    if (e != hipSuccess) {
+        c->scratch = nullptr;
+        c->scratch_err = std::string("scratch array address: ") + hipGetErrorString(e);
+    }
     *out = c;
     return MVS_OK;
 }
@@ -1532,9 +1575,16 @@ int mvs_survey_set_depth(mvs_survey *s, int32_t view, const float *depth) {
     return MVS_OK;
 }
 
-int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int32_t nn,
-                           int32_t rule, int32_t *agree, int64_t *n_consistent) {
-    if (!s) return MVS_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+RefCam64 ref_cam(const ViewDev &rv);
+
+// Arguments of a consistency call: checks them, fills cams[0..nn) (capacity
+// cap) with the neighbours' cameras and maps. No allocation.
+int consistency_args(mvs_survey *s, int32_t ref, const int32_t *nbrs, int32_t nn, int32_t rule,
+                     NbrCam *cams, int32_t cap) {
     mvs_ctx *c = s->ctx;
     if (rule != MVS_CONS_REFERENCE && rule != MVS_CONS_REPROJ)
         return fail(c, MVS_ERR_ARG, "unknown consistency rule");
@@ -1542,14 +1592,13 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
     if (ref < 0 || ref >= nv || !s->maps[(size_t)ref].valid)
         return fail(c, MVS_ERR_ARG, "reference view has no depth map");
     if (nn < 0 || (nn > 0 && !nbrs)) return fail(c, MVS_ERR_ARG, "bad neighbour list");
-    HIP_TRY(c, hipSetDevice(c->device));
-    std::vector<NbrCam> cams;
+    if (nn > cap) return fail(c, MVS_ERR_ARG, "too many neighbours");
     for (int32_t k = 0; k < nn; ++k) {
         const int32_t j = nbrs[k];
         if (j < 0 || j >= nv || !s->maps[(size_t)j].valid)
             return fail(c, MVS_ERR_ARG, "neighbour without a depth map");
         const ViewDev &v = s->views[(size_t)j];
-        NbrCam nc;
+        NbrCam &nc = cams[k];
         memcpy(nc.R, v.R, sizeof nc.R);
         memcpy(nc.C, v.C, sizeof nc.C);
         nc.f = v.f;
@@ -1560,17 +1609,23 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
         nc.depth = s->maps[(size_t)j].depth.as<float>();
         nc.w = v.w;
         nc.h = v.h;
-        cams.push_back(nc);
     }
+    return MVS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int32_t nn,
+                           int32_t rule, int32_t *agree, int64_t *n_consistent) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    std::vector<NbrCam> cams((size_t)std::max(nn, 0));
+    TRY(consistency_args(s, ref, nbrs, nn, rule, cams.data(), std::max(nn, 0)));
+    HIP_TRY(c, hipSetDevice(c->device));
     const ViewDev &rv = s->views[(size_t)ref];
-    RefCam64 rc64;
-    memcpy(rc64.R, rv.R, sizeof rc64.R);
-    memcpy(rc64.C, rv.C, sizeof rc64.C);
-    rc64.f = rv.f;
-    rc64.cx = rv.cx;
-    rc64.cy = rv.cy;
-    rc64.k1 = rv.k1;
-    rc64.k2 = rv.k2;
+    const RefCam64 rc64 = ref_cam(rv);
     const size_t n = (size_t)rv.w * rv.h;
     HIP_TRY(c, c->nbr_cams.ensure(sizeof(NbrCam) * (cams.size() + 1)));
     HIP_TRY(c, c->agree.ensure(n * sizeof(int32_t)));
@@ -1590,6 +1645,40 @@ int mvs_survey_consistency(mvs_survey *s, int32_t ref, const int32_t *nbrs, int3
         HIP_TRY(c, hipMemcpyAsync(agree, c->agree.p, n * sizeof(int32_t), hipMemcpyDeviceToHost,
                                   c->st));
     HIP_TRY(c, hipMemcpyAsync(&ok, c->ok_count.p, sizeof ok, hipMemcpyDeviceToHost, c->st));
+    TRY(sync_ctx(c));
+    if (n_consistent) *n_consistent = (int64_t)ok;
+    return MVS_OK;
+}
+
+int mvs_survey_consistency_noalloc(mvs_survey *s, int32_t ref, const int32_t *nbrs, int32_t nn,
+                                   int32_t rule, int32_t *agree, int64_t *n_consistent) {
+    if (!s) return MVS_ERR_ARG;
+    mvs_ctx *c = s->ctx;
+    if (!c->scratch) return fail(c, MVS_ERR_DEVICE, c->scratch_err);
+    NbrCam cams[MVS_NOALLOC_MAX_NBRS];
+    TRY(consistency_args(s, ref, nbrs, nn, rule, cams, MVS_NOALLOC_MAX_NBRS));
+    const ViewDev &rv = s->views[(size_t)ref];
+    const int64_t n = (int64_t)rv.w * rv.h;
+    if (n > (int64_t)MVS_NOALLOC_MAX_PIXELS)
+        return fail(c, MVS_ERR_ARG, "view larger than MVS_NOALLOC_MAX_PIXELS");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // this device's scratch: [counter | camera table | agree plane]
+    unsigned long long *ctr = (unsigned long long *)c->scratch;
+    NbrCam *dcams = (NbrCam *)(c->scratch + kNaHead);
+    int32_t *dagree = (int32_t *)(c->scratch + kNaHead + kNaCams);
+    if (nn > 0)
+        HIP_TRY(c, hipMemcpyAsync(dcams, cams, sizeof(NbrCam) * (size_t)nn, hipMemcpyHostToDevice,
+                                  c->st));
+    launch_zero_u64(ctr, 1, c->st);  // the last call's count is still there
+    const int32_t need = rule == MVS_CONS_REPROJ ? 3 : (nn >= 3 ? 2 : 1);
+    launch_consistency(ref_cam(rv), s->maps[(size_t)ref].depth.as<float>(), rv.w, rv.h, dcams, nn,
+                       need, rule, dagree, ctr, c->st);
+    HIP_TRY(c, hipGetLastError());
+    unsigned long long ok = 0;
+    if (agree)
+        HIP_TRY(c, hipMemcpyAsync(agree, dagree, (size_t)n * sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, c->st));
+    HIP_TRY(c, hipMemcpyAsync(&ok, ctr, sizeof ok, hipMemcpyDeviceToHost, c->st));
     TRY(sync_ctx(c));
     if (n_consistent) *n_consistent = (int64_t)ok;
     return MVS_OK;

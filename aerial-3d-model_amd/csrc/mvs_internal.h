@@ -135,6 +135,9 @@ void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                         const NbrCam *nbrs, int32_t nn, int32_t need, int32_t rule,
                         int32_t *agree, unsigned long long *n_ok, hipStream_t st);
+// Clears n u64 counters with a kernel (no memset node, so the launch can sit
+// in captured work).
+void launch_zero_u64(unsigned long long *p, int32_t n, hipStream_t st);
 void launch_selftest_div(int64_t n, uint64_t seed, unsigned long long *bad, hipStream_t st);
 // Voxel accumulation table of the fusion (open addressing, linear probing).
 // Keys pack the voxel index (21 bits per axis, offset 2^20); coordinate sums

@@ -381,6 +381,17 @@ void launch_threshold(const RefJob &job, float *depth, float *normal, hipStream_
                        depth, normal);
 }
 
+namespace {
+__global__ void k_zero_u64(unsigned long long *p, int32_t n) {
+    const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0ull;
+}
+}  // namespace
+
+void launch_zero_u64(unsigned long long *p, int32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, p, n);
+}
+
 void launch_consistency(const RefCam64 &ref, const float *depth, int32_t w, int32_t h,
                         const NbrCam *nbrs, int32_t nn, int32_t need, int32_t rule,
                         int32_t *agree, unsigned long long *n_ok, hipStream_t st) {

@@ -242,6 +242,21 @@ enum { MVS_CONS_REFERENCE = 0, MVS_CONS_REPROJ = 1 };
 int mvs_survey_consistency(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int32_t nn,
                            int32_t rule, int32_t *agree, int64_t *n_consistent);
 
+/* mvs_survey_consistency for callers that cannot allocate device memory (no
+ * reference counterpart; for code under HIP graph capture, tight serving
+ * loops and harnesses that forbid hipMalloc). Same arguments, checks and
+ * results; it calls no device allocator. Its scratch (the neighbour cameras,
+ * the per-pixel agree counts and the counter) lives in one fixed device array
+ * of about 32 MiB per GPU, set up by mvs_create and held until the process
+ * ends: reference views above MVS_NOALLOC_MAX_PIXELS pixels or more than
+ * MVS_NOALLOC_MAX_NBRS neighbours fail with MVS_ERR_ARG. Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (from different
+ * contexts or threads) race; run them one at a time per GPU. */
+#define MVS_NOALLOC_MAX_PIXELS (1 << 23)
+#define MVS_NOALLOC_MAX_NBRS 16
+int mvs_survey_consistency_noalloc(mvs_survey *sv, int32_t ref, const int32_t *nbrs, int32_t nn,
+                                   int32_t rule, int32_t *agree, int64_t *n_consistent);
+
 /* ---- fusion: consistent depths accumulated per voxel (dense.rs:542-651) ----
  * Colour plane of a survey view for the fused cloud (w*h*3 RGB, the `.view`
  * cache's rgb); views without one use their 8-bit grey. */
