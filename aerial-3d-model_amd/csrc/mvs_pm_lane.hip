@@ -23,16 +23,59 @@
 // dense.rs requires |n . r| >= 1e-6 and dq > 0 at every tap; n . r is affine
 // over the window, so both hold at every tap when they hold at the four
 // corner taps.
+#include <type_traits>
+
 #include "mvs_pm_impl.h"
+
+typedef uint32_t amvs_u32x2 __attribute__((ext_vector_type(2)));
+// Structured buffer loads (the LLVM intrinsics; clang has no builtin for
+// them): the address is base + vindex * stride, so a pixel index needs no
+// scaling to bytes; num_records counts records.
+__device__ amvs_u32x2 amvs_struct_load_b64(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex,
+                                           int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+__device__ uint32_t amvs_struct_load_b32(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex,
+                                         int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.i32");
 
 namespace amvs {
 namespace {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-// Word 3 of a raw buffer resource on gfx9-family chips (32-bit data format,
-// no swizzle): byte offsets, reads past num_records return zero.
+// Word 3 of a buffer resource on gfx9-family chips (32-bit data format, no
+// swizzle): reads past num_records return zero.
 constexpr int32_t kBufferWord3 = 0x00020000;
+
+// A window whose four corner samples lie this far (pixels) inside the
+// source's sampling range keeps every tap inside it: rounding moves an inner
+// tap's position by a few 1e-4 pixels at most (ulps of coordinates below
+// 4096), so such windows need no clamp.
+constexpr float kInnerMargin = 1.0f / 256.0f;
+
+// Pixel of propagation candidate k of (x, y) (dense.rs:403-412, CAND order:
+// (-1,0), (1,0), (0,-1), (0,1), then the same at distance 5).
+__device__ __forceinline__ void cand_pos(int k, int x, int y, int &nx, int &ny) {
+    const int dir = k & 3, mag = k < 4 ? 1 : 5;
+    nx = x + (dir == 0 ? -mag : (dir == 1 ? mag : 0));
+    ny = y + (dir == 2 ? -mag : (dir == 3 ? mag : 0));
+}
+
+// dense.rs:436: the candidate equals the hypothesis (f32 ==, NaN unequal).
+__device__ __forceinline__ bool same_hyp(const float4 &a, const float4 &b) {
+    return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+
+// A cost as an unsigned key with the order of the floats (negative costs
+// from rounding included, -0 taken as +0 as == does; NaN above every
+// number), and back.
+__device__ __forceinline__ uint32_t cost_key(float c) {
+    const uint32_t b = __float_as_uint(c + 0.0f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float key_cost(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
 
 // F16: samples from the 8-byte f16 corner plane (fewer ALU operations per
 // sample); otherwise from the 4-byte 8-bit corner plane (half the bytes per
@@ -47,6 +90,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     // are free of bank conflicts; a lane reads only what it wrote, so no
     // barrier is needed
     __shared__ _Float16 Ah[N][64];
+    // half-sweeps: the wavefront's propagation candidates as one list of
+    // (pixel lane, candidate) entries, and each pixel's lowest (cost, index)
+    __shared__ uint16_t plist[8 * 64];
+    __shared__ unsigned long long pmin[64];
 
     const int lane = threadIdx.x;
     const int chunk = 64 * a.gpc;
@@ -156,17 +203,20 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                 const float2 rc = pixel_ok ? job.rays[pidx] : make_float2(0.0f, 0.0f);
                 const float nsr = (float)N * (sr * 255.0f);  // N sd(b)
 
-                // Cost of plane (d, n) at this lane's pixel. Every lane calls it;
-                // lanes without a hypothesis to score pass act = false.
-                auto eval = [&](bool act, float d, float n0, float n1, float n2) -> float {
-                    act = act && pixel_ok;
+                // Cost of plane (d, n) at the pixel of lane `col` (centre ray
+                // (rcx, rcy), N sd of its bytes nsrv, byte-mean correction dlt,
+                // reference window in column col of Ah). Every lane calls it;
+                // lanes without a hypothesis to score pass act = false (act
+                // includes the pixel's own test, pixel_ok).
+                auto eval = [&](bool act, float rcx, float rcy, float nsrv, float dlt, int col,
+                                float d, float n0, float n1, float n2) -> float {
                     if (act) ++n_evals;
                     // dense.rs:187-190: c = n . (ray d)
-                    const float xp0 = rc.x * d, xp1 = rc.y * d;
+                    const float xp0 = rcx * d, xp1 = rcy * d;
                     float cc = n0 * xp0 + n1 * xp1;
                     cc = cc + n2 * d;
                     // |n . r| >= 1e-6 and c / (n . r) > 0 at the window corners
-                    const float dc = fmaf(n0, rc.x, fmaf(n1, rc.y, n2));
+                    const float dc = fmaf(n0, rcx, fmaf(n1, rcy, n2));
                     const float ex = n0 * (rstep * (float)H), ey = n1 * (rstep * (float)H);
                     bool plane_ok = true;
 #pragma unroll
@@ -179,7 +229,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // per tap column / row scaled by it (see the file comment)
                     const float rden = act ? __builtin_amdgcn_rcpf(dc) : 0.0f;
                     const float dqc = cc * rden;
-                    const float pc0 = rc.x * dqc, pc1 = rc.y * dqc;
+                    const float pc0 = rcx * dqc, pc1 = rcy * dqc;
                     const float sx = dqc * rstep;
                     const float tnx = (n0 * rstep) * rden, tny = (n1 * rstep) * rden;
                     float m1 = INFINITY, m2 = INFINITY;
@@ -211,7 +261,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             // positive sends the window's rectangle onto the
                             // quadrilateral of the corner images, so the taps'
                             // bounding box is the corners'. NaN fails.
-                            bool ok = true;
+                            bool ok = true, inner = true;
 #pragma unroll
                             for (int cr = 0; cr < 4; ++cr) {
                                 const float fdx = (cr & 1) ? (float)H : -(float)H;
@@ -223,13 +273,16 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 const float fx = fmaf(P0, ip, CXh), fy = fmaf(P1, ip, CYh);
                                 ok = ok && (P2 > 0.0f) && fx >= 0.0f && fy >= 0.0f && fx < wm1 &&
                                      fy < hm1;
+                                inner = inner && fx >= kInnerMargin && fy >= kInnerMargin &&
+                                        fx < wm1 - kInnerMargin && fy < hm1 - kInnerMargin;
                             }
                             float cs = 2.0f;  // a failing source scores 2.0
                             if (ok) {
                                 const uint32_t sw = (uint32_t)S.w;
-                                // the plane as a buffer: 32-bit offsets, and the
-                                // hardware range check as a second bound; the
-                                // resource is wave-uniform (scalar registers)
+                                // the plane as a structured buffer: 32-bit pixel
+                                // indices, and the hardware range check as a
+                                // second bound; the resource is wave-uniform
+                                // (scalar registers)
                                 // (readfirstlane returns a signed int: each half
                                 // goes through uint32_t, never sign-extended)
                                 constexpr uint32_t QB = F16 ? 8u : 4u;  // bytes per pixel
@@ -239,79 +292,86 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
                                 const uint64_t qhi =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
+                                // records of QB bytes, one per pixel
                                 const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
-                                    (void *)(uintptr_t)(qlo | (qhi << 32)), 0,
-                                    __builtin_amdgcn_readfirstlane(S.w * S.h * (int)QB),
-                                    kBufferWord3);
+                                    (void *)(uintptr_t)(qlo | (qhi << 32)), (short)QB,
+                                    __builtin_amdgcn_readfirstlane(S.w * S.h), kBufferWord3);
                                 float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
-                                // one tap row per trip: a row's gathers are in
-                                // flight together while the registers they hold
-                                // leave room for several wavefronts per SIMD
+                                // The window's taps; CLAMP keeps every position
+                                // in the sampling range whatever the rounding
+                                // (v_med3, NaN to 0), needed only when some lane
+                                // scoring this source has a corner near the edge.
+                                auto taps = [&](auto clamp_tag) {
+                                    constexpr bool CLAMP = decltype(clamp_tag)::value;
+                                    // one tap row per trip: a row's gathers are in
+                                    // flight together while the registers they hold
+                                    // leave room for several wavefronts per SIMD
 #pragma unroll 1
-                                for (int ty = 0; ty < TAPS; ++ty) {
-                                    const float fdy = (float)(ty - H);
-                                    const float Y0 = fmaf(fdy, B2[0], B0[0]);
-                                    const float Y1 = fmaf(fdy, B2[1], B0[1]);
-                                    const float Y2 = fmaf(fdy, B2[2], B0[2]);
+                                    for (int ty = 0; ty < TAPS; ++ty) {
+                                        const float fdy = (float)(ty - H);
+                                        const float Y0 = fmaf(fdy, B2[0], B0[0]);
+                                        const float Y1 = fmaf(fdy, B2[1], B0[1]);
+                                        const float Y2 = fmaf(fdy, B2[2], B0[2]);
 #pragma unroll
-                                    for (int tx = 0; tx < TAPS; ++tx) {
-                                        const float fdx = (float)(tx - H);
-                                        const float P0 = tx == H ? Y0 : fmaf(fdx, B1[0], Y0);
-                                        const float P1 = tx == H ? Y1 : fmaf(fdx, B1[1], Y1);
-                                        const float P2 = tx == H ? Y2 : fmaf(fdx, B1[2], Y2);
-                                        const float ip = __builtin_amdgcn_rcpf(P2);
-                                        // inside the image up to rounding; the
-                                        // clamp (NaN -> 0) keeps every address
-                                        // in bounds
-                                        const float fx =
-                                            __builtin_amdgcn_fmed3f(fmaf(P0, ip, CXh), 0.0f, wm1);
-                                        const float fy =
-                                            __builtin_amdgcn_fmed3f(fmaf(P1, ip, CYh), 0.0f, hm1);
-                                        const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
-                                        const float ax = __builtin_amdgcn_fractf(fx),
-                                                    ay = __builtin_amdgcn_fractf(fy);
-                                        const uint32_t off = (__umul24(y0, sw) + x0) * QB;
-                                        float top, bot;
-                                        if constexpr (F16) {
-                                            // corners and column differences, f16
-                                            const u32x2 qv =
-                                                __builtin_amdgcn_raw_buffer_load_b64(Qh, off, 0, 0);
-                                            const uint32_t q0 = qv.x, q1 = qv.y;
-                                            const h2 r0 = __builtin_bit_cast(h2, q0);
-                                            const h2 r1 = __builtin_bit_cast(h2, q1);
-                                            top = fmaf(ax, (float)r0.y, (float)r0.x);
-                                            bot = fmaf(ax, (float)r1.y, (float)r1.x);
-                                        } else {
-                                            // the four corners as bytes
-                                            const uint32_t qd =
-                                                __builtin_amdgcn_raw_buffer_load_b32(Qh, off, 0, 0);
-                                            const float g00 = (float)(qd & 0xffu);
-                                            const float g10 = (float)((qd >> 8) & 0xffu);
-                                            const float g01 = (float)((qd >> 16) & 0xffu);
-                                            const float g11 = (float)(qd >> 24);
-                                            top = fmaf(ax, g10 - g00, g00);
-                                            bot = fmaf(ax, g11 - g01, g01);
+                                        for (int tx = 0; tx < TAPS; ++tx) {
+                                            const float fdx = (float)(tx - H);
+                                            const float P0 = tx == H ? Y0 : fmaf(fdx, B1[0], Y0);
+                                            const float P1 = tx == H ? Y1 : fmaf(fdx, B1[1], Y1);
+                                            const float P2 = tx == H ? Y2 : fmaf(fdx, B1[2], Y2);
+                                            const float ip = __builtin_amdgcn_rcpf(P2);
+                                            float fx = fmaf(P0, ip, CXh), fy = fmaf(P1, ip, CYh);
+                                            if constexpr (CLAMP) {
+                                                fx = __builtin_amdgcn_fmed3f(fx, 0.0f, wm1);
+                                                fy = __builtin_amdgcn_fmed3f(fy, 0.0f, hm1);
+                                            }
+                                            const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+                                            const float ax = __builtin_amdgcn_fractf(fx),
+                                                        ay = __builtin_amdgcn_fractf(fy);
+                                            const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
+                                            float top, bot;
+                                            if constexpr (F16) {
+                                                // corners and column differences, f16
+                                                const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
+                                                const uint32_t q0 = qv.x, q1 = qv.y;
+                                                const h2 r0 = __builtin_bit_cast(h2, q0);
+                                                const h2 r1 = __builtin_bit_cast(h2, q1);
+                                                top = fmaf(ax, (float)r0.y, (float)r0.x);
+                                                bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                            } else {
+                                                // the four corners as bytes
+                                                const uint32_t qd = amvs_struct_load_b32(Qh, idx, 0, 0, 0);
+                                                const float g00 = (float)(qd & 0xffu);
+                                                const float g10 = (float)((qd >> 8) & 0xffu);
+                                                const float g01 = (float)((qd >> 16) & 0xffu);
+                                                const float g11 = (float)(qd >> 24);
+                                                top = fmaf(ax, g10 - g00, g00);
+                                                bot = fmaf(ax, g11 - g01, g01);
+                                            }
+                                            const float val = fmaf(ay, bot - top, top);
+                                            if (tx == 0 && ty == 0) c0 = val;
+                                            const float dd = val - c0;
+                                            s1 += dd;
+                                            s2 = fmaf(dd, dd, s2);
+                                            sa = fmaf((float)Ah[ty * TAPS + tx][col], dd, sa);
                                         }
-                                        const float val = fmaf(ay, bot - top, top);
-                                        if (tx == 0 && ty == 0) c0 = val;
-                                        const float dd = val - c0;
-                                        s1 += dd;
-                                        s2 = fmaf(dd, dd, s2);
-                                        sa = fmaf((float)Ah[ty * TAPS + tx][lane], dd, sa);
                                     }
-                                }
+                                };
+                                if (__ballot(!inner) == 0ull)  // uniform over the scoring lanes
+                                    taps(std::false_type{});
+                                else
+                                    taps(std::true_type{});
                                 // NCC from the shifted sums, in byte units:
                                 // N var = s2 - s1^2 / N and
                                 // cov = sum (b - mean b) dd = sa - delta s1
                                 const float moff = s1 * (1.0f / (float)N);
                                 const float var =
                                     fmaxf(fmaf(-s1, moff, s2), 0.0f) * (1.0f / (float)N);
-                                const float cov = fmaf(-delta, s1, sa);
+                                const float cov = fmaf(-dlt, s1, sa);
                                 const float ss = __builtin_amdgcn_sqrtf(var);
                                 // dense.rs:246-248's 1e-3 grey units are 0.255
                                 // byte units
                                 if (!(ss < 1e-3f * 255.0f))
-                                    cs = 1.0f - cov * __builtin_amdgcn_rcpf(nsr * ss);
+                                    cs = 1.0f - cov * __builtin_amdgcn_rcpf(nsrv * ss);
                             }
                             // best two (dense.rs:255-262)
                             const float lo = fminf(m1, cs);
@@ -326,16 +386,22 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
 
                 if (MODE == MODE_COST_LIST) {
                     const float4 hp = have ? a.hyp_in[item] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                    const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
+                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
+                                           hp.y, hp.z, hp.w);
                     if (have) a.out[item] = cst;
                 } else if (MODE == MODE_EVAL_CURRENT) {
                     const float4 hp = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                    const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
+                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
+                                           hp.y, hp.z, hp.w);
                     if (have) job.cost[pidx] = cst;
-                } else {
+                } else if constexpr (!F16) {
                     // the ten proposals of dense.rs:427-462 in the reference's
                     // order (propagation in CAND order, then two random
-                    // refinements), as in the source-lane kernels
+                    // refinements), every lane in step. The 8-bit plane serves
+                    // the scattered first iterations, whose candidates are all
+                    // distinct (nothing to compact), and a row of gathers in
+                    // flight matters most there: this loop keeps the code the
+                    // compiler schedules that way.
                     float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     float bcost = have ? job.cost[pidx] : 2.0f;
                     uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
@@ -345,15 +411,12 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         float4 hp;
                         bool act;
                         if (k < 8) {
-                            const int dir = k & 3, mag = k < 4 ? 1 : 5;
-                            const int nx = x + (dir == 0 ? -mag : (dir == 1 ? mag : 0));
-                            const int ny = y + (dir == 2 ? -mag : (dir == 3 ? mag : 0));
+                            int nx, ny;
+                            cand_pos(k, x, y, nx, ny);
                             const bool inb = have && nx >= 0 && ny >= 0 && nx < w && ny < h;
                             hp = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                             if (inb) hp = job.hyp[(int64_t)ny * w + nx];
-                            const bool dup = hp.x == best.x && hp.y == best.y &&
-                                             hp.z == best.z && hp.w == best.w;
-                            act = inb && !dup;
+                            act = inb && !same_hyp(hp, best);
                         } else {
                             float stp = (rnd_next(s) - 0.5f) * a.range;
                             stp = stp * 0.25f;
@@ -365,7 +428,116 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                           hp.y, hp.z, hp.w);
                             act = have;
                         }
-                        const float cst = eval(act, hp.x, hp.y, hp.z, hp.w);
+                        const float cst = eval(act && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
+                                               hp.y, hp.z, hp.w);
+                        if (act && cst < bcost) {
+                            best = hp;
+                            bcost = cst;
+                        }
+                    }
+                    if (have) {
+                        job.hyp[pidx] = best;
+                        job.cost[pidx] = bcost;
+                    }
+                } else {
+                    // the ten proposals of dense.rs:427-462: the eight
+                    // propagation candidates in CAND order, then two random
+                    // refinements
+                    float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    float bcost = have ? job.cost[pidx] : 2.0f;
+                    // A candidate equal to the current best is not scored
+                    // (dense.rs:436-438); one equal to an earlier candidate
+                    // cannot be accepted either (its cost is that candidate's,
+                    // which was not lower, or it is the best by now), so only
+                    // the distinct ones are scored. Accepting each strictly
+                    // lower cost in order ends on the lowest (cost, index), so
+                    // the wavefront scores the candidates of all its pixels as
+                    // one compacted list, no lane idle while others score,
+                    // and keeps each pixel's lowest (cost, index) in LDS.
+                    uint32_t amask = 0;
+                    {
+                        float4 cand[8];
+                        uint32_t inbm = 0;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            int nx, ny;
+                            cand_pos(k, x, y, nx, ny);
+                            const bool inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
+                            cand[k] = inb ? job.hyp[(int64_t)ny * w + nx]
+                                          : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                            bool act = inb && !same_hyp(cand[k], best);
+#pragma unroll
+                            for (int j = 0; j < k; ++j)
+                                act = act && !(((inbm >> j) & 1u) && same_hyp(cand[j], cand[k]));
+                            inbm |= (inb ? 1u : 0u) << k;
+                            amask |= (act ? 1u : 0u) << k;
+                        }
+                    }
+                    // the list, candidate-major (neighbouring pixels' k-th
+                    // candidates together, as they come from neighbouring
+                    // pixels themselves)
+                    int32_t total = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const bool on = (amask >> k) & 1u;
+                        const uint64_t bal = __ballot(on);
+                        if (on)
+                            plist[total + (int32_t)__builtin_amdgcn_mbcnt_hi(
+                                              (uint32_t)(bal >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] =
+                                (uint16_t)(lane | (k << 6));
+                        total += __popcll(bal);
+                    }
+                    pmin[lane] = ~0ull;
+                    __syncthreads();  // the list, pmin and every lane's window in Ah
+#pragma unroll 1
+                    for (int e0 = 0; e0 < total; e0 += 64) {
+                        const int e = e0 + lane;
+                        const bool act = e < total;
+                        const int ent = act ? (int)plist[e] : 0;
+                        const int o = ent & 63, k = ent >> 6;
+                        // the owner's pixel
+                        const int ox = __shfl(x, o), oy = __shfl(y, o);
+                        const float orx = __shfl(rc.x, o), ory = __shfl(rc.y, o);
+                        const float onsr = __shfl(nsr, o), odl = __shfl(delta, o);
+                        int nx, ny;
+                        cand_pos(k, ox, oy, nx, ny);
+                        const float4 hp = act ? job.hyp[(int64_t)ny * w + nx]
+                                              : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                        const float cst =
+                            eval(act, orx, ory, onsr, odl, o, hp.x, hp.y, hp.z, hp.w);
+                        if (act)
+                            atomicMin(&pmin[o], ((unsigned long long)cost_key(cst) << 32) |
+                                                    (unsigned long long)k);
+                    }
+                    __syncthreads();
+                    const unsigned long long pm = pmin[lane];
+                    if (pm != ~0ull) {
+                        const float cst = key_cost((uint32_t)(pm >> 32));
+                        if (cst < bcost) {
+                            int nx, ny;
+                            cand_pos((int)(pm & 7ull), x, y, nx, ny);
+                            best = job.hyp[(int64_t)ny * w + nx];
+                            bcost = cst;
+                        }
+                    }
+                    // the two random refinements from the best after
+                    // propagation (dense.rs:443-462)
+                    uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
+                    const float span = job.imax - job.imin;
+#pragma unroll 1
+                    for (int k = 8; k < 10; ++k) {
+                        float4 hp;
+                        float stp = (rnd_next(s) - 0.5f) * a.range;
+                        stp = stp * 0.25f;
+                        stp = stp * span;
+                        const float id = 1.0f / best.x + stp;
+                        hp.x = (id > job.imin * 0.5f && id < job.imax * 1.5f) ? 1.0f / id : best.x;
+                        random_normal(s, rc.x, rc.y, best.y, best.z, best.w, 0.25f * a.range, hp.y,
+                                      hp.z, hp.w);
+                        const bool act = have && pixel_ok;
+                        const float cst =
+                            eval(act, rc.x, rc.y, nsr, delta, lane, hp.x, hp.y, hp.z, hp.w);
                         if (act && cst < bcost) {
                             best = hp;
                             bcost = cst;
