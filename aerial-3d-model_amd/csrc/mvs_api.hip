@@ -57,6 +57,8 @@ struct ViewDev {
     DevBuf quadh;  // corner-packed f16 plane with column differences (8-bit inputs)
     DevBuf quadf;  // corner-packed grey plane
     bool has_quad = false;
+    DevBuf quadd;  // corner-difference plane (f32 grey inputs)
+    bool has_quadd = false;
     DevBuf rgb;    // w*h*3 colour for the fused cloud (optional)
     bool has_rgb = false;
     double f = 0, cx = 0, cy = 0, k1 = 0, k2 = 0;
@@ -312,12 +314,19 @@ int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v, bool lean = false) {
                              dv.quadh.as<uint2>(), c->st);
         HIP_TRY(c, hipGetLastError());
         dv.has_quad = true;
+        dv.quadd.release();
+        dv.has_quadd = false;
     } else {
         dv.has_quad = false;
         HIP_TRY(c, hipMemcpy2DAsync(dv.gray.p, (size_t)v->w * sizeof(float), v->pixels,
                                     (size_t)v->stride * sizeof(float),
                                     (size_t)v->w * sizeof(float), v->h, hipMemcpyHostToDevice,
                                     c->st));
+        // the plane the fast one-lane-per-pixel kernels sample on f32 views
+        HIP_TRY(c, dv.quadd.ensure(n * sizeof(float4)));
+        launch_quadd(dv.gray.as<float>(), v->w, v->h, dv.quadd.as<float4>(), c->st);
+        HIP_TRY(c, hipGetLastError());
+        dv.has_quadd = true;
     }
     if (lean && dv.has_quad) {
         dv.quadf.release();
@@ -398,6 +407,7 @@ SrcCam make_src(const ViewDev &ref, const ViewDev &s) {
     c.quad = s.has_quad ? s.quad.as<uint32_t>() : nullptr;
     c.quadh = s.has_quad ? s.quadh.as<uint2>() : nullptr;
     c.quadf = s.quadf.as<float4>();
+    c.quadd = s.has_quadd ? s.quadd.as<float4>() : nullptr;
     c.w = s.w;
     c.h = s.h;
     c.pinhole_exact = (s.k1 == 0.0 && s.k2 == 0.0 && c.f_lo == 0.0f && c.cx_lo == 0.0f &&
@@ -503,6 +513,10 @@ std::vector<Schedule> candidates(const KClass &k) {
         cand.push_back(Schedule{sp[i], 1, 16});  // persistent
         cand.push_back(Schedule{sp[i], 4, 16});
     }
+    if (k.fast == 2) {
+        // f32 views: only the one-lane-per-pixel kernels on the f32 plane
+        return {Schedule{-3, 1, 16}, Schedule{-3, 1, 8}};
+    }
     if (k.fast && k.pinhole == 2) {
         cand.push_back(Schedule{-1, 1, 16});  // f16 corner plane
         cand.push_back(Schedule{-1, 1, 8});
@@ -575,6 +589,7 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
 // gathers each read one source image, on the 8-bit corner plane (scattered
 // gathers favour the smaller plane), else the first source-lane candidate.
 Schedule eval_schedule(const KClass &k) {
+    if (k.fast == 2) return Schedule{-3, 1, 0};
     if (k.fast && k.pinhole == 2) return Schedule{-2, 1, 0};
     return candidates(k)[0];
 }
@@ -738,11 +753,12 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
         c->host_tables.emplace_back();
         std::vector<SrcCam> &tab = c->host_tables.back();
-        bool pinhole = true, quads = true, lane_size = true;
+        bool pinhole = true, quads = true, quadds = ref.has_quadd, lane_size = true;
         for (const ViewDev *sv : js.srcs) {
             tab.push_back(make_src(ref, *sv));
             pinhole = pinhole && tab.back().pinhole_exact;
             quads = quads && sv->has_quad;
+            quadds = quadds && sv->has_quadd;
             // the one-lane-per-pixel kernels address a source with 32-bit byte
             // offsets into its 8-byte-per-pixel plane
             lane_size = lane_size && (int64_t)sv->w * sv->h <= (1ll << 28);
@@ -780,15 +796,23 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         job.half = (prm.taps / 2) * prm.stride;
         job.pinhole = pinhole ? 1 : 0;
         job.ray_step = (float)((double)prm.stride / ref.f);
-        // the fast kernels read the corner-packed 8-bit planes; views given
-        // as f32 grey run the exact kernels
-        const int32_t fast = (prm.mode == MVS_MODE_FAST && quads) ? 1 : 0;
         const int32_t steps = n_steps < 0 ? 2 * prm.iterations : n_steps;
         // 2: the reference is undistorted as well (its window rays are affine)
         // and 8-bit (its grey is a byte over 255)
-        const int32_t ph =
-            pinhole ? (ref.k1 == 0.0 && ref.k2 == 0.0 && ref.has_quad && lane_size ? 2 : 1) : 0;
+        const bool ref_affine = ref.k1 == 0.0 && ref.k2 == 0.0 && lane_size;
+        int32_t ph = pinhole ? (ref_affine && ref.has_quad ? 2 : 1) : 0;
         job.lane_ok = ph == 2 && quads && K <= 16 ? 1 : 0;
+        // the fast kernels read the corner-packed 8-bit planes (fast = 1);
+        // views all given as f32 grey on undistorted cameras run the
+        // one-lane-per-pixel kernels on the f32 corner-difference plane
+        // (fast = 2); other f32 views run the exact kernels
+        int32_t fast = 0;
+        if (prm.mode == MVS_MODE_FAST && quads) {
+            fast = 1;
+        } else if (prm.mode == MVS_MODE_FAST && quadds && pinhole && ref_affine && K <= 16) {
+            fast = 2;
+            ph = 2;
+        }
         // a lean view (8-bit, without the f32 corner plane) serves only the
         // fast kernels on undistorted views, which read the 8-bit and f16 planes
         bool lean = !ref.quadf.p;

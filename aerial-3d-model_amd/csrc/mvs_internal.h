@@ -28,6 +28,9 @@ struct SrcCam {
     const uint2 *quadh;
     // per pixel the grey of the same four corners (every view)
     const float4 *quadf;
+    // f32 grey inputs only (else null): per pixel {g00, g10 - g00, g01,
+    // g11 - g01}, a bilinear sample's corners and column differences
+    const float4 *quadd;
     int32_t w, h;
     int32_t pinhole_exact;
     int32_t pad;
@@ -93,6 +96,7 @@ void launch_gray_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t strid
 void launch_quad_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride,
                          uint32_t *dst, hipStream_t st);
 void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st);
+void launch_quadd(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st);
 void launch_quadh_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stride, uint2 *dst,
                           hipStream_t st);
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
@@ -111,11 +115,15 @@ enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // some source distorted, 1 undistorted sources, 2 undistorted sources and
 // reference. split = -1 selects the one-lane-per-pixel kernels (fast and
 // pinhole = 2 only) sampling the f16 corner plane, split = -2 the same kernels
-// sampling the 8-bit corner plane.
+// sampling the 8-bit corner plane, split = -3 the same kernels on f32 grey
+// views sampling the f32 corner-difference plane (fast = 2).
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
 // One lane per reference pixel (mvs_pm_lane.hip); 64 pixels per group.
-int launch_pml(int mode, int32_t taps, bool f16, bool persist, const PmArgs &a, int32_t grid,
+// plane: PL_U8 (8-bit corner plane), PL_F16 (f16 corner plane), PL_F32
+// (f32 corner-difference plane of f32 grey views).
+enum { PL_U8 = 0, PL_F16 = 1, PL_F32 = 2 };
+int launch_pml(int mode, int32_t taps, int plane, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st);
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,
                  const PmArgs &a, int32_t grid, hipStream_t st);

@@ -74,6 +74,20 @@ __global__ void k_quadh_from_u8(const uint8_t *__restrict__ src, int32_t w, int3
                         h16(r1[x]) | (h16((int)r1[x1] - (int)r1[x]) << 16));
 }
 
+// Corner-difference plane of f32 grey views: pixel (x, y) holds g(x, y),
+// g(x+1, y) - g(x, y), g(x, y+1), g(x+1, y+1) - g(x, y+1) (clamped at the
+// last column/row), the two row interpolations of a bilinear sample as two
+// multiply-adds from one 16-byte load.
+__global__ void k_quadd(const float *__restrict__ g, int32_t w, int32_t h,
+                        float4 *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    const int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
+    const int32_t x1 = x + 1 < w ? x + 1 : x, y1 = y + 1 < h ? y + 1 : y;
+    const float *r0 = g + (int64_t)y * w, *r1 = g + (int64_t)y1 * w;
+    dst[i] = make_float4(r0[x], r0[x1] - r0[x], r1[x], r1[x1] - r1[x]);
+}
+
 // Corner-packed grey plane: pixel (x, y) holds the grey of (x, y), (x+1, y),
 // (x, y+1), (x+1, y+1) (clamped at the last column/row), so one 16-byte load
 // fetches a bilinear sample's four corners with their exact values.
@@ -307,6 +321,12 @@ void launch_quadh_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stri
                        w, h, stride, dst);
 }
 
+void launch_quadd(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st) {
+    const int64_t n = (int64_t)w * h;
+    hipLaunchKernelGGL(k_quadd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gray, w, h,
+                       dst);
+}
+
 void launch_quadf(const float *gray, int32_t w, int32_t h, float4 *dst, hipStream_t st) {
     const int64_t n = (int64_t)w * h;
     hipLaunchKernelGGL(k_quadf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gray, w, h,
@@ -336,8 +356,9 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (split < 0) {
-        if (!fast || pinhole != 2 || split < -2) return -1;
-        return launch_pml(mode, taps, split == -1, persist, a, grid, st);
+        if (!fast || pinhole != 2 || split < -3) return -1;
+        return launch_pml(mode, taps, split == -1 ? PL_F16 : (split == -2 ? PL_U8 : PL_F32),
+                          persist, a, grid, st);
     }
     if (fast) {
         if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole != 0, persist, a, grid, st);

@@ -37,6 +37,10 @@ __device__ amvs_u32x2 amvs_struct_load_b64(__amdgpu_buffer_rsrc_t rsrc, int32_t 
 __device__ uint32_t amvs_struct_load_b32(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex,
                                          int32_t voffset, int32_t soffset, int32_t aux)
     __asm("llvm.amdgcn.struct.ptr.buffer.load.i32");
+typedef float amvs_f32x4 __attribute__((ext_vector_type(4)));
+__device__ amvs_f32x4 amvs_struct_load_f4(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex,
+                                          int32_t voffset, int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
 
 namespace amvs {
 namespace {
@@ -77,19 +81,23 @@ __device__ __forceinline__ float key_cost(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// F16: samples from the 8-byte f16 corner plane (fewer ALU operations per
-// sample); otherwise from the 4-byte 8-bit corner plane (half the bytes per
-// gather). Both give the same values.
-template <int TAPS, int MODE, bool PERSIST, bool F16>
+// PLANE: PL_F16 samples the 8-byte f16 corner plane (fewer ALU operations
+// per sample), PL_U8 the 4-byte 8-bit corner plane (half the bytes per
+// gather) — both give the same values, from 8-bit views, in byte units; PL_F32
+// samples the 16-byte f32 corner-difference plane of f32 grey views (the
+// `.view` cache's own grey, dense.rs:104-111), in grey units.
+template <int TAPS, int MODE, bool PERSIST, int PLANE>
 __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
     constexpr int H = TAPS / 2;
+    constexpr bool F32 = PLANE == PL_F32;
     // each lane's reference window as 8-bit values less an integer near their
-    // mean (exact in f16), tap-major so that the per-tap reads of a wavefront
-    // are free of bank conflicts; a lane reads only what it wrote, so no
-    // barrier is needed
-    __shared__ _Float16 Ah[N][64];
+    // mean (exact in f16; f32 views: the grey less the window mean, f32),
+    // tap-major so that the per-tap reads of a wavefront are free of bank
+    // conflicts
+    __shared__ _Float16 Ah[F32 ? 1 : N][64];
+    __shared__ float Af[F32 ? N : 1][64];
     // half-sweeps: the wavefront's propagation candidates as one list of
     // (pixel lane, candidate) entries, and each pixel's lowest (cost, index)
     __shared__ uint16_t plist[8 * 64];
@@ -193,7 +201,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             const float g = row[(tx - H) * stride];
                             const float d = g - mr;
                             var = var + d * d;
-                            Ah[ty * TAPS + tx][lane] = (_Float16)(rintf(g * 255.0f) - mi);
+                            if constexpr (F32)
+                                Af[ty * TAPS + tx][lane] = d;
+                            else
+                                Ah[ty * TAPS + tx][lane] = (_Float16)(rintf(g * 255.0f) - mi);
                         }
                     }
                     var = var / (float)N;
@@ -201,7 +212,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                 }
                 const bool pixel_ok = inside && !(sr < job.texture_min);
                 const float2 rc = pixel_ok ? job.rays[pidx] : make_float2(0.0f, 0.0f);
-                const float nsr = (float)N * (sr * 255.0f);  // N sd(b)
+                // N sd of the window in the samples' units; for f32 views the
+                // window's taps are centred by mr itself (delta = mr)
+                const float nsr = F32 ? (float)N * sr : (float)N * (sr * 255.0f);
+                if constexpr (F32) delta = mr;
 
                 // Cost of plane (d, n) at the pixel of lane `col` (centre ray
                 // (rcx, rcy), N sd of its bytes nsrv, byte-mean correction dlt,
@@ -285,9 +299,12 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 // (scalar registers)
                                 // (readfirstlane returns a signed int: each half
                                 // goes through uint32_t, never sign-extended)
-                                constexpr uint32_t QB = F16 ? 8u : 4u;  // bytes per pixel
-                                const uint64_t qa = F16 ? (uint64_t)(uintptr_t)S.quadh
-                                                        : (uint64_t)(uintptr_t)S.quad;
+                                constexpr uint32_t QB =
+                                    PLANE == PL_F32 ? 16u : (PLANE == PL_F16 ? 8u : 4u);
+                                const uint64_t qa =
+                                    PLANE == PL_F32   ? (uint64_t)(uintptr_t)S.quadd
+                                    : PLANE == PL_F16 ? (uint64_t)(uintptr_t)S.quadh
+                                                      : (uint64_t)(uintptr_t)S.quad;
                                 const uint64_t qlo =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
                                 const uint64_t qhi =
@@ -329,7 +346,13 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                         ay = __builtin_amdgcn_fractf(fy);
                                             const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                             float top, bot;
-                                            if constexpr (F16) {
+                                            if constexpr (PLANE == PL_F32) {
+                                                // g00, g10 - g00, g01, g11 - g01
+                                                const amvs_f32x4 e =
+                                                    amvs_struct_load_f4(Qh, idx, 0, 0, 0);
+                                                top = fmaf(ax, e.y, e.x);
+                                                bot = fmaf(ax, e.w, e.z);
+                                            } else if constexpr (PLANE == PL_F16) {
                                                 // corners and column differences, f16
                                                 const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
                                                 const uint32_t q0 = qv.x, q1 = qv.y;
@@ -352,7 +375,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                             const float dd = val - c0;
                                             s1 += dd;
                                             s2 = fmaf(dd, dd, s2);
-                                            sa = fmaf((float)Ah[ty * TAPS + tx][col], dd, sa);
+                                            if constexpr (F32)
+                                                sa = fmaf(Af[ty * TAPS + tx][col], dd, sa);
+                                            else
+                                                sa = fmaf((float)Ah[ty * TAPS + tx][col], dd, sa);
                                         }
                                     }
                                 };
@@ -363,14 +389,16 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 // NCC from the shifted sums, in byte units:
                                 // N var = s2 - s1^2 / N and
                                 // cov = sum (b - mean b) dd = sa - delta s1
+                                // (f32 views: grey units, and sa already sums
+                                // the centred window, cov = sa)
                                 const float moff = s1 * (1.0f / (float)N);
                                 const float var =
                                     fmaxf(fmaf(-s1, moff, s2), 0.0f) * (1.0f / (float)N);
-                                const float cov = fmaf(-dlt, s1, sa);
+                                const float cov = F32 ? sa : fmaf(-dlt, s1, sa);
                                 const float ss = __builtin_amdgcn_sqrtf(var);
                                 // dense.rs:246-248's 1e-3 grey units are 0.255
                                 // byte units
-                                if (!(ss < 1e-3f * 255.0f))
+                                if (!(ss < (F32 ? 1e-3f : 1e-3f * 255.0f)))
                                     cs = 1.0f - cov * __builtin_amdgcn_rcpf(nsrv * ss);
                             }
                             // best two (dense.rs:255-262)
@@ -386,15 +414,15 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
 
                 if (MODE == MODE_COST_LIST) {
                     const float4 hp = have ? a.hyp_in[item] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
-                                           hp.y, hp.z, hp.w);
+                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane,
+                                           hp.x, hp.y, hp.z, hp.w);
                     if (have) a.out[item] = cst;
                 } else if (MODE == MODE_EVAL_CURRENT) {
                     const float4 hp = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
-                                           hp.y, hp.z, hp.w);
+                    const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane,
+                                           hp.x, hp.y, hp.z, hp.w);
                     if (have) job.cost[pidx] = cst;
-                } else if constexpr (!F16) {
+                } else if constexpr (PLANE == PL_U8) {
                     // the ten proposals of dense.rs:427-462 in the reference's
                     // order (propagation in CAND order, then two random
                     // refinements), every lane in step. The 8-bit plane serves
@@ -428,8 +456,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                           hp.y, hp.z, hp.w);
                             act = have;
                         }
-                        const float cst = eval(act && pixel_ok, rc.x, rc.y, nsr, delta, lane, hp.x,
-                                               hp.y, hp.z, hp.w);
+                        const float cst = eval(act && pixel_ok, rc.x, rc.y, nsr, delta, lane,
+                                               hp.x, hp.y, hp.z, hp.w);
                         if (act && cst < bcost) {
                             best = hp;
                             bcost = cst;
@@ -504,8 +532,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         cand_pos(k, ox, oy, nx, ny);
                         const float4 hp = act ? job.hyp[(int64_t)ny * w + nx]
                                               : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                        const float cst =
-                            eval(act, orx, ory, onsr, odl, o, hp.x, hp.y, hp.z, hp.w);
+                        const float cst = eval(act, orx, ory, onsr, odl, o, hp.x,
+                                               hp.y, hp.z, hp.w);
                         if (act)
                             atomicMin(&pmin[o], ((unsigned long long)cost_key(cst) << 32) |
                                                     (unsigned long long)k);
@@ -536,8 +564,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         random_normal(s, rc.x, rc.y, best.y, best.z, best.w, 0.25f * a.range, hp.y,
                                       hp.z, hp.w);
                         const bool act = have && pixel_ok;
-                        const float cst =
-                            eval(act, rc.x, rc.y, nsr, delta, lane, hp.x, hp.y, hp.z, hp.w);
+                        const float cst = eval(act, rc.x, rc.y, nsr, delta, lane, 
+                                               hp.x, hp.y, hp.z, hp.w);
                         if (act && cst < bcost) {
                             best = hp;
                             bcost = cst;
@@ -560,38 +588,47 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     }
 }
 
-template <int TAPS, int MODE, bool PERSIST, bool F16>
+template <int TAPS, int MODE, bool PERSIST, int PLANE>
 void launch_pml_class(const PmArgs &a, int32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_pm_prep, dim3(1), dim3(64), 0, st, a, (int32_t)MODE, 64 * a.gpc);
-    hipLaunchKernelGGL((k_pml<TAPS, MODE, PERSIST, F16>), dim3((unsigned)grid), dim3(64), 0, st,
+    hipLaunchKernelGGL((k_pml<TAPS, MODE, PERSIST, PLANE>), dim3((unsigned)grid), dim3(64), 0, st,
                        a);
 }
 
-template <int TAPS, bool F16>
+template <int TAPS, int PLANE>
 void pml_taps(int mode, bool persist, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (mode == MODE_EVAL_CURRENT)
-        launch_pml_class<TAPS, MODE_EVAL_CURRENT, false, F16>(a, grid, st);
+        launch_pml_class<TAPS, MODE_EVAL_CURRENT, false, PLANE>(a, grid, st);
     else if (mode == MODE_SWEEP && persist)
-        launch_pml_class<TAPS, MODE_SWEEP, true, F16>(a, grid, st);
+        launch_pml_class<TAPS, MODE_SWEEP, true, PLANE>(a, grid, st);
     else if (mode == MODE_SWEEP)
-        launch_pml_class<TAPS, MODE_SWEEP, false, F16>(a, grid, st);
+        launch_pml_class<TAPS, MODE_SWEEP, false, PLANE>(a, grid, st);
     else
-        launch_pml_class<TAPS, MODE_COST_LIST, false, F16>(a, grid, st);
+        launch_pml_class<TAPS, MODE_COST_LIST, false, PLANE>(a, grid, st);
+}
+
+template <int TAPS>
+void pml_planes(int mode, int plane, bool persist, const PmArgs &a, int32_t grid,
+                hipStream_t st) {
+    if (plane == PL_F16)
+        pml_taps<TAPS, PL_F16>(mode, persist, a, grid, st);
+    else if (plane == PL_U8)
+        pml_taps<TAPS, PL_U8>(mode, persist, a, grid, st);
+    else
+        pml_taps<TAPS, PL_F32>(mode, persist, a, grid, st);
 }
 
 }  // namespace
 
-int launch_pml(int mode, int32_t taps, bool f16, bool persist, const PmArgs &a, int32_t grid,
+int launch_pml(int mode, int32_t taps, int plane, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st) {
+    if (plane < PL_U8 || plane > PL_F32) return -1;
     if (taps == 5)
-        f16 ? pml_taps<5, true>(mode, persist, a, grid, st)
-            : pml_taps<5, false>(mode, persist, a, grid, st);
+        pml_planes<5>(mode, plane, persist, a, grid, st);
     else if (taps == 7)
-        f16 ? pml_taps<7, true>(mode, persist, a, grid, st)
-            : pml_taps<7, false>(mode, persist, a, grid, st);
+        pml_planes<7>(mode, plane, persist, a, grid, st);
     else if (taps == 11)
-        f16 ? pml_taps<11, true>(mode, persist, a, grid, st)
-            : pml_taps<11, false>(mode, persist, a, grid, st);
+        pml_planes<11>(mode, plane, persist, a, grid, st);
     else
         return -1;
     return 0;

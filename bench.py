@@ -79,7 +79,7 @@ PEAK_HBM_GBS = 8000.0
 # profiles/k_pml_sweep_fast_8x2000x1500_k8_7x7_counters.md (fast arithmetic,
 # one lane per pixel) and profiles/k_pm_sweep_exact_8x2000x1500_k8_7x7_counters.md
 # hold the per-launch figures.
-BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (969.8, 4.99)}
+BYTES_PER_EVAL = {"exact": (2157.1, 6.59), "fast": (1096.1, 5.84)}
 
 
 # The roofline object's "bound" names the compute roofline ("mfma": the FP32
@@ -341,6 +341,58 @@ def cpu_baseline_view_prep(photo, outputs):
     return res
 
 
+def reference_grey(views, seed=0xF32):
+    """The views as the reference's `.view` grey (dense.rs:104-111): an RGB
+    photo made from each 8-bit image (per-channel offsets and noise) through
+    the grey formula of dense.rs:108 in f32, so the values are off the byte
+    lattice and only the f32 kernels can take them."""
+    import copy
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in views:
+        g = v.gray.astype(np.int16)
+        rgb = [np.clip(g + rng.integers(-12, 13, g.shape, dtype=np.int16) + off, 0,
+                       255).astype(np.float32) for off in (6, -4, 9)]
+        grey = ((np.float32(0.299) * rgb[0] + np.float32(0.587) * rgb[1])
+                + np.float32(0.114) * rgb[2]) / np.float32(255.0)
+        v2 = copy.copy(v)
+        v2.gray_f32 = grey
+        out.append(v2)
+    return out
+
+
+def f32_input_leg(ctx, surv, timed, prm):
+    """The batched refined path on f32 grey input (MVS_GRAY_F32, the `.view`
+    cache's own grey): same views, neighbours and parameters as the headline
+    run; one untimed batch tunes the schedules."""
+    from aerial_mvs import capi
+    t = time.perf_counter()
+    views = reference_grey(surv.views)
+    sv = capi.Survey(ctx, views)
+    ctx.synchronize()
+    setup = time.perf_counter() - t
+    sv.depth_maps(timed[:1], prm)
+    ctx.synchronize()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    sv.depth_maps(timed, prm)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    pixels = sum(surv.views[j[0]].w * surv.views[j[0]].h for j in timed)
+    errs = [depth_errors(sv.download(i)[0], surv.views[i].depth_gt) for (i, _, _) in timed]
+    sv.close()
+    out = dict(value=pixels / dt / 1e6, unit="depth Mpix/s", views=len(timed), seconds=dt,
+               split=st["split"], sweep_ms=st["sweep_ms"], refine_ms=st["refine_ms"],
+               input="f32 grey from RGB by dense.rs:108 (off the byte lattice), uploaded "
+                     f"as MVS_GRAY_F32 ({setup:.1f} s to make and upload)",
+               path="batched GPU, fast arithmetic, one lane per pixel on the f32 "
+                    "corner-difference plane, refinement on the two-lane kernel",
+               depth_error=mean_errors(errs))
+    log(f"f32 input: {out['value']:.2f} depth Mpix/s over {len(timed)} views")
+    return out
+
+
 def run_uneven(args):
     """The batched uneven workload: every view of an uneven survey (nadir and
     oblique views of two sizes, per-view neighbour count 3..12 and window
@@ -548,6 +600,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--no-dsm", action="store_true")
+    ap.add_argument("--no-f32", action="store_true",
+                    help="skip the leg on f32 grey input (the `.view` cache's format)")
     ap.add_argument("--grad-items", type=int, default=-1,
                     help="pixels of the full-size gradient check in the CPU leg "
                          "(default 10000 for c4, 2000 otherwise; 0 skips it)")
@@ -757,6 +811,11 @@ def main():
                       ply_bytes=int(len(xyz)) * 15)
         log(f"fusion: {len(xyz)} voxels of {voxel:.4f} m in {dtf:.2f} s")
 
+    # the same batch on the reference's own input format (f32 grey)
+    f32_leg = None
+    if not args.no_f32 and args.mode == "fast":
+        f32_leg = f32_input_leg(ctx, surv, timed, prm)
+
     # the surface model from the fused cloud (dsm.rs:128-290)
     dsm = dsm_got = None
     if fusion is not None and not args.no_dsm and len(xyz) >= 100:
@@ -811,7 +870,7 @@ def main():
         refine=dict(ms=st["refine_ms"], launches=st["refine_launches"],
                     pixels=st["refine_pixels"], accepted_steps=st["refine_accepted"]),
         graphs=dict(captures=st["graph_captures"], replays=st["graph_replays"]),
-        fusion=fusion, view_prep=prep, dsm=dsm,
+        fusion=fusion, view_prep=prep, dsm=dsm, f32_input=f32_leg,
         end_to_end=dict(upload_s=upload_s,
                         upload_gb=sum(v.w * v.h for v in surv.views) / 1e9,
                         mpix_s_with_upload=pixels / (elapsed + upload_s) / 1e6,

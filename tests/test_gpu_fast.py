@@ -257,19 +257,99 @@ def test_fast_survey_batch_matches_single_view(gpu_ctx, tiny_survey):
         sv.close()
 
 
-def test_fast_mode_with_f32_input_runs_exact_kernels(gpu_ctx, tiny_survey):
-    """The fast kernels read the corner-packed 8-bit planes; views given as f32
-    grey run the exact kernels instead, so their costs equal the oracle's."""
+def _f32_views(s, seed):
+    """The survey's views as the reference's `.view` grey: an RGB photo made
+    from each 8-bit image (per-channel offsets and noise), through the grey
+    formula of dense.rs:108 in f32 — values off the byte lattice."""
     import copy
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in s.views:
+        g = v.gray.astype(np.int32)
+        rgb = [np.clip(g + rng.integers(-12, 13, g.shape) + off, 0, 255).astype(np.float32)
+               for off in (6, -4, 9)]
+        grey = ((np.float32(0.299) * rgb[0] + np.float32(0.587) * rgb[1])
+                + np.float32(0.114) * rgb[2]) / np.float32(255.0)
+        v2 = copy.copy(v)
+        v2.gray_f32 = grey.astype(np.float32)
+        out.append(v2)
+    return out
+
+
+@pytest.mark.parametrize("name,taps,stride,k", [("tiny", 5, 2, 4), ("small", 7, 1, 8),
+                                                ("small", 11, 1, 4), ("small", 7, 1, 1)])
+def test_fast_f32_costs_within_tolerance(gpu_ctx, name, taps, stride, k):
+    """f32 grey views on undistorted cameras run the one-lane-per-pixel
+    kernels on the f32 corner-difference plane (split -3)."""
+    s = survey_cached(name)
+    views = _f32_views(s, 5 + taps + k)
+    ref = s.reference_index()
+    v, srcs = views[ref], [views[j] for j in s.neighbours[ref][:k]]
+    rng = np.random.default_rng(500 + taps + k)
+    xs, ys, hy = _hyps(v, 4000, rng)
+    want = orc.OracleDepth(v, srcs, *s.spec.depth_range, taps=taps, stride=stride).cost(xs, ys, hy)
+    gpu_ctx.reset_stats()
+    got = gpu_ctx.ncc_cost(v, srcs, *s.spec.depth_range,
+                           capi.params(taps=taps, stride=stride, fast=True), xs, ys, hy)
+    assert gpu_ctx.stats()["split"] == -3
+    dw, dg = want < 2.0, got < 2.0
+    mismatch = np.mean(dw != dg)
+    both = dw & dg
+    diff = np.abs(got[both].astype(np.float64) - want[both])
+    _say(f"fast f32 {name} {taps}x{taps} K={k}: defined {dw.mean():.3f} mismatch "
+         f"{mismatch:.5f} max|diff| {diff.max():.3g} median {np.median(diff):.3g}")
+    assert mismatch <= 0.002
+    assert diff.max() <= COST_TOL
+
+
+@pytest.mark.parametrize("name,k,iters", [("tiny", 4, 3), ("small", 8, 4)])
+def test_fast_f32_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
+    s = survey_cached(name)
+    spec = s.spec
+    views = _f32_views(s, 17 + k)
+    ref = s.reference_index()
+    v, srcs = views[ref], [views[j] for j in s.neighbours[ref][:k]]
+    od, _, _, _ = orc.OracleDepth(v, srcs, *spec.depth_range, taps=spec.taps,
+                                  stride=spec.stride).depth_map(iterations=iters)
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=iters, fast=True)
+    _say(f"fast f32 depth map {name} K={k} I={iters}")
+    gpu_ctx.reset_stats()
+    d1, _, c1 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+    st = gpu_ctx.stats()
+    d2, _, c2 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+    assert st["split"] == -3
+    assert np.array_equal(d1, d2) and np.array_equal(c1, c2)  # deterministic
+    both = (d1 > 0) & (od > 0)
+    rel = np.abs(d1[both] - od[both]) / od[both]
+    within = np.mean(rel <= 0.005)
+    valid_agree = np.mean((d1 > 0) == (od > 0))
+    _say(f"  valid both {both.mean():.3f}, valid agree {valid_agree:.4f}, within 0.5% "
+         f"{within:.5f}")
+    assert both.sum() > 1000
+    assert within >= 0.99 and valid_agree >= 0.98
+    # a survey batch of f32 views equals one-view calls
+    refs = [ref, (ref + 1) % len(views)]
+    jobs = [(r, list(s.neighbours[r][:k]), spec.depth_range) for r in refs]
+    sv = capi.Survey(gpu_ctx, views)
+    try:
+        sv.depth_maps(jobs, prm)
+        for (r, nb, rg) in jobs:
+            db, _, _ = sv.download(r)
+            d1r, _, _ = gpu_ctx.depth_map(views[r], [views[j] for j in nb], rg[0], rg[1], prm)
+            assert np.array_equal(db, d1r), r
+    finally:
+        sv.close()
+
+
+def test_fast_mode_with_distorted_f32_input_runs_exact_kernels(gpu_ctx, tiny_survey):
+    """The f32 pixel-lane kernels need undistorted cameras; distorted f32
+    views run the exact kernels, so their costs equal the oracle's."""
     s = tiny_survey
     ref = s.reference_index()
-    lut = orc.gray_lut()
+    views = _f32_views(s, 9)
+    for v in views:
+        v.k1, v.k2 = -0.02, 0.005
     rng = np.random.default_rng(9)
-    views = []
-    for v in s.views:
-        v2 = copy.copy(v)
-        v2.gray_f32 = (lut[v.gray] * 0.9 + rng.uniform(0, 0.05, v.gray.shape)).astype(np.float32)
-        views.append(v2)
     nb = s.neighbours[ref][:4]
     xs, ys, hy = _hyps(views[ref], 2000, rng)
     got = gpu_ctx.ncc_cost(views[ref], [views[j] for j in nb], 40, 80, capi.params(fast=True),
