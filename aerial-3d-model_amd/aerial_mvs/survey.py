@@ -139,6 +139,10 @@ SPECS = {
                        taps=5, stride=2, iterations=3, seed=0xA3D0000A),
     "small": SurveySpec("small", 320, 240, 900.0, 60.0, 3, 4, patch=(80.0, 60.0), neighbours=4,
                         taps=7, stride=1, iterations=4, seed=0xA3D0000B),
+    # the same with radial distortion (geometry.rs:26-32) in every camera
+    "small_dist": SurveySpec("small_dist", 320, 240, 900.0, 60.0, 3, 4, patch=(80.0, 60.0),
+                             neighbours=4, taps=7, stride=1, iterations=4, k1=-0.02, k2=0.005,
+                             seed=0xA3D0000E),
 }
 
 

@@ -813,6 +813,9 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
             fast = 2;
             ph = 2;
         }
+        // f32 views on undistorted cameras: the one-lane-per-pixel
+        // refinement on the f32 corner-difference plane, in either mode
+        if (!job.lane_ok && quadds && pinhole && ref_affine && K <= 16) job.lane_ok = 2;
         // a lean view (8-bit, without the f32 corner plane) serves only the
         // fast kernels on undistorted views, which read the 8-bit and f16 planes
         bool lean = !ref.quadf.p;

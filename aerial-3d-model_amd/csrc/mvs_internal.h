@@ -54,8 +54,10 @@ struct RefJob {
     int32_t half;         // window half footprint, pixels
     int32_t pinhole;      // every source has pinhole_exact set
     float ray_step;       // tap spacing / f of the reference camera
-    int32_t lane_ok;      // 8-bit views, undistorted reference and sources: the
-                          // one-lane-per-pixel kernels apply
+    int32_t lane_ok;      // undistorted reference and sources, at most 16: the
+                          // one-lane-per-pixel refinement applies (1: 8-bit
+                          // views, f16 corner plane; 2: f32 grey views, f32
+                          // corner-difference plane)
 };
 
 // Arguments of one batched PatchMatch launch (k_pm in mvs_pm_impl.h).

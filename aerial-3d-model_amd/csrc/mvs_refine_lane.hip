@@ -1,7 +1,10 @@
-// Gauss-Newton refinement with one lane per reference pixel, for 8-bit views
-// on undistorted cameras. It performs the same refinement as k_refine in
-// mvs_refine.hip: LM-damped Gauss-Newton on (depth, two normal angles) of the
-// 1 - NCC cost over the pair of sources that is best at the PatchMatch plane,
+// Gauss-Newton refinement with one lane per reference pixel, for views on
+// undistorted cameras: 8-bit views sample the f16 corner plane with the
+// reference window as bytes (lane_ok 1), f32 grey views the f32
+// corner-difference plane with the window in f32 (lane_ok 2). It performs
+// the same refinement as k_refine in mvs_refine.hip: LM-damped Gauss-Newton
+// on (depth, two normal angles) of the 1 - NCC cost over the pair of sources
+// that is best at the PatchMatch plane,
 // the cost the reference optimises (dense.rs:255-262). No reference counterpart:
 // the reference's depth stage stops after PatchMatch (dense.rs:468-477).
 //
@@ -21,6 +24,8 @@
 //   NCC = a.s' / (|a| |s'|),  s' = s - mean(s),
 //   dNCC/dtheta = (a.D / |a| - NCC s'.D / |s'|) / |s'|,  D_k = ds_k/dtheta,
 //   J^T J = D^T P D / |s'|^2,  P = I - 11^T/N - s's'^T/|s'|^2.
+#include <type_traits>
+
 #include "mvs_device.h"
 
 namespace amvs {
@@ -52,10 +57,10 @@ struct LaneRef {
 
 // NCC terms of source S for the plane (d, n) at this lane's pixel; NDIR = 3
 // adds the derivatives along depth and the two normal directions dirs.
-template <int TAPS, int NDIR>
+template <int TAPS, int NDIR, class WT>
 __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rstep, float d,
                                    const float n[3], const float (*dirs)[3],
-                                   const _Float16 (*Ah)[64], int lane) {
+                                   const WT (*Ah)[64], int lane) {
     constexpr int N = TAPS * TAPS, H = TAPS / 2;
     constexpr int ND = NDIR > 0 ? NDIR : 1;
     constexpr int NP = NDIR > 0 ? pk(NDIR) : 1;
@@ -101,8 +106,10 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
         rdy[q] = dv[1] * rstep;
         drc[q] = d * rdc[q];
     }
+    constexpr bool F32 = std::is_same<WT, float>::value;
     const uint32_t sw = (uint32_t)S.w;
     const u32x2 *Qp = (const u32x2 *)S.quadh;
+    const float4 *Qd = S.quadd;
     float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
     float sD[ND], sbD[ND], saD[ND], sDD[NP];
 #pragma unroll
@@ -137,11 +144,26 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
             const float fy = __builtin_amdgcn_fmed3f(fmaf(F, q1, CYh), 0.0f, hm1);
             const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
             const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
-            const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
-            const uint32_t qa = qv.x, qb = qv.y;
-            const h2 r0 = __builtin_bit_cast(h2, qa), r1 = __builtin_bit_cast(h2, qb);
-            const float top = fmaf(ax, (float)r0.y, (float)r0.x);
-            const float bot = fmaf(ax, (float)r1.y, (float)r1.x);
+            // a sample's top and bottom rows as corner and column difference
+            // (f16 bytes of 8-bit views, or f32 grey of f32 views)
+            float t0v, t0d, b0v, b0d;
+            if constexpr (F32) {
+                const float4 e = Qd[__umul24(y0, sw) + x0];
+                t0v = e.x;
+                t0d = e.y;
+                b0v = e.z;
+                b0d = e.w;
+            } else {
+                const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
+                const uint32_t qa = qv.x, qb = qv.y;
+                const h2 r0 = __builtin_bit_cast(h2, qa), r1 = __builtin_bit_cast(h2, qb);
+                t0v = (float)r0.x;
+                t0d = (float)r0.y;
+                b0v = (float)r1.x;
+                b0d = (float)r1.y;
+            }
+            const float top = fmaf(ax, t0d, t0v);
+            const float bot = fmaf(ax, b0d, b0v);
             const float val = fmaf(ay, bot - top, top);
             if (tx == 0 && ty == 0) c0 = val;
             const float sv = val - c0;  // shifted sample
@@ -152,7 +174,7 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
             if constexpr (NDIR > 0) {
                 // image gradient at the sample (per source pixel)
                 const float Iv = bot - top;
-                const float Iu = fmaf(ay, (float)r1.y - (float)r0.y, (float)r0.y);
+                const float Iu = fmaf(ay, b0d - t0d, t0d);
                 // d(u, v)/d dq and the chain to depth and the normal angles
                 const float fi = F * inv;
                 const float du = fmaf(-q0, w[2], w[0]) * fi;
@@ -225,11 +247,40 @@ __device__ bool lane_solve3(const float H[6], const float b[3], float x[3]) {
 // 8-bit view, a = b - mi into LDS as f16 integers (exact), the correction
 // delta = mean(b) - mi, |a| and the centre ray. The caller checks that the
 // window lies inside the view.
-template <int TAPS>
-__device__ LaneRef lane_ref(const RefJob &job, int32_t pix, _Float16 (*Ah)[64], int lane) {
+template <int TAPS, class WT>
+__device__ LaneRef lane_ref(const RefJob &job, int32_t pix, WT (*Ah)[64], int lane) {
     constexpr int N = TAPS * TAPS, H = TAPS / 2;
     LaneRef pr;
     const int st = job.stride, w = job.w;
+    if constexpr (std::is_same<WT, float>::value) {
+        // f32 views: the grey less its window mean, f32 (delta = 0)
+        float sg = 0.0f;
+#pragma unroll 1
+        for (int ty = 0; ty < TAPS; ++ty) {
+            const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
+#pragma unroll
+            for (int tx = 0; tx < TAPS; ++tx) sg += row[(tx - H) * st];
+        }
+        const float mg = sg * (1.0f / (float)N);
+        float s2 = 0.0f;
+#pragma unroll 1
+        for (int ty = 0; ty < TAPS; ++ty) {
+            const float *row = job.gray + pix + (int64_t)((ty - H) * st) * w;
+#pragma unroll
+            for (int tx = 0; tx < TAPS; ++tx) {
+                const float av = row[(tx - H) * st] - mg;
+                Ah[ty * TAPS + tx][lane] = av;
+                s2 = fmaf(av, av, s2);
+            }
+        }
+        pr.delta = 0.0f;
+        pr.na = sqrtf(s2);
+        const float2 rc = job.rays[pix];
+        pr.rcx = rc.x;
+        pr.rcy = rc.y;
+        pr.ok = pr.na > 1e-6f;
+        return pr;
+    }
     float sb = 0.0f;
 #pragma unroll 1
     for (int ty = 0; ty < TAPS; ++ty) {
@@ -245,7 +296,7 @@ __device__ LaneRef lane_ref(const RefJob &job, int32_t pix, _Float16 (*Ah)[64], 
 #pragma unroll
         for (int tx = 0; tx < TAPS; ++tx) {
             const float av = rintf(row[(tx - H) * st] * 255.0f) - mi;
-            Ah[ty * TAPS + tx][lane] = (_Float16)av;
+            Ah[ty * TAPS + tx][lane] = (WT)av;
             s2 = fmaf(av, av, s2);
         }
     }
@@ -262,16 +313,16 @@ __device__ LaneRef lane_ref(const RefJob &job, int32_t pix, _Float16 (*Ah)[64], 
 // The two sources best at the plane (d, n) (highest NCC first, ties to the
 // lower index, dense.rs:255-256), every source in step across the wavefront;
 // i1 >= nsrc when fewer than two are defined.
-template <int TAPS>
+template <int TAPS, class WT>
 __device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, float rstep,
-                               float d, const float n[3], const _Float16 (*Ah)[64], int lane,
+                               float d, const float n[3], const WT (*Ah)[64], int lane,
                                bool search, int &i0, int &i1) {
     float b0 = -INFINITY, b1 = -INFINITY;
     i0 = 1 << 20;
     i1 = 1 << 20;
     for (int k = 0; k < nsrc; ++k) {
         if (search) {
-            const LTerms<0> T = lane_terms<TAPS, 0>(Ss[k], pr, rstep, d, n, nullptr, Ah, lane);
+            const LTerms<0> T = lane_terms<TAPS, 0, WT>(Ss[k], pr, rstep, d, n, nullptr, Ah, lane);
             if (T.ok) {
                 if (lane_before(T.ncc, k, b0, i0)) {
                     b1 = b0;
@@ -291,13 +342,13 @@ __device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, fl
 // upper, w.r.t. depth and the angles along dirs) and J^T r of the stacked
 // residuals, and the pair cost 1 - (NCC_A + NCC_B) / 2. false where either
 // source is undefined.
-template <int TAPS>
+template <int TAPS, class WT>
 __device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr, float rstep,
                              float d, const float n[3], const float dirs[2][3],
-                             const _Float16 (*Ah)[64], int lane, float Hm[6], float g[3],
+                             const WT (*Ah)[64], int lane, float Hm[6], float g[3],
                              float &cost) {
-    const LTerms<3> TA = lane_terms<TAPS, 3>(A, pr, rstep, d, n, dirs, Ah, lane);
-    const LTerms<3> TB = lane_terms<TAPS, 3>(B, pr, rstep, d, n, dirs, Ah, lane);
+    const LTerms<3> TA = lane_terms<TAPS, 3, WT>(A, pr, rstep, d, n, dirs, Ah, lane);
+    const LTerms<3> TB = lane_terms<TAPS, 3, WT>(B, pr, rstep, d, n, dirs, Ah, lane);
     if (!(TA.ok && TB.ok)) return false;
 #pragma unroll
     for (int q = 0; q < 6; ++q) Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
@@ -312,11 +363,11 @@ __device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr
 // One lane per pixel of the job's textured-pixel lists (64 per wavefront);
 // pixels whose PatchMatch cost is below cost_max are refined in place.
 // counters (nullable): [0] += accepted steps, [1] += pixels refined.
-template <int TAPS>
+template <int TAPS, class WT>
 __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
                                                     unsigned long long *counters) {
     constexpr int N = TAPS * TAPS;
-    __shared__ _Float16 Ah[N][64];
+    __shared__ WT Ah[N][64];
     __shared__ SrcCam Ss[kMaxSrc];
     const int lane = threadIdx.x;
     const int64_t c0 = job.counts[0], total = c0 + job.counts[1];
@@ -337,13 +388,13 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     have = have && job.cost[pix] < job.cost_max && x >= job.half && y >= job.half &&
            x < w - job.half && y < job.h - job.half;
     LaneRef pr;
-    if (have) pr = lane_ref<TAPS>(job, pix, Ah, lane);
+    if (have) pr = lane_ref<TAPS, WT>(job, pix, Ah, lane);
     const float rstep = job.ray_step;
     float nrm[3] = {hp.y, hp.z, hp.w};
     // best pair at the PatchMatch plane: every source in step
     int i0, i1;
     const bool search = have && pr.ok;
-    lane_best_pair<TAPS>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, search, i0, i1);
+    lane_best_pair<TAPS, WT>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, search, i0, i1);
     bool live = search && i1 < job.nsrc;
     const bool refined = live;
     const int p0 = live ? i0 : 0, p1 = live ? i1 : 0;
@@ -360,7 +411,7 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
         tangent_dirs(nrm, dirs);
         if (!have_terms) {
             // J^T J, J^T r and the cost of the pair at the current plane
-            live = lane_pair_gn<TAPS>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g,
+            live = lane_pair_gn<TAPS, WT>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g,
                                       cost0);
             if (!live) break;
             have_terms = true;
@@ -389,8 +440,10 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
                           d1 >= lo && d1 <= hi;
         bool better = false;
         if (cand) {
-            const LTerms<0> CA = lane_terms<TAPS, 0>(Ss[p0], pr, rstep, d1, n1, nullptr, Ah, lane);
-            const LTerms<0> CB = lane_terms<TAPS, 0>(Ss[p1], pr, rstep, d1, n1, nullptr, Ah, lane);
+            const LTerms<0> CA =
+                lane_terms<TAPS, 0, WT>(Ss[p0], pr, rstep, d1, n1, nullptr, Ah, lane);
+            const LTerms<0> CB =
+                lane_terms<TAPS, 0, WT>(Ss[p1], pr, rstep, d1, n1, nullptr, Ah, lane);
             better = CA.ok && CB.ok && 1.0f - 0.5f * (CA.ncc + CB.ncc) < cost0;
         }
         if (better) {
@@ -423,14 +476,14 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
 // pair search and terms, mapped to (d, n0, n1, n2). pairs: an item given
 // (-1, -1) gets its best pair at that plane, written back. Undefined items:
 // cost 2, gradient and J^T J 0.
-template <int TAPS>
+template <int TAPS, class WT>
 __global__ __launch_bounds__(64) void k_terms_lane(RefJob job, int64_t n,
                                                    const int32_t *__restrict__ px,
                                                    const int32_t *__restrict__ py,
                                                    const float4 *__restrict__ hyp, int32_t *pairs,
                                                    float *cost, float4 *grad, float *jtj) {
     constexpr int N = TAPS * TAPS;
-    __shared__ _Float16 Ah[N][64];
+    __shared__ WT Ah[N][64];
     __shared__ SrcCam Ss[kMaxSrc];
     const int lane = threadIdx.x;
     {
@@ -448,13 +501,13 @@ __global__ __launch_bounds__(64) void k_terms_lane(RefJob job, int64_t n,
                       y < job.h - job.half;
     const int32_t pix = y * job.w + x;
     LaneRef pr;
-    if (have) pr = lane_ref<TAPS>(job, pix, Ah, lane);
+    if (have) pr = lane_ref<TAPS, WT>(job, pix, Ah, lane);
     const float rstep = job.ray_step;
     const float nrm[3] = {hp.y, hp.z, hp.w};
     int p0 = inlist ? pairs[2 * item] : 0, p1 = inlist ? pairs[2 * item + 1] : 0;
     const bool choose = p0 < 0;
     int i0, i1;
-    lane_best_pair<TAPS>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, have && pr.ok && choose,
+    lane_best_pair<TAPS, WT>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, have && pr.ok && choose,
                          i0, i1);
     if (choose) {
         p0 = i1 < job.nsrc ? i0 : -1;
@@ -465,7 +518,7 @@ __global__ __launch_bounds__(64) void k_terms_lane(RefJob job, int64_t n,
     tangent_dirs(nrm, dirs);
     bool ok = false;
     if (have && pr.ok && p0 >= 0 && p1 >= 0 && p0 != p1)
-        ok = lane_pair_gn<TAPS>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g, c);
+        ok = lane_pair_gn<TAPS, WT>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g, c);
     if (!inlist) return;
     if (choose) {
         pairs[2 * item] = p0;
@@ -488,16 +541,24 @@ void terms_lane_taps(const RefJob &job, int64_t n, const int32_t *px, const int3
                      hipStream_t st) {
     const unsigned blocks = (unsigned)((n + 63) / 64);
     if (blocks == 0) return;
-    hipLaunchKernelGGL((k_terms_lane<TAPS>), dim3(blocks), dim3(64), 0, st, job, n, px, py, hyp,
-                       pairs, cost, grad, jtj);
+    if (job.lane_ok == 2)
+        hipLaunchKernelGGL((k_terms_lane<TAPS, float>), dim3(blocks), dim3(64), 0, st, job, n, px,
+                           py, hyp, pairs, cost, grad, jtj);
+    else
+        hipLaunchKernelGGL((k_terms_lane<TAPS, _Float16>), dim3(blocks), dim3(64), 0, st, job, n,
+                           px, py, hyp, pairs, cost, grad, jtj);
 }
 
 template <int TAPS>
 void refine_lane_taps(const RefJob &job, int32_t steps, unsigned long long *counters,
                       hipStream_t st) {
     const unsigned blocks = (unsigned)((2 * (int64_t)job.list_cap + 63) / 64);
-    hipLaunchKernelGGL((k_refine_lane<TAPS>), dim3(blocks), dim3(64), 0, st, job, steps,
-                       counters);
+    if (job.lane_ok == 2)
+        hipLaunchKernelGGL((k_refine_lane<TAPS, float>), dim3(blocks), dim3(64), 0, st, job, steps,
+                           counters);
+    else
+        hipLaunchKernelGGL((k_refine_lane<TAPS, _Float16>), dim3(blocks), dim3(64), 0, st, job,
+                           steps, counters);
 }
 
 }  // namespace

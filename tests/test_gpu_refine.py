@@ -197,32 +197,37 @@ def _jtj_rel_err(H, want, depth):
     return e / np.maximum(r, 1e-30)
 
 
-TERMS_CASES = [("tiny", 4, 5, 2, False), ("small", 4, 7, 1, False), ("small", 8, 7, 1, False),
-               ("small", 4, 11, 1, False), ("small", 4, 7, 1, True), ("small", 8, 11, 1, True)]
+# view kinds: "u8" and "f32" undistorted views run the one-lane-per-pixel
+# kernel (f16 corner plane / f32 corner-difference plane), "dist" (8-bit views
+# of the distorted survey) the two-lanes-per-pixel kernel
+TERMS_CASES = [("tiny", 4, 5, 2, "u8"), ("small", 4, 7, 1, "u8"), ("small", 8, 7, 1, "u8"),
+               ("small", 4, 11, 1, "u8"), ("small", 4, 7, 1, "f32"), ("small", 8, 11, 1, "f32"),
+               ("small_dist", 4, 7, 1, "dist"), ("small_dist", 8, 11, 1, "dist")]
 
 
-@pytest.mark.parametrize("name,k,taps,stride,f32", TERMS_CASES)
-def test_refinement_terms_match_oracle(gpu_ctx, name, k, taps, stride, f32):
+@pytest.mark.parametrize("name,k,taps,stride,kind", TERMS_CASES)
+def test_refinement_terms_match_oracle(gpu_ctx, name, k, taps, stride, kind):
     """The Gauss-Newton terms the refinement kernels form — one lane per pixel
-    on 8-bit undistorted views (k_refine_lane), two lanes per pixel on f32
-    views (k_refine) — against the f64 oracle: pair cost within 3e-5, cost
-    gradient and J^T J within 0.02% at the median and 0.3% on 99% of items
-    (max-abs relative, depth and normal weighed alike), pairs chosen at the
-    plane costing no more than the f64 best pair."""
+    on undistorted views (k_refine_lane, 8-bit or f32), two lanes per pixel
+    on distorted ones (k_refine) — against the f64 oracle: pair cost within
+    3e-5, cost gradient and J^T J within 0.02% at the median and 0.3% on 99%
+    of items (max-abs relative, depth and normal weighed alike), pairs chosen
+    at the plane costing no more than the f64 best pair."""
     s, v, srcs, o = _setup(name, k, taps, stride)
+    f32 = kind == "f32"
     if f32:
         v, srcs = _as_f32([v])[0], _as_f32(srcs)
     prm = capi.params(taps=taps, stride=stride)
-    rng = np.random.default_rng(61 + taps + k + 7 * f32)
+    rng = np.random.default_rng(61 + taps + k + 7 * f32 + 13 * (kind == "dist"))
     xs, ys, hy = _items(v, 1500, rng)
     pairs = np.stack([rng.integers(0, k, len(xs)), rng.integers(0, k, len(xs))], 1)
     pairs[:, 1] = np.where(pairs[:, 0] == pairs[:, 1], (pairs[:, 1] + 1) % k, pairs[:, 1])
     pairs[: len(xs) // 4] = -1  # a quarter of the items choose their pair
     gpu_ctx.reset_stats()
-    _say(f"refine terms {name} k={k} taps={taps} {'f32' if f32 else '8-bit'}: {len(xs)} items")
+    _say(f"refine terms {name} k={k} taps={taps} {kind}: {len(xs)} items")
     cost, pq, grad, jtj = gpu_ctx.refine_terms(v, srcs, *s.spec.depth_range, prm, xs, ys, hy,
                                                pairs=pairs)
-    assert gpu_ctx.stats()["split"] == (1 if f32 else -1)  # the kernel under test
+    assert gpu_ctx.stats()["split"] == (1 if kind == "dist" else -1)  # the kernel under test
     given = pairs[:, 0] >= 0
     assert np.array_equal(pq[given], pairs[given])
     chose = ~given & (pq[:, 0] >= 0)
@@ -320,8 +325,8 @@ def test_lane_refinement_terms_match_finite_differences(gpu_ctx):
 
 def _as_f32(views):
     """The same views given as f32 grey (the table values of their bytes): the
-    refinement then runs the two-lanes-per-pixel kernel instead of the
-    one-lane-per-pixel kernel of 8-bit views."""
+    refinement then samples the f32 corner-difference plane instead of the f16
+    corner plane of 8-bit views."""
     import copy
     lut = orc.gray_lut()
     out = []
@@ -334,8 +339,11 @@ def _as_f32(views):
 
 @pytest.mark.parametrize("name,k,taps,stride,iters,f32", [("tiny", 4, 5, 2, 3, False),
                                                           ("small", 4, 7, 1, 4, False),
-                                                          ("small", 4, 7, 1, 4, True)])
+                                                          ("small", 4, 7, 1, 4, True),
+                                                          ("small_dist", 4, 7, 1, 4, False)])
 def test_refinement_lowers_cost_and_depth_error(gpu_ctx, name, k, taps, stride, iters, f32):
+    """One lane per pixel on undistorted 8-bit and f32 views, two lanes per
+    pixel on the distorted survey."""
     s, v, srcs, o = _setup(name, k, taps, stride)
     if f32:
         v, srcs = _as_f32([v])[0], _as_f32(srcs)
