@@ -502,20 +502,23 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
 
 // Candidate schedules of a class: tap split x chunk size, and for the fast
 // arithmetic on undistorted cameras the one-lane-per-pixel kernels (split -1
-// on the f16 corner plane, -2 on the 8-bit one).
+// on the f16 corner plane, -2 on the 8-bit one). A class without source lanes
+// (srcl 0: jobs of mixed neighbour counts) has only the latter.
 std::vector<Schedule> candidates(const KClass &k) {
-    int32_t sp[4];
-    const int ns = splits_for(k.taps, sp);
-    std::vector<Schedule> cand;
-    for (int i = 0; i < ns; ++i) {
-        if (k.srcl * sp[i] > 64) continue;
-        cand.push_back(Schedule{sp[i], 1, 0});   // static grid
-        cand.push_back(Schedule{sp[i], 1, 16});  // persistent
-        cand.push_back(Schedule{sp[i], 4, 16});
-    }
     if (k.fast == 2) {
         // f32 views: only the one-lane-per-pixel kernels on the f32 plane
         return {Schedule{-3, 1, 16}, Schedule{-3, 1, 8}};
+    }
+    std::vector<Schedule> cand;
+    if (k.srcl > 0) {
+        int32_t sp[4];
+        const int ns = splits_for(k.taps, sp);
+        for (int i = 0; i < ns; ++i) {
+            if (k.srcl * sp[i] > 64) continue;
+            cand.push_back(Schedule{sp[i], 1, 0});   // static grid
+            cand.push_back(Schedule{sp[i], 1, 16});  // persistent
+            cand.push_back(Schedule{sp[i], 4, 16});
+        }
     }
     if (k.fast && k.pinhole == 2) {
         cand.push_back(Schedule{-1, 1, 16});  // f16 corner plane
@@ -824,7 +827,11 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
             return fail(c, MVS_ERR_ARG,
                         "a lean view (MVS_SURVEY_LEAN_VIEWS) serves only fast-mode jobs on "
                         "undistorted 8-bit views");
-        ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_for(K), ph, fast);
+        // The one-lane-per-pixel kernels loop over each job's own sources, so
+        // fast jobs on undistorted views of any neighbour count share one class
+        // (source lanes 0) unless a source-lane schedule is forced.
+        const bool lanes_only = fast && ph == 2 && c->forced.split <= 0;
+        ckey[(size_t)i] = ClassKey(prm.taps, steps, lanes_only ? 0 : lanes_for(K), ph, fast);
     }
 
     // group jobs by kernel class; jobs of a class are contiguous in the table
@@ -929,6 +936,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
             HIP_TRY(c, hipEventRecord(ea, c->st));
         }
         TRY(sweep_launches(c, active, it, parity, evals + 1, ls));
+        c->stats.sweep_kernels += (int64_t)ls.size();
         if (c->graphs && known)
             TRY(run_sweeps_graph(c, ls));
         else

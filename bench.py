@@ -444,6 +444,7 @@ def run_uneven(args):
                               f"{spec.iterations} iterations"),
                     views=n, mode=args.mode, refine_steps=args.refine_steps, classes=kinds),
         stats=dict(sweep_ms=st["sweep_ms"], sweep_launches=st["sweep_launches"],
+                   sweep_kernels=st["sweep_kernels"],
                    init_ms=st["init_ms"], refine_ms=st["refine_ms"], evals=st["evals"],
                    graph_replays=st["graph_replays"]),
         depth_error=mean_errors(errs),

@@ -299,7 +299,8 @@ typedef struct {
     int64_t sweep_evals;      /* of which in the half-sweeps */
     int32_t split;            /* schedule of the last batch or cost list: lanes
                                * per source (-1: one lane per pixel, f16 corner
-                               * plane; -2: one lane per pixel, 8-bit plane) */
+                               * plane; -2: one lane per pixel, 8-bit plane;
+                               * -3: one lane per pixel, f32 plane) */
     int32_t gpc;              /* and pixel groups per work chunk */
     int64_t refine_launches;  /* Gauss-Newton refinement kernel */
     double refine_ms;
@@ -310,6 +311,10 @@ typedef struct {
     int64_t fuse_passes;      /* passes of mvs_survey_fuse over its views (a
                                * pass that overflows the voxel table is
                                * repeated with a larger one) */
+    int64_t sweep_kernels;    /* half-sweep kernel launches: one per kernel
+                               * class (window, projection, mode, and for the
+                               * source-lane kernels the neighbour-count
+                               * bucket) per half-sweep of a batch */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);

@@ -61,6 +61,29 @@ def test_uneven_batch_matches_single_views(gpu_ctx, fast):
         sv.close()
 
 
+def test_uneven_fast_batch_one_launch_per_window(gpu_ctx):
+    """Fast jobs on undistorted views share one persistent launch per window
+    and half-sweep, whatever their neighbour counts, sizes and strides."""
+    s, per = _uneven()
+    jobs, prms = _jobs(s, per, True)
+    windows = sorted(set(p["taps"] for p in per))
+    ks = sorted(set(p["k"] for p in per))
+    sv = capi.Survey(gpu_ctx, s.views)
+    try:
+        sv.depth_maps(jobs, prms)  # tunes the schedules
+        gpu_ctx.reset_stats()
+        sv.depth_maps(jobs, prms)
+        st = gpu_ctx.stats()
+        half_sweeps = 2 * 3
+        _say(f"fast batch: windows {windows}, K {ks}: {st['sweep_kernels']} sweep kernels "
+             f"in {half_sweeps} half-sweeps, schedule {st['split']}")
+        assert len(ks) > 2
+        assert st["sweep_kernels"] == half_sweeps * len(windows)
+        assert st["split"] in (-1, -2)
+    finally:
+        sv.close()
+
+
 def test_uneven_batch_matches_oracle(gpu_ctx):
     s, per = _uneven()
     jobs, prms = _jobs(s, per, False)
