@@ -98,8 +98,69 @@ def build_all():
     build_mvs()
 
 
+# ---- sanitizer builds of the host code (AddressSanitizer + UBSan) ----
+#
+# Host code only: every -fsanitize= goes to the host pass (-Xarch_host), the
+# GPU code is built as usual. Any finding aborts (-fno-sanitize-recover).
+# The libraries need clang's ASan runtime loaded first in the process
+# (LD_PRELOAD=sanitizer_runtime()); select them with AERIAL_MVS_LIB and
+# AERIAL_MVS_SYNTH_LIB.
+SAN_DIR = os.path.join(SRC_ROOT, "_san")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=all", "-Xarch_host", "-fno-omit-frame-pointer",
+            "-Xarch_host", "-g"]
+SAN_C = ["-fsanitize=address", "-fsanitize=undefined", "-fno-sanitize-recover=all",
+         "-fno-omit-frame-pointer", "-g"]
+CLANG = os.environ.get("AERIAL_MVS_CLANG", "/opt/rocm/llvm/bin/clang")
+
+
+def sanitizer_runtime():
+    """Path of clang's shared ASan runtime (it carries the UBSan handlers)."""
+    out = subprocess.run([CLANG, "-print-file-name=libclang_rt.asan-x86_64.so"], check=True,
+                         capture_output=True, text=True).stdout.strip()
+    if not os.path.isabs(out) or not os.path.exists(out):
+        raise RuntimeError("clang's ASan runtime (libclang_rt.asan-x86_64.so) not found")
+    return out
+
+
+def build_sanitized(out_dir=SAN_DIR, device=False):
+    """libaerial_mvs.so and libsynth.so with instrumented host code in out_dir.
+    device=False builds the host pass only (--offload-host-only: the library
+    loads and runs its host-side entry points but has no GPU kernels), which
+    is what a machine without a GPU can exercise; device=True adds the gfx950
+    code for runs on a GPU."""
+    os.makedirs(out_dir, exist_ok=True)
+    objdir = os.path.join(out_dir, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"] + SAN_HOST
+    if not device:
+        cflags.append("--offload-host-only")
+    procs = []
+    for u in UNITS:
+        obj = os.path.join(objdir, u.replace(".hip", ".o"))
+        cmd = [HIPCC] + cflags + ["-c", os.path.join(CSRC, u), "-o", obj]
+        print("+ " + " ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), obj))
+    objs = []
+    for p, obj in procs:
+        if p.wait() != 0:
+            raise RuntimeError(f"hipcc failed for {obj}")
+        objs.append(obj)
+    lib_mvs = os.path.join(out_dir, "libaerial_mvs.so")
+    link = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-fno-gpu-rdc"] + SAN_HOST
+    if not device:
+        link.append("--offload-host-only")
+    _run(link + ["-shared-libsan"] + objs + ["-o", lib_mvs])
+    lib_synth = os.path.join(out_dir, "libsynth.so")
+    c_flags = C_FLAGS + SAN_C
+    _run([CLANG] + c_flags + ["-shared-libsan", os.path.join(CSRC, "synth.c"), "-o", lib_synth,
+                              "-lm"])
+    return lib_mvs, lib_synth
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "mvs": build_mvs, "synth": build_synth,
-         "synth_gpu": build_synth_gpu}[w]()
+         "synth_gpu": build_synth_gpu, "sanitized": build_sanitized,
+         "sanitized_gpu": lambda: build_sanitized(device=True)}[w]()

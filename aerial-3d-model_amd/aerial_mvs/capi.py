@@ -11,7 +11,9 @@ import numpy as np
 
 from . import build as _build
 
-LIB_PATH = _build.LIB_MVS
+# AERIAL_MVS_LIB selects another build of the same library (the sanitizer
+# build of the host code, for instance)
+LIB_PATH = os.environ.get("AERIAL_MVS_LIB") or _build.LIB_MVS
 
 MVS_OK = 0
 MVS_NO_NEIGHBOURS = 1

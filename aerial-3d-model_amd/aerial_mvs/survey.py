@@ -27,9 +27,10 @@ _lib = None
 def _synth():
     global _lib
     if _lib is None:
-        if not os.path.exists(_build.LIB_SYNTH):
+        path = os.environ.get("AERIAL_MVS_SYNTH_LIB") or _build.LIB_SYNTH
+        if not os.path.exists(path):
             raise RuntimeError("libsynth.so not built")
-        L = C.CDLL(_build.LIB_SYNTH)
+        L = C.CDLL(path)
         L.synth_scene_create.argtypes = [C.c_double] * 6 + [C.c_int32, C.c_double, C.c_double,
                                                             C.c_uint64, C.c_double, C.c_int32,
                                                             C.c_int32]

@@ -26,5 +26,19 @@ def build():
     return LIB
 
 
+def build_sanitized(out_dir, clang="/opt/rocm/llvm/bin/clang"):
+    """The same library with AddressSanitizer + UBSan (clang, shared runtime;
+    load clang's libclang_rt.asan first), into out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+    lib = os.path.join(out_dir, "liboracle.so")
+    cmd = [clang] + FLAGS + ["-fsanitize=address", "-fsanitize=undefined",
+                             "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g",
+                             "-shared-libsan", os.path.join(HERE, "dense_oracle.c"), "-o", lib,
+                             "-lm"]
+    print("+ " + " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return lib
+
+
 if __name__ == "__main__":
     build()
