@@ -11,9 +11,11 @@ rules (buffers indexed through checked slices, dense.rs:416-417); this is the
 check that stands in for them on the C and C++ side.
 
 The GPU-side host paths (batches, graphs, fusion passes) need a device and are
-not exercised here; `python aerial-3d-model_amd/aerial_mvs/build.py
-sanitized_gpu` builds the same instrumentation with the gfx950 code for runs
-on a GPU.
+not exercised here. `python aerial-3d-model_amd/aerial_mvs/build.py
+sanitized_gpu` builds the same instrumentation; a GPU run of it (ROCm 7.2,
+ASan runtime preloaded ahead of the uninstrumented HIP runtime) aborted inside
+the first device initialisation (mvs_create) without a sanitizer report, so
+those paths stay unsanitized.
 """
 import importlib.util
 import os
