@@ -297,7 +297,9 @@ int check_params(mvs_ctx *c, const mvs_params *p) {
 int upload_view(mvs_ctx *c, ViewDev &dv, const mvs_view *v, bool lean = false) {
     TRY(check_view(c, v, "view"));
     const size_t n = (size_t)v->w * v->h;
-    HIP_TRY(c, dv.gray.ensure(n * sizeof(float)));
+    const size_t pad = (size_t)grey_pad(v->w);
+    HIP_TRY(c, dv.gray.ensure((n + pad) * sizeof(float)));
+    HIP_TRY(c, hipMemsetAsync(dv.gray.as<float>() + n, 0, pad * sizeof(float), c->st));
     if (v->format == MVS_GRAY_U8) {
         const size_t bytes = (size_t)v->stride * v->h;
         HIP_TRY(c, c->up_u8.ensure(bytes));
@@ -506,8 +508,10 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
 // (srcl 0: jobs of mixed neighbour counts) has only the latter.
 std::vector<Schedule> candidates(const KClass &k) {
     if (k.fast == 2) {
-        // f32 views: only the one-lane-per-pixel kernels on the f32 plane
-        return {Schedule{-3, 1, 16}, Schedule{-3, 1, 8}};
+        // f32 views: only the one-lane-per-pixel kernels, on the f32
+        // corner-difference plane or the grey plane (identical values)
+        return {Schedule{-3, 1, 16}, Schedule{-3, 1, 8}, Schedule{-4, 1, 16},
+                Schedule{-4, 1, 8}};
     }
     std::vector<Schedule> cand;
     if (k.srcl > 0) {
@@ -590,9 +594,10 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
 // Schedule of the initial-cost evaluation (one cost per pixel, scattered
 // random planes): the one-lane-per-pixel kernels where they apply, whose
 // gathers each read one source image, on the 8-bit corner plane (scattered
-// gathers favour the smaller plane), else the first source-lane candidate.
+// gathers favour the smaller plane; f32 views: the grey plane itself), else
+// the first source-lane candidate.
 Schedule eval_schedule(const KClass &k) {
-    if (k.fast == 2) return Schedule{-3, 1, 0};
+    if (k.fast == 2) return Schedule{-4, 1, 0};
     if (k.fast && k.pinhole == 2) return Schedule{-2, 1, 0};
     return candidates(k)[0];
 }
@@ -752,7 +757,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         HIP_TRY(c, wk.cost.ensure((size_t)n * sizeof(float)));
         HIP_TRY(c, wk.textured.ensure((size_t)n));
         HIP_TRY(c, wk.list.ensure((size_t)(2 * cap) * sizeof(int32_t)));
-        HIP_TRY(c, wk.counts.ensure(4 * sizeof(int32_t)));
+        HIP_TRY(c, wk.counts.ensure((4 + 2 * (size_t)list_waves(ref.w, ref.h)) * sizeof(int32_t)));
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
         c->host_tables.emplace_back();
         std::vector<SrcCam> &tab = c->host_tables.back();
@@ -1030,10 +1035,11 @@ int abort_call(mvs_ctx *c, int rc) {
     return rc;
 }
 
-// Work-buffer bytes of one job slot (hyp, cost, mask, lists, sources).
+// Work-buffer bytes of one job slot (hyp, cost, mask, lists, list counts,
+// sources).
 int64_t slot_bytes(const ViewDev &v) {
     const int64_t n = (int64_t)v.w * v.h;
-    return n * (16 + 4 + 1 + 4) + 16 * (int64_t)sizeof(SrcCam) + 64;
+    return n * (16 + 4 + 1 + 4) + 8 * list_waves(v.w, v.h) + 16 * (int64_t)sizeof(SrcCam) + 64;
 }
 
 }  // namespace
@@ -1161,8 +1167,9 @@ int mvs_synchronize(mvs_ctx *c) {
 
 int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune) {
-    if (!c || split < -2 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
-        return MVS_ERR_ARG;
+    if (!c) return MVS_ERR_ARG;
+    if (split < -4 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
+        return fail(c, MVS_ERR_ARG, "schedule out of range (split >= -4, gpc and per_cu 0..1024)");
     c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;
     return MVS_OK;

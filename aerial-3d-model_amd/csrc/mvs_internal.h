@@ -104,8 +104,10 @@ void launch_quadh_from_u8(const uint8_t *src, int32_t w, int32_t h, int32_t stri
 void launch_rays(double f, double cx, double cy, double k1, double k2, int32_t w, int32_t h,
                  float2 *rays, hipStream_t st);
 // write_hyp = 0 keeps a caller-provided state and only rebuilds mask + lists.
-// Also zeroes the job's list counters first.
+// job.counts holds 4 + 2 list_waves(w, h) ints: the list lengths, then the
+// per-wavefront counts and offsets the lists are built from.
 void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t st);
+int64_t list_waves(int32_t w, int32_t h);
 // One batched PatchMatch launch (prep + persistent kernel). mode: 0 evaluate
 // the current hypotheses, 1 half-sweep, 2 explicit cost list. srcl: source
 // lanes per pixel (4, 8, 16); split: lanes per source. Returns -1 for a class
@@ -118,13 +120,19 @@ enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // reference. split = -1 selects the one-lane-per-pixel kernels (fast and
 // pinhole = 2 only) sampling the f16 corner plane, split = -2 the same kernels
 // sampling the 8-bit corner plane, split = -3 the same kernels on f32 grey
-// views sampling the f32 corner-difference plane (fast = 2).
+// views sampling the f32 corner-difference plane (fast = 2), split = -4 the
+// same on f32 grey views sampling the grey plane itself (two row loads per
+// sample, a quarter of the bytes per pixel; same values as split = -3).
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
 // One lane per reference pixel (mvs_pm_lane.hip); 64 pixels per group.
 // plane: PL_U8 (8-bit corner plane), PL_F16 (f16 corner plane), PL_F32
-// (f32 corner-difference plane of f32 grey views).
-enum { PL_U8 = 0, PL_F16 = 1, PL_F32 = 2 };
+// (f32 corner-difference plane of f32 grey views), PL_F32G (the f32 grey plane
+// itself, read as two 8-byte row pairs per sample).
+enum { PL_U8 = 0, PL_F16 = 1, PL_F32 = 2, PL_F32G = 3 };
+// Zero floats after a view's grey plane (w*h floats): a row-pair read at the
+// last column or row, weighted 0, stays inside the allocation on finite data.
+__host__ __device__ inline int64_t grey_pad(int32_t w) { return (int64_t)w + 64; }
 int launch_pml(int mode, int32_t taps, int plane, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st);
 int launch_pm_t5(int mode, int32_t srcl, int32_t split, bool pinhole, bool persist,

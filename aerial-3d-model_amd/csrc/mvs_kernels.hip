@@ -121,14 +121,16 @@ __global__ void k_rays(double f, double cx, double cy, double k1, double k2, int
 }
 
 // The threads of k_init walk the view in tiles of kListTileW x kListTileH
-// pixels (tiles row-major, pixels row-major inside a tile), so the parity
-// lists come out tile by tile and the sweeps, which take list items in order,
-// have a compact square of pixels in flight on each XCD instead of a band of
-// full rows: while the hypotheses are still scattered, the source lines a
-// square reaches along its epipolar segments are a fraction of a band's. Each
-// wavefront covers 64 pixels of one tile row (kListTileW is a multiple of 64),
-// so a list run is still 32 pixels of one row. The order changes no result:
-// every list reader takes (x, y) from the stored index.
+// pixels (tiles row-major, pixels row-major inside a tile), and the parity
+// lists hold the textured pixels in exactly that order (per-wavefront counts,
+// one scan, then a fill in walk order): the sweeps, which take list items in
+// order, have a compact square of pixels in flight on each XCD instead of a
+// band of full rows (while the hypotheses are still scattered, the source
+// lines a square reaches along its epipolar segments are a fraction of a
+// band's), and each 64-item chunk is the next 64 textured pixels of one
+// parity along a tile row, so the wavefront's windows fall in or out of each
+// source's coverage together. The order changes no result: every list reader
+// takes (x, y) from the stored index.
 #ifndef AMVS_LIST_TILE_W
 #define AMVS_LIST_TILE_W 128
 #endif
@@ -139,12 +141,10 @@ constexpr int32_t kListTileW = AMVS_LIST_TILE_W;
 constexpr int32_t kListTileH = AMVS_LIST_TILE_H;
 static_assert(kListTileW % 64 == 0 && kListTileH > 0, "list tiles hold whole wavefront rows");
 
-// Initial hypotheses (dense.rs:365-381), texture mask (dense.rs:383-402) and
-// the per-parity lists of textured pixels the sweeps iterate over; thread t
-// takes pixel (x, y) of the tiled walk above.
-__global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t write_hyp) {
-    const int32_t w = job.w, h = job.h;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Pixel (x, y) of position t of the tiled walk; false past the view's edge
+// (x = y = 0 then).
+__device__ __forceinline__ bool walk_pixel(int32_t w, int32_t h, int64_t t, int32_t &x,
+                                           int32_t &y) {
     const int32_t ntx = (w + kListTileW - 1) / kListTileW;
     const int64_t tile = t / (kListTileW * kListTileH);
     const int32_t r = (int32_t)(t - tile * (kListTileW * kListTileH));
@@ -153,8 +153,19 @@ __global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t 
     const int32_t yt = ty * kListTileH + r / kListTileW;
     const int32_t xt = tx * kListTileW + r % kListTileW;
     const bool valid = xt < w && yt < h;
-    const int32_t y = valid ? yt : 0;
-    const int32_t x = valid ? xt : 0;
+    y = valid ? yt : 0;
+    x = valid ? xt : 0;
+    return valid;
+}
+
+// Initial hypotheses (dense.rs:365-381), texture mask (dense.rs:383-402) and
+// each wavefront's count of textured pixels per parity (counts[4 + 2 wave +
+// parity]); thread t takes pixel (x, y) of the tiled walk above.
+__global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t write_hyp) {
+    const int32_t w = job.w, h = job.h;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t x, y;
+    const bool valid = walk_pixel(w, h, t, x, y);
     const int64_t i = (int64_t)y * w + x;
     bool tex = false;
     if (valid) {
@@ -189,22 +200,69 @@ __global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t 
         }
         job.textured[i] = tex ? 1 : 0;
     }
-    // Wave-aggregated append to the parity lists.
     const int parity = (x + y) & 1;
     const uint64_t m0 = __ballot(valid && tex && parity == 0);
     const uint64_t m1 = __ballot(valid && tex && parity == 1);
-    const int lane = threadIdx.x & 63;
-    int b0 = 0, b1 = 0;
-    if (lane == 0) {
-        if (m0) b0 = atomicAdd(&job.counts[0], __popcll(m0));
-        if (m1) b1 = atomicAdd(&job.counts[1], __popcll(m1));
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t wave = t >> 6;
+        job.counts[4 + 2 * wave] = __popcll(m0);
+        job.counts[5 + 2 * wave] = __popcll(m1);
     }
-    b0 = __shfl(b0, 0);
-    b1 = __shfl(b1, 0);
-    if (valid && tex) {
+}
+
+// Exclusive scan of the per-wavefront counts in walk order (one workgroup),
+// in place; the list lengths go to counts[0] (parity 0) and counts[1].
+__global__ __launch_bounds__(1024) void k_list_scan(int32_t *counts, int64_t nwaves) {
+    __shared__ int32_t part[2][1024];
+    const int tid = (int)threadIdx.x;
+    const int64_t per = (nwaves + 1023) / 1024;
+    const int64_t b = tid * per, e = b + per < nwaves ? b + per : nwaves;
+    int32_t s0 = 0, s1 = 0;
+    for (int64_t i = b; i < e; ++i) {
+        s0 += counts[4 + 2 * i];
+        s1 += counts[5 + 2 * i];
+    }
+    part[0][tid] = s0;
+    part[1][tid] = s1;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t a0 = tid >= off ? part[0][tid - off] : 0;
+        const int32_t a1 = tid >= off ? part[1][tid - off] : 0;
+        __syncthreads();
+        part[0][tid] += a0;
+        part[1][tid] += a1;
+        __syncthreads();
+    }
+    int32_t o0 = tid ? part[0][tid - 1] : 0, o1 = tid ? part[1][tid - 1] : 0;
+    for (int64_t i = b; i < e; ++i) {
+        const int32_t c0 = counts[4 + 2 * i], c1 = counts[5 + 2 * i];
+        counts[4 + 2 * i] = o0;
+        counts[5 + 2 * i] = o1;
+        o0 += c0;
+        o1 += c1;
+    }
+    if (tid == 1023) {
+        counts[0] = part[0][1023];
+        counts[1] = part[1][1023];
+    }
+}
+
+// The parity lists: each wavefront writes its textured pixels of each parity
+// at its scanned offset, in walk order.
+__global__ __launch_bounds__(256) void k_list_fill(RefJob job) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t x, y;
+    const bool valid = walk_pixel(job.w, job.h, t, x, y);
+    const int64_t i = (int64_t)y * job.w + x;
+    const bool tex = valid && job.textured[i] != 0;
+    const int parity = (x + y) & 1;
+    const uint64_t m0 = __ballot(tex && parity == 0);
+    const uint64_t m1 = __ballot(tex && parity == 1);
+    if (tex) {
+        const int lane = (int)(threadIdx.x & 63);
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const uint64_t m = parity ? m1 : m0;
-        const int pos = (parity ? b1 : b0) + __popcll(m & below);
+        const int64_t wave = t >> 6;
+        const int32_t pos = job.counts[4 + 2 * wave + parity] + __popcll((parity ? m1 : m0) & below);
         job.list[(int64_t)parity * job.list_cap + pos] = (int32_t)i;
     }
 }
@@ -344,21 +402,30 @@ __global__ void k_zero_i32(int32_t *p, int32_t n) {
     if ((int32_t)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+int64_t list_waves(int32_t w, int32_t h) {
+    const int64_t ntx = (w + kListTileW - 1) / kListTileW;
+    const int64_t nty = (h + kListTileH - 1) / kListTileH;
+    return ntx * nty * (kListTileW * kListTileH / 64);  // tiles cover the view
+}
+
 void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t st) {
-    const int64_t ntx = (job.w + kListTileW - 1) / kListTileW;
-    const int64_t nty = (job.h + kListTileH - 1) / kListTileH;
-    const int64_t n = ntx * nty * kListTileW * kListTileH;  // tiles cover the view
+    const int64_t nw = list_waves(job.w, job.h), n = 64 * nw;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, job.counts, 4);
     hipLaunchKernelGGL(k_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, job, taps,
                        write_hyp);
+    hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, st, job.counts, nw);
+    hipLaunchKernelGGL(k_list_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, job);
 }
 
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (split < 0) {
-        if (!fast || pinhole != 2 || split < -3) return -1;
-        return launch_pml(mode, taps, split == -1 ? PL_F16 : (split == -2 ? PL_U8 : PL_F32),
-                          persist, a, grid, st);
+        if (!fast || pinhole != 2 || split < -4) return -1;
+        const int plane = split == -1 ? PL_F16
+                          : split == -2 ? PL_U8
+                          : split == -3 ? PL_F32
+                                        : PL_F32G;
+        return launch_pml(mode, taps, plane, persist, a, grid, st);
     }
     if (fast) {
         if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole != 0, persist, a, grid, st);

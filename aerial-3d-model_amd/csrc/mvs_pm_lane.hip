@@ -41,6 +41,10 @@ typedef float amvs_f32x4 __attribute__((ext_vector_type(4)));
 __device__ amvs_f32x4 amvs_struct_load_f4(__amdgpu_buffer_rsrc_t rsrc, int32_t vindex,
                                           int32_t voffset, int32_t soffset, int32_t aux)
     __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+// Raw buffer load: the address is base + voffset + soffset (bytes).
+__device__ amvs_u32x2 amvs_raw_load_b64(__amdgpu_buffer_rsrc_t rsrc, int32_t voffset,
+                                        int32_t soffset, int32_t aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v2i32");
 
 namespace amvs {
 namespace {
@@ -85,13 +89,16 @@ __device__ __forceinline__ float key_cost(uint32_t k) {
 // per sample), PL_U8 the 4-byte 8-bit corner plane (half the bytes per
 // gather) — both give the same values, from 8-bit views, in byte units; PL_F32
 // samples the 16-byte f32 corner-difference plane of f32 grey views (the
-// `.view` cache's own grey, dense.rs:104-111), in grey units.
+// `.view` cache's own grey, dense.rs:104-111), in grey units; PL_F32G reads
+// the same views' 4-byte grey plane as two row pairs per sample (the same
+// corner values and column differences, so the same costs, from a quarter of
+// the plane bytes: the scattered first iterations miss the caches less).
 template <int TAPS, int MODE, bool PERSIST, int PLANE>
 __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
     constexpr int H = TAPS / 2;
-    constexpr bool F32 = PLANE == PL_F32;
+    constexpr bool F32 = PLANE == PL_F32 || PLANE == PL_F32G;
     // each lane's reference window as 8-bit values less an integer near their
     // mean (exact in f16; f32 views: the grey less the window mean, f32),
     // tap-major so that the per-tap reads of a wavefront are free of bank
@@ -302,17 +309,27 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 constexpr uint32_t QB =
                                     PLANE == PL_F32 ? 16u : (PLANE == PL_F16 ? 8u : 4u);
                                 const uint64_t qa =
-                                    PLANE == PL_F32   ? (uint64_t)(uintptr_t)S.quadd
-                                    : PLANE == PL_F16 ? (uint64_t)(uintptr_t)S.quadh
-                                                      : (uint64_t)(uintptr_t)S.quad;
+                                    PLANE == PL_F32    ? (uint64_t)(uintptr_t)S.quadd
+                                    : PLANE == PL_F16  ? (uint64_t)(uintptr_t)S.quadh
+                                    : PLANE == PL_F32G ? (uint64_t)(uintptr_t)S.gray
+                                                       : (uint64_t)(uintptr_t)S.quad;
                                 const uint64_t qlo =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
                                 const uint64_t qhi =
                                     (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
-                                // records of QB bytes, one per pixel
+                                // records of QB bytes, one per pixel (PL_F32G:
+                                // a raw buffer of the grey plane and its zero
+                                // padding, in bytes)
                                 const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
-                                    (void *)(uintptr_t)(qlo | (qhi << 32)), (short)QB,
-                                    __builtin_amdgcn_readfirstlane(S.w * S.h), kBufferWord3);
+                                    (void *)(uintptr_t)(qlo | (qhi << 32)),
+                                    (short)(PLANE == PL_F32G ? 0u : QB),
+                                    __builtin_amdgcn_readfirstlane(
+                                        PLANE == PL_F32G
+                                            ? (int32_t)(4 * (S.w * S.h + (int32_t)grey_pad(S.w)))
+                                            : S.w * S.h),
+                                    kBufferWord3);
+                                // bytes per grey row (PL_F32G)
+                                const int32_t row4 = __builtin_amdgcn_readfirstlane(4 * S.w);
                                 float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
                                 // The window's taps; CLAMP keeps every position
                                 // in the sampling range whatever the rounding
@@ -344,15 +361,34 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                             const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
                                             const float ax = __builtin_amdgcn_fractf(fx),
                                                         ay = __builtin_amdgcn_fractf(fy);
-                                            const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                             float top, bot;
-                                            if constexpr (PLANE == PL_F32) {
+                                            if constexpr (PLANE == PL_F32G) {
+                                                // g00, g10 and g01, g11 from the
+                                                // rows y0 and y0 + 1; at the last
+                                                // column or row the pair's second
+                                                // value (weight 0) is the next
+                                                // row's first or the zero padding
+                                                const int32_t off = (int32_t)(
+                                                    __umul24(y0, (uint32_t)row4) + (x0 << 2));
+                                                const amvs_u32x2 r0 =
+                                                    amvs_raw_load_b64(Qh, off, 0, 0);
+                                                const amvs_u32x2 r1 =
+                                                    amvs_raw_load_b64(Qh, off, row4, 0);
+                                                const float g00 = __uint_as_float(r0.x);
+                                                const float g10 = __uint_as_float(r0.y);
+                                                const float g01 = __uint_as_float(r1.x);
+                                                const float g11 = __uint_as_float(r1.y);
+                                                top = fmaf(ax, g10 - g00, g00);
+                                                bot = fmaf(ax, g11 - g01, g01);
+                                            } else if constexpr (PLANE == PL_F32) {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // g00, g10 - g00, g01, g11 - g01
                                                 const amvs_f32x4 e =
                                                     amvs_struct_load_f4(Qh, idx, 0, 0, 0);
                                                 top = fmaf(ax, e.y, e.x);
                                                 bot = fmaf(ax, e.w, e.z);
                                             } else if constexpr (PLANE == PL_F16) {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // corners and column differences, f16
                                                 const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
                                                 const uint32_t q0 = qv.x, q1 = qv.y;
@@ -361,6 +397,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                 top = fmaf(ax, (float)r0.y, (float)r0.x);
                                                 bot = fmaf(ax, (float)r1.y, (float)r1.x);
                                             } else {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // the four corners as bytes
                                                 const uint32_t qd = amvs_struct_load_b32(Qh, idx, 0, 0, 0);
                                                 const float g00 = (float)(qd & 0xffu);
@@ -614,15 +651,17 @@ void pml_planes(int mode, int plane, bool persist, const PmArgs &a, int32_t grid
         pml_taps<TAPS, PL_F16>(mode, persist, a, grid, st);
     else if (plane == PL_U8)
         pml_taps<TAPS, PL_U8>(mode, persist, a, grid, st);
-    else
+    else if (plane == PL_F32)
         pml_taps<TAPS, PL_F32>(mode, persist, a, grid, st);
+    else
+        pml_taps<TAPS, PL_F32G>(mode, persist, a, grid, st);
 }
 
 }  // namespace
 
 int launch_pml(int mode, int32_t taps, int plane, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st) {
-    if (plane < PL_U8 || plane > PL_F32) return -1;
+    if (plane < PL_U8 || plane > PL_F32G) return -1;
     if (taps == 5)
         pml_planes<5>(mode, plane, persist, a, grid, st);
     else if (taps == 7)

@@ -10,7 +10,7 @@ refinement + threshold).
 Three paths:
   - batched+refined (the headline `value`): all K views as one batch - every
     half-sweep of all views is one persistent launch, run as a cached graph -
-    with the fast-arithmetic kernels (costs within 1e-5 of the exact ones) and
+    with the fast-arithmetic kernels (costs within 2e-5 of the exact ones) and
     5 Gauss-Newton steps, plus the consistency filter, images resident in HBM;
   - plain: one view per call with the exact kernels and no refinement - the
     bit-identical drop-in of the reference's algorithm - on a subset;
@@ -71,6 +71,8 @@ def default_threads():
     if h["omp_num_threads"]:
         n = min(n, h["omp_num_threads"])
     return max(1, n)
+
+
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 # HBM-side traffic of the half-sweep kernel per cost evaluation, from rocprofv3
@@ -387,7 +389,11 @@ def f32_input_leg(ctx, surv, timed, prm):
                input="f32 grey from RGB by dense.rs:108 (off the byte lattice), uploaded "
                      f"as MVS_GRAY_F32 ({setup:.1f} s to make and upload)",
                path="batched GPU, fast arithmetic, one lane per pixel on the f32 "
-                    "corner-difference plane, refinement on the two-lane kernel",
+                    "corner-difference plane or the grey plane (per iteration, same costs), "
+                    "refinement one lane per pixel",
+               schedule=[dict(iteration=t["iteration"], split=t["split"], gpc=t["gpc"],
+                              per_cu=t["per_cu"])
+                         for t in ctx.tuned_schedules() if t["mode"] == 2],
                depth_error=mean_errors(errs))
     log(f"f32 input: {out['value']:.2f} depth Mpix/s over {len(timed)} views")
     return out

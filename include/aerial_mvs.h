@@ -324,8 +324,10 @@ int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
  * split = lanes sharing one source's window (0 = choose automatically; -1 =
  * one lane per reference pixel, for MVS_MODE_FAST with undistorted 8-bit
  * reference and source views, sampling an f16 corner plane; -2 = the same
- * sampling the 8-bit corner plane; other classes take their first
- * candidate), gpc = pixel
+ * sampling the 8-bit corner plane; -3 = the same for undistorted f32 grey
+ * views, sampling the f32 corner-difference plane; -4 = the same sampling
+ * the f32 grey plane itself; a class the forced split does not apply to
+ * takes its first candidate), gpc = pixel
  * groups per work chunk (0 = default), per_cu = persistent workgroups per
  * compute unit (0 = one workgroup per chunk). autotune = 1 times the candidate
  * schedules per kernel class and iteration on the first batch and keeps the

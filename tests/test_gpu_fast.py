@@ -149,16 +149,20 @@ def _oblique():
 _OBL = {}
 
 
+@pytest.mark.parametrize("grey", ["u8", "f32"])
 @pytest.mark.parametrize("name,taps,stride,k", [("tiny", 5, 2, 4), ("small", 7, 1, 8),
                                                 ("small", 11, 1, 8), ("oblique", 7, 1, 6),
                                                 ("oblique", 11, 1, 6)])
-def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k):
-    """The 8-bit and the f16 corner planes hold the same corner values, so the
-    one-lane-per-pixel kernels give the same costs, bit for bit, on either
-    (split -2: 8-bit plane, bytes unpacked and differenced per sample; split
-    -1: f16 plane with the column differences stored). Hypotheses near the
-    surface and scattered, on nadir views and on an oblique view whose sources
-    see its windows at scales far from one."""
+def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k, grey):
+    """The planes of one view hold the same corner values, so the
+    one-lane-per-pixel kernels give the same costs, bit for bit, on either.
+    8-bit views: split -2 (8-bit plane, bytes unpacked and differenced per
+    sample) and split -1 (f16 plane with the column differences stored). f32
+    views: split -3 (f32 corner-difference plane) and split -4 (the grey plane,
+    two row pairs per sample, differenced per sample; windows at the last
+    column or row read the next row or the zero padding at weight 0).
+    Hypotheses near the surface and scattered, on nadir views and on an
+    oblique view whose sources see its windows at scales far from one."""
     if name == "oblique":
         s, ref, rng_d = _oblique()
         nb = list(s.neighbours[ref][1:k + 1]) if s.neighbours[ref][0] == ref else \
@@ -168,8 +172,10 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k):
         ref = s.reference_index()
         nb = s.neighbours[ref][:k]
         rng_d = s.spec.depth_range
-    v = s.views[ref]
-    srcs = [s.views[j] for j in nb]
+    views = s.views if grey == "u8" else _f32_views(s, 31 + taps + k)
+    v = views[ref]
+    srcs = [views[j] for j in nb]
+    splits = (-1, -2) if grey == "u8" else (-3, -4)
     rng = np.random.default_rng(700 + taps + k)
     n = 30000  # the oblique view's costs are defined for ~3% of random items
     xs = rng.integers(0, v.w, n).astype(np.int32)
@@ -184,41 +190,48 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k):
     hy[:, 1:] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
     prm = capi.params(taps=taps, stride=stride, fast=True)
     got = {}
-    _say(f"planes {name} {taps}x{taps} K={k}: view {ref} {v.w}x{v.h}, sources {nb}")
+    _say(f"planes {grey} {name} {taps}x{taps} K={k}: view {ref} {v.w}x{v.h}, sources {nb}")
     try:
-        for split in (-1, -2):
+        for split in splits:
             gpu_ctx.set_schedule(split, 1, 16, False)
             gpu_ctx.reset_stats()
             got[split] = gpu_ctx.ncc_cost(v, srcs, *rng_d, prm, xs, ys, hy)
             assert gpu_ctx.stats()["split"] == split
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
-    defined = got[-1] < 2.0
-    _say(f"  defined {defined.mean():.3f}, identical {np.mean(got[-1] == got[-2]):.6f}")
+    a, b = got[splits[0]], got[splits[1]]
+    defined = a < 2.0
+    _say(f"  defined {defined.mean():.3f}, identical {np.mean(a == b):.6f}")
     assert defined.sum() >= 800
-    assert np.array_equal(got[-1], got[-2])
+    assert np.array_equal(a, b)
 
 
-def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx):
-    """Whole depth maps (initial costs, every half-sweep) on the 8-bit plane
-    equal those on the f16 plane."""
+@pytest.mark.parametrize("grey", ["u8", "f32"])
+def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
+    """Whole depth maps (initial costs, every half-sweep) on one plane equal
+    those on the other: 8-bit and f16 planes of 8-bit views, corner-difference
+    and grey planes of f32 views."""
     s = survey_cached("small")
     spec = s.spec
     ref = s.reference_index()
-    v = s.views[ref]
-    srcs = [s.views[j] for j in s.neighbours[ref][:8]]
+    views = s.views if grey == "u8" else _f32_views(s, 41)
+    v = views[ref]
+    srcs = [views[j] for j in s.neighbours[ref][:8]]
     prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=4, fast=True)
+    splits = (-1, -2) if grey == "u8" else (-3, -4)
     maps = {}
     try:
-        for split in (-1, -2):
-            _say(f"planes: depth map small split={split}")
+        for split in splits:
+            _say(f"planes: depth map small {grey} split={split}")
             gpu_ctx.set_schedule(split, 1, 16, False)
+            gpu_ctx.reset_stats()
             maps[split] = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
+            assert gpu_ctx.stats()["split"] == split
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
-    for a, b in zip(maps[-1], maps[-2]):
+    for a, b in zip(maps[splits[0]], maps[splits[1]]):
         assert np.array_equal(a, b)
-    assert (maps[-1][0] > 0).mean() > 0.5
+    assert (maps[splits[0]][0] > 0).mean() > 0.5
 
 
 def test_pixel_lane_schedule_leaves_exact_mode_exact(gpu_ctx, tiny_survey):
@@ -317,7 +330,7 @@ def test_fast_f32_depth_maps_within_tolerance(gpu_ctx, name, k, iters):
     d1, _, c1 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
     st = gpu_ctx.stats()
     d2, _, c2 = gpu_ctx.depth_map(v, srcs, *spec.depth_range, prm)
-    assert st["split"] == -3
+    assert st["split"] in (-3, -4)
     assert np.array_equal(d1, d2) and np.array_equal(c1, c2)  # deterministic
     both = (d1 > 0) & (od > 0)
     rel = np.abs(d1[both] - od[both]) / od[both]
