@@ -594,8 +594,9 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
 // Schedule of the initial-cost evaluation (one cost per pixel, scattered
 // random planes): the one-lane-per-pixel kernels where they apply, whose
 // gathers each read one source image, on the 8-bit corner plane (scattered
-// gathers favour the smaller plane; f32 views: the grey plane itself), else
-// the first source-lane candidate.
+// gathers favour the smaller plane; f32 views: the grey plane, a quarter of
+// the corner-difference plane's bytes — 92 against 126 ms on 77 views
+// 2000x1500), else the first source-lane candidate.
 Schedule eval_schedule(const KClass &k) {
     if (k.fast == 2) return Schedule{-4, 1, 0};
     if (k.fast && k.pinhole == 2) return Schedule{-2, 1, 0};
