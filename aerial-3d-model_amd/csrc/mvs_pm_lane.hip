@@ -55,6 +55,19 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // swizzle): reads past num_records return zero.
 constexpr int32_t kBufferWord3 = 0x00020000;
 
+// Half-sweeps of the first iterations (random planes) score their
+// compacted candidates in order of depth, in kDepthBins bins of inverse depth
+// over the job's range: a batch of 64 then holds candidates of similar depth
+// from the wavefront's block of pixels, whose windows land close together in
+// each source (shared cache lines, and in or out of the source's coverage
+// together). Later iterations keep the candidate-major order, whose
+// neighbouring pixels' k-th candidates already agree.
+#ifndef AMVS_SORT_ITERS
+#define AMVS_SORT_ITERS 2
+#endif
+constexpr int kSortIters = AMVS_SORT_ITERS;
+constexpr int kDepthBins = 16;
+
 // A window whose four corner samples lie this far (pixels) inside the
 // source's sampling range keeps every tap inside it: rounding moves an inner
 // tap's position by a few 1e-4 pixels at most (ulps of coordinates below
@@ -109,6 +122,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     // (pixel lane, candidate) entries, and each pixel's lowest (cost, index)
     __shared__ uint16_t plist[8 * 64];
     __shared__ unsigned long long pmin[64];
+    __shared__ int32_t pbin[kDepthBins];
 
     const int lane = threadIdx.x;
     const int chunk = 64 * a.gpc;
@@ -519,7 +533,9 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // the wavefront scores the candidates of all its pixels as
                     // one compacted list, no lane idle while others score,
                     // and keeps each pixel's lowest (cost, index) in LDS.
-                    uint32_t amask = 0;
+                    uint32_t amask = 0, bins = 0;
+                    const bool by_depth = a.it < kSortIters;  // uniform
+                    const float bscale = (float)kDepthBins / (job.imax - job.imin);
                     {
                         float4 cand[8];
                         uint32_t inbm = 0;
@@ -536,22 +552,56 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 act = act && !(((inbm >> j) & 1u) && same_hyp(cand[j], cand[k]));
                             inbm |= (inb ? 1u : 0u) << k;
                             amask |= (act ? 1u : 0u) << k;
+                            if (by_depth) {
+                                const float iv = __builtin_amdgcn_rcpf(cand[k].x);
+                                const int b = (int)__builtin_amdgcn_fmed3f(
+                                    (iv - job.imin) * bscale, 0.0f, (float)(kDepthBins - 1));
+                                bins |= (uint32_t)b << (4 * k);
+                            }
                         }
                     }
-                    // the list, candidate-major (neighbouring pixels' k-th
-                    // candidates together, as they come from neighbouring
-                    // pixels themselves)
                     int32_t total = 0;
+                    if (by_depth) {
+                        // the list by depth bin: counts, their scan, then
+                        // each candidate at its bin's next slot (the order
+                        // within a bin does not matter: the minimum is)
+                        if (lane < kDepthBins) pbin[lane] = 0;
+                        __syncthreads();
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const bool on = (amask >> k) & 1u;
-                        const uint64_t bal = __ballot(on);
-                        if (on)
-                            plist[total + (int32_t)__builtin_amdgcn_mbcnt_hi(
-                                              (uint32_t)(bal >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] =
-                                (uint16_t)(lane | (k << 6));
-                        total += __popcll(bal);
+                        for (int k = 0; k < 8; ++k)
+                            if ((amask >> k) & 1u) atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1);
+                        __syncthreads();
+                        const int cnt = lane < kDepthBins ? pbin[lane] : 0;
+                        int incl = cnt;
+#pragma unroll
+                        for (int off = 1; off < kDepthBins; off <<= 1) {
+                            const int t = __shfl_up(incl, off);
+                            if (lane >= off) incl += t;
+                        }
+                        total = __shfl(incl, kDepthBins - 1);
+                        __syncthreads();
+                        if (lane < kDepthBins) pbin[lane] = incl - cnt;
+                        __syncthreads();
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if ((amask >> k) & 1u)
+                                plist[atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1)] =
+                                    (uint16_t)(lane | (k << 6));
+                    } else {
+                        // the list, candidate-major (neighbouring pixels' k-th
+                        // candidates together, as they come from neighbouring
+                        // pixels themselves)
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const bool on = (amask >> k) & 1u;
+                            const uint64_t bal = __ballot(on);
+                            if (on)
+                                plist[total + (int32_t)__builtin_amdgcn_mbcnt_hi(
+                                                  (uint32_t)(bal >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] =
+                                    (uint16_t)(lane | (k << 6));
+                            total += __popcll(bal);
+                        }
                     }
                     pmin[lane] = ~0ull;
                     __syncthreads();  // the list, pmin and every lane's window in Ah
