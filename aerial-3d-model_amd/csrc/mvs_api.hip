@@ -527,6 +527,8 @@ std::vector<Schedule> candidates(const KClass &k) {
     if (k.fast && k.pinhole == 2) {
         cand.push_back(Schedule{-1, 1, 16});  // f16 corner plane
         cand.push_back(Schedule{-1, 1, 8});
+        cand.push_back(Schedule{-5, 1, 16});  // f16 corner plane staged in LDS tiles
+        cand.push_back(Schedule{-5, 1, 8});
         cand.push_back(Schedule{-2, 1, 16});  // 8-bit corner plane
         cand.push_back(Schedule{-2, 1, 8});
     }
@@ -1169,8 +1171,8 @@ int mvs_synchronize(mvs_ctx *c) {
 int mvs_set_schedule(mvs_ctx *c, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune) {
     if (!c) return MVS_ERR_ARG;
-    if (split < -4 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
-        return fail(c, MVS_ERR_ARG, "schedule out of range (split >= -4, gpc and per_cu 0..1024)");
+    if (split < -5 || gpc < 0 || gpc > 1024 || per_cu < 0 || per_cu > 1024)
+        return fail(c, MVS_ERR_ARG, "schedule out of range (split >= -5, gpc and per_cu 0..1024)");
     c->forced = Schedule{split, gpc, per_cu};
     c->autotune = autotune != 0;
     return MVS_OK;

@@ -122,14 +122,16 @@ enum { PM_EVAL_CURRENT = 0, PM_SWEEP = 1, PM_COST_LIST = 2 };
 // sampling the 8-bit corner plane, split = -3 the same kernels on f32 grey
 // views sampling the f32 corner-difference plane (fast = 2), split = -4 the
 // same on f32 grey views sampling the grey plane itself (two row loads per
-// sample, a quarter of the bytes per pixel; same values as split = -3).
+// sample, a quarter of the bytes per pixel; same values as split = -3), split
+// = -5 the f16 corner plane staged in LDS tiles (same values as split = -1).
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st);
 // One lane per reference pixel (mvs_pm_lane.hip); 64 pixels per group.
 // plane: PL_U8 (8-bit corner plane), PL_F16 (f16 corner plane), PL_F32
 // (f32 corner-difference plane of f32 grey views), PL_F32G (the f32 grey plane
-// itself, read as two 8-byte row pairs per sample).
-enum { PL_U8 = 0, PL_F16 = 1, PL_F32 = 2, PL_F32G = 3 };
+// itself, read as two 8-byte row pairs per sample), PL_F16T (the f16 corner
+// plane staged in LDS per batch and source where the windows fit a tile).
+enum { PL_U8 = 0, PL_F16 = 1, PL_F32 = 2, PL_F32G = 3, PL_F16T = 4 };
 // Zero floats after a view's grey plane (w*h floats): a row-pair read at the
 // last column or row, weighted 0, stays inside the allocation on finite data.
 __host__ __device__ inline int64_t grey_pad(int32_t w) { return (int64_t)w + 64; }

@@ -422,11 +422,12 @@ void launch_init(const RefJob &job, int32_t taps, int32_t write_hyp, hipStream_t
 int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinhole,
               bool persist, bool fast, const PmArgs &a, int32_t grid, hipStream_t st) {
     if (split < 0) {
-        if (!fast || pinhole != 2 || split < -4) return -1;
+        if (!fast || pinhole != 2 || split < -5) return -1;
         const int plane = split == -1 ? PL_F16
                           : split == -2 ? PL_U8
                           : split == -3 ? PL_F32
-                                        : PL_F32G;
+                          : split == -4 ? PL_F32G
+                                        : PL_F16T;
         return launch_pml(mode, taps, plane, persist, a, grid, st);
     }
     if (fast) {

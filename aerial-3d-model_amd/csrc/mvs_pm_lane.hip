@@ -68,6 +68,12 @@ constexpr int32_t kBufferWord3 = 0x00020000;
 constexpr int kSortIters = AMVS_SORT_ITERS;
 constexpr int kDepthBins = 16;
 
+// PL_F16T: the f16 corner plane staged in LDS per (batch, source) when the
+// batch's windows fit a tile of kTileCap texels (8 bytes each) and lie
+// inside the source: a few coalesced loads replace the per-tap gathers,
+// and the taps read LDS. Batches that do not fit gather as PL_F16 does.
+constexpr int kTileCap = 512;
+
 // A window whose four corner samples lie this far (pixels) inside the
 // source's sampling range keeps every tap inside it: rounding moves an inner
 // tap's position by a few 1e-4 pixels at most (ulps of coordinates below
@@ -112,6 +118,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     constexpr int N = TAPS * TAPS;
     constexpr int H = TAPS / 2;
     constexpr bool F32 = PLANE == PL_F32 || PLANE == PL_F32G;
+    constexpr bool TILE = PLANE == PL_F16T;
+    constexpr bool F16Q = PLANE == PL_F16 || TILE;  // samples the f16 corner plane
     // each lane's reference window as 8-bit values less an integer near their
     // mean (exact in f16; f32 views: the grey less the window mean, f32),
     // tap-major so that the per-tap reads of a wavefront are free of bank
@@ -123,6 +131,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     __shared__ uint16_t plist[8 * 64];
     __shared__ unsigned long long pmin[64];
     __shared__ int32_t pbin[kDepthBins];
+    __shared__ u32x2 tile[TILE ? kTileCap : 1];
 
     const int lane = threadIdx.x;
     const int chunk = 64 * a.gpc;
@@ -268,7 +277,9 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     const float sx = dqc * rstep;
                     const float tnx = (n0 * rstep) * rden, tny = (n1 * rstep) * rden;
                     float m1 = INFINITY, m2 = INFINITY;
-                    if (act) {
+                    // (PL_F16T: every lane walks the sources, so that the whole
+                    // wavefront can stage each source's tile)
+                    if (TILE || act) {
                         for (int k = 0; k < K; ++k) {
                             const SrcCam S = scalar_load(job.srcs + k);
                             const float F = S.f, CXh = S.cx - 0.5f, CYh = S.cy - 0.5f;
@@ -297,6 +308,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             // quadrilateral of the corner images, so the taps'
                             // bounding box is the corners'. NaN fails.
                             bool ok = true, inner = true;
+                            float fxlo = INFINITY, fxhi = -INFINITY, fylo = INFINITY,
+                                  fyhi = -INFINITY;
 #pragma unroll
                             for (int cr = 0; cr < 4; ++cr) {
                                 const float fdx = (cr & 1) ? (float)H : -(float)H;
@@ -310,47 +323,108 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                      fy < hm1;
                                 inner = inner && fx >= kInnerMargin && fy >= kInnerMargin &&
                                         fx < wm1 - kInnerMargin && fy < hm1 - kInnerMargin;
+                                if constexpr (TILE) {
+                                    fxlo = fminf(fxlo, fx);
+                                    fxhi = fmaxf(fxhi, fx);
+                                    fylo = fminf(fylo, fy);
+                                    fyhi = fmaxf(fyhi, fy);
+                                }
+                            }
+                            if constexpr (TILE) ok = ok && act;
+                            const uint32_t sw = (uint32_t)S.w;
+                            // the plane as a structured buffer: 32-bit pixel
+                            // indices, and the hardware range check as a
+                            // second bound; the resource is wave-uniform
+                            // (scalar registers)
+                            // (readfirstlane returns a signed int: each half
+                            // goes through uint32_t, never sign-extended)
+                            constexpr uint32_t QB =
+                                PLANE == PL_F32 ? 16u : (F16Q ? 8u : 4u);
+                            const uint64_t qa =
+                                PLANE == PL_F32    ? (uint64_t)(uintptr_t)S.quadd
+                                : F16Q             ? (uint64_t)(uintptr_t)S.quadh
+                                : PLANE == PL_F32G ? (uint64_t)(uintptr_t)S.gray
+                                                   : (uint64_t)(uintptr_t)S.quad;
+                            const uint64_t qlo =
+                                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
+                            const uint64_t qhi =
+                                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
+                            // records of QB bytes, one per pixel (PL_F32G:
+                            // a raw buffer of the grey plane and its zero
+                            // padding, in bytes)
+                            const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
+                                (void *)(uintptr_t)(qlo | (qhi << 32)),
+                                (short)(PLANE == PL_F32G ? 0u : QB),
+                                __builtin_amdgcn_readfirstlane(
+                                    PLANE == PL_F32G
+                                        ? (int32_t)(4 * (S.w * S.h + (int32_t)grey_pad(S.w)))
+                                        : S.w * S.h),
+                                kBufferWord3);
+                            // bytes per grey row (PL_F32G)
+                            const int32_t row4 = __builtin_amdgcn_readfirstlane(4 * S.w);
+                            // PL_F16T: the batch's texels of this source in LDS when
+                            // every scoring window lies inside it (1/256 px, so
+                            // rounding keeps each tap's texel within the corners'
+                            // box) and the box fits the tile (uniform decision)
+                            bool tiled = false;
+                            uint32_t tw = 1u;
+                            int32_t tbase = 0, tcnt = 1;
+                            if constexpr (TILE) {
+                                typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+                                const float m = kInnerMargin;
+                                // (x0, y0) low corner and (0xffff - x1, 0xffff - y1):
+                                // both reduce by packed 16-bit minimum
+                                u16x2 lo = {0xffff, 0xffff}, hi = {0xffff, 0xffff};
+                                if (ok) {
+                                    lo = u16x2{(uint16_t)(int)(fxlo - m), (uint16_t)(int)(fylo - m)};
+                                    hi = u16x2{(uint16_t)(0xffff - (int)(fxhi + m)),
+                                               (uint16_t)(0xffff - (int)(fyhi + m))};
+                                }
+#pragma unroll
+                                for (int off = 32; off >= 1; off >>= 1) {
+                                    const u16x2 ol = __builtin_bit_cast(
+                                        u16x2, __shfl_xor(__builtin_bit_cast(uint32_t, lo), off));
+                                    const u16x2 oh = __builtin_bit_cast(
+                                        u16x2, __shfl_xor(__builtin_bit_cast(uint32_t, hi), off));
+                                    lo = __builtin_elementwise_min(lo, ol);
+                                    hi = __builtin_elementwise_min(hi, oh);
+                                }
+                                const uint32_t lov = __builtin_amdgcn_readfirstlane(
+                                    __builtin_bit_cast(uint32_t, lo));
+                                const uint32_t hiv = __builtin_amdgcn_readfirstlane(
+                                    __builtin_bit_cast(uint32_t, hi));
+                                const int32_t X0 = (int32_t)(lov & 0xffffu), Y0 = (int32_t)(lov >> 16);
+                                const int32_t X1 = 0xffff - (int32_t)(hiv & 0xffffu),
+                                              Y1 = 0xffff - (int32_t)(hiv >> 16);
+                                const bool any = __ballot(ok) != 0ull;
+                                const bool allin = __ballot(ok && !inner) == 0ull;
+                                const int32_t TW = X1 - X0 + 1, TH = Y1 - Y0 + 1;
+                                tiled = any && allin && TW > 0 && TH > 0 && TW <= kTileCap &&
+                                        TH <= kTileCap && TW * TH <= kTileCap;
+                                if (tiled) {
+                                    tw = (uint32_t)TW;
+                                    tcnt = TW * TH;
+                                    tbase = Y0 * TW + X0;
+                                    const float rtw = 1.0f / (float)TW;
+                                    for (int i = lane; i < tcnt; i += 64) {
+                                        const int r = (int)(((float)i + 0.5f) * rtw);
+                                        const int cl = i - r * TW;
+                                        tile[i] = amvs_struct_load_b64(
+                                            Qh, (int32_t)((uint32_t)(Y0 + r) * sw) + X0 + cl, 0, 0, 0);
+                                    }
+                                    __syncthreads();
+                                }
                             }
                             float cs = 2.0f;  // a failing source scores 2.0
                             if (ok) {
-                                const uint32_t sw = (uint32_t)S.w;
-                                // the plane as a structured buffer: 32-bit pixel
-                                // indices, and the hardware range check as a
-                                // second bound; the resource is wave-uniform
-                                // (scalar registers)
-                                // (readfirstlane returns a signed int: each half
-                                // goes through uint32_t, never sign-extended)
-                                constexpr uint32_t QB =
-                                    PLANE == PL_F32 ? 16u : (PLANE == PL_F16 ? 8u : 4u);
-                                const uint64_t qa =
-                                    PLANE == PL_F32    ? (uint64_t)(uintptr_t)S.quadd
-                                    : PLANE == PL_F16  ? (uint64_t)(uintptr_t)S.quadh
-                                    : PLANE == PL_F32G ? (uint64_t)(uintptr_t)S.gray
-                                                       : (uint64_t)(uintptr_t)S.quad;
-                                const uint64_t qlo =
-                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)qa);
-                                const uint64_t qhi =
-                                    (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(qa >> 32));
-                                // records of QB bytes, one per pixel (PL_F32G:
-                                // a raw buffer of the grey plane and its zero
-                                // padding, in bytes)
-                                const __amdgpu_buffer_rsrc_t Qh = __builtin_amdgcn_make_buffer_rsrc(
-                                    (void *)(uintptr_t)(qlo | (qhi << 32)),
-                                    (short)(PLANE == PL_F32G ? 0u : QB),
-                                    __builtin_amdgcn_readfirstlane(
-                                        PLANE == PL_F32G
-                                            ? (int32_t)(4 * (S.w * S.h + (int32_t)grey_pad(S.w)))
-                                            : S.w * S.h),
-                                    kBufferWord3);
-                                // bytes per grey row (PL_F32G)
-                                const int32_t row4 = __builtin_amdgcn_readfirstlane(4 * S.w);
                                 float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
                                 // The window's taps; CLAMP keeps every position
                                 // in the sampling range whatever the rounding
                                 // (v_med3, NaN to 0), needed only when some lane
                                 // scoring this source has a corner near the edge.
-                                auto taps = [&](auto clamp_tag) {
+                                auto taps = [&](auto clamp_tag, auto tile_tag) {
                                     constexpr bool CLAMP = decltype(clamp_tag)::value;
+                                    constexpr bool TL = decltype(tile_tag)::value;
                                     // one tap row per trip: a row's gathers are in
                                     // flight together while the registers they hold
                                     // leave room for several wavefronts per SIMD
@@ -376,7 +450,18 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                             const float ax = __builtin_amdgcn_fractf(fx),
                                                         ay = __builtin_amdgcn_fractf(fy);
                                             float top, bot;
-                                            if constexpr (PLANE == PL_F32G) {
+                                            if constexpr (TL) {
+                                                // the staged texel (index clamped into
+                                                // the tile: a safety bound only)
+                                                const uint32_t ti = __builtin_elementwise_min(
+                                                    (uint32_t)((int32_t)(__umul24(y0, tw) + x0) - tbase),
+                                                    (uint32_t)(tcnt - 1));
+                                                const u32x2 qv = tile[ti];
+                                                const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
+                                                const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
+                                                top = fmaf(ax, (float)r0.y, (float)r0.x);
+                                                bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                            } else if constexpr (PLANE == PL_F32G) {
                                                 // g00, g10 and g01, g11 from the
                                                 // rows y0 and y0 + 1; at the last
                                                 // column or row the pair's second
@@ -401,7 +486,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                     amvs_struct_load_f4(Qh, idx, 0, 0, 0);
                                                 top = fmaf(ax, e.y, e.x);
                                                 bot = fmaf(ax, e.w, e.z);
-                                            } else if constexpr (PLANE == PL_F16) {
+                                            } else if constexpr (F16Q) {
                                                 const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // corners and column differences, f16
                                                 const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
@@ -433,10 +518,18 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                         }
                                     }
                                 };
-                                if (__ballot(!inner) == 0ull)  // uniform over the scoring lanes
-                                    taps(std::false_type{});
-                                else
-                                    taps(std::true_type{});
+                                if constexpr (TILE) {
+                                    if (tiled)
+                                        taps(std::false_type{}, std::true_type{});
+                                    else if (__ballot(!inner) == 0ull)
+                                        taps(std::false_type{}, std::false_type{});
+                                    else
+                                        taps(std::true_type{}, std::false_type{});
+                                } else if (__ballot(!inner) == 0ull) {  // uniform over the scoring lanes
+                                    taps(std::false_type{}, std::false_type{});
+                                } else {
+                                    taps(std::true_type{}, std::false_type{});
+                                }
                                 // NCC from the shifted sums, in byte units:
                                 // N var = s2 - s1^2 / N and
                                 // cov = sum (b - mean b) dd = sa - delta s1
@@ -703,15 +796,17 @@ void pml_planes(int mode, int plane, bool persist, const PmArgs &a, int32_t grid
         pml_taps<TAPS, PL_U8>(mode, persist, a, grid, st);
     else if (plane == PL_F32)
         pml_taps<TAPS, PL_F32>(mode, persist, a, grid, st);
-    else
+    else if (plane == PL_F32G)
         pml_taps<TAPS, PL_F32G>(mode, persist, a, grid, st);
+    else
+        pml_taps<TAPS, PL_F16T>(mode, persist, a, grid, st);
 }
 
 }  // namespace
 
 int launch_pml(int mode, int32_t taps, int plane, bool persist, const PmArgs &a, int32_t grid,
                hipStream_t st) {
-    if (plane < PL_U8 || plane > PL_F32G) return -1;
+    if (plane < PL_U8 || plane > PL_F16T) return -1;
     if (taps == 5)
         pml_planes<5>(mode, plane, persist, a, grid, st);
     else if (taps == 7)

@@ -47,7 +47,8 @@ def _hyps(v, n, rng, near=0.6):
     ("small", 11, 1, 8, 2), ("small", 11, 1, 4, 4), ("tiny", 5, 2, 4, -1),
     ("small", 7, 1, 8, -1), ("small", 7, 1, 3, -1), ("small", 11, 1, 8, -1),
     ("small", 5, 2, 1, -1), ("small", 7, 1, 1, 1), ("small", 7, 1, 1, -1),
-    ("tiny", 5, 2, 4, -2), ("small", 7, 1, 8, -2), ("small", 11, 1, 4, -2)])
+    ("tiny", 5, 2, 4, -2), ("small", 7, 1, 8, -2), ("small", 11, 1, 4, -2),
+    ("small", 7, 1, 8, -5), ("tiny", 5, 2, 4, -5)])
 def test_fast_costs_within_tolerance(gpu_ctx, name, taps, stride, k, split):
     s = survey_cached(name)
     ref = s.reference_index()
@@ -157,7 +158,8 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k, 
     """The planes of one view hold the same corner values, so the
     one-lane-per-pixel kernels give the same costs, bit for bit, on either.
     8-bit views: split -2 (8-bit plane, bytes unpacked and differenced per
-    sample) and split -1 (f16 plane with the column differences stored). f32
+    sample), split -1 (f16 plane with the column differences stored) and split
+    -5 (the f16 plane staged in LDS tiles where a batch's windows fit). f32
     views: split -3 (f32 corner-difference plane) and split -4 (the grey plane,
     two row pairs per sample, differenced per sample; windows at the last
     column or row read the next row or the zero padding at weight 0).
@@ -175,7 +177,7 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k, 
     views = s.views if grey == "u8" else _f32_views(s, 31 + taps + k)
     v = views[ref]
     srcs = [views[j] for j in nb]
-    splits = (-1, -2) if grey == "u8" else (-3, -4)
+    splits = (-1, -2, -5) if grey == "u8" else (-3, -4)
     rng = np.random.default_rng(700 + taps + k)
     n = 30000  # the oblique view's costs are defined for ~3% of random items
     xs = rng.integers(0, v.w, n).astype(np.int32)
@@ -188,6 +190,20 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k, 
     up = np.asarray(v.R, np.float64).reshape(3, 3) @ np.array([0.0, 0.0, 1.0])  # camera frame
     nrm = rng.normal(up, np.where(close, 0.05, 0.5)[:, None], (n, 3))
     hy[:, 1:] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    # and first, blocks of 64 neighbouring pixels (32 x 2) with planes near
+    # the surface, one 64-item batch each: windows that fit an LDS tile
+    # (split -5)
+    nblk, m = 60, 2 * taps * stride
+    bx = rng.integers(m, v.w - 32 - m, nblk)
+    by = rng.integers(m, v.h - 2 - m, nblk)
+    gx = (bx[:, None] + np.tile(np.arange(32), 2)[None, :]).ravel().astype(np.int32)
+    gy = (by[:, None] + np.repeat(np.arange(2), 32)[None, :]).ravel().astype(np.int32)
+    gt = v.depth_gt[gy, gx]
+    bh = np.zeros((len(gx), 4), np.float32)
+    bh[:, 0] = np.where(gt > 0, gt * rng.uniform(0.999, 1.001, len(gx)), rng_d[0])
+    nb_ = rng.normal(up, 0.02, (len(gx), 3))
+    bh[:, 1:] = nb_ / np.linalg.norm(nb_, axis=1, keepdims=True)
+    xs, ys, hy = np.concatenate([gx, xs]), np.concatenate([gy, ys]), np.concatenate([bh, hy])
     prm = capi.params(taps=taps, stride=stride, fast=True)
     got = {}
     _say(f"planes {grey} {name} {taps}x{taps} K={k}: view {ref} {v.w}x{v.h}, sources {nb}")
@@ -199,18 +215,20 @@ def test_pixel_lane_planes_give_identical_costs(gpu_ctx, name, taps, stride, k, 
             assert gpu_ctx.stats()["split"] == split
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
-    a, b = got[splits[0]], got[splits[1]]
+    a = got[splits[0]]
     defined = a < 2.0
-    _say(f"  defined {defined.mean():.3f}, identical {np.mean(a == b):.6f}")
+    _say(f"  defined {defined.mean():.3f}, identical "
+         f"{[float(np.mean(a == got[sp])) for sp in splits[1:]]}")
     assert defined.sum() >= 800
-    assert np.array_equal(a, b)
+    for sp in splits[1:]:
+        assert np.array_equal(a, got[sp]), sp
 
 
 @pytest.mark.parametrize("grey", ["u8", "f32"])
 def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
     """Whole depth maps (initial costs, every half-sweep) on one plane equal
-    those on the other: 8-bit and f16 planes of 8-bit views, corner-difference
-    and grey planes of f32 views."""
+    those on the others: 8-bit, f16 and LDS-staged f16 planes of 8-bit views,
+    corner-difference and grey planes of f32 views."""
     s = survey_cached("small")
     spec = s.spec
     ref = s.reference_index()
@@ -218,7 +236,7 @@ def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
     v = views[ref]
     srcs = [views[j] for j in s.neighbours[ref][:8]]
     prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=4, fast=True)
-    splits = (-1, -2) if grey == "u8" else (-3, -4)
+    splits = (-1, -2, -5) if grey == "u8" else (-3, -4)
     maps = {}
     try:
         for split in splits:
@@ -229,8 +247,9 @@ def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
             assert gpu_ctx.stats()["split"] == split
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
-    for a, b in zip(maps[splits[0]], maps[splits[1]]):
-        assert np.array_equal(a, b)
+    for sp in splits[1:]:
+        for a, b in zip(maps[splits[0]], maps[sp]):
+            assert np.array_equal(a, b), sp
     assert (maps[splits[0]][0] > 0).mean() > 0.5
 
 
