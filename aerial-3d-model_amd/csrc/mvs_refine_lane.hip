@@ -401,21 +401,20 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
     const float lo = job.dmin * 0.7f, hi = job.dmax * 1.4f;
     float lambda = 1e-3f;
     unsigned long long acc = 0;
-    // J^T J, the gradient and the cost at the current plane: after a rejected
-    // step the plane is unchanged, so they are reused, not recomputed
+    // J^T J, the gradient and the cost at the current plane, and the tangent
+    // directions they are taken along. A trial plane's pass forms the same
+    // terms (except on the last step, which needs only its cost), so an
+    // accepted step carries them to the next one and a rejected step keeps
+    // the current ones: one window pass per source and step.
     float Hm[6], g[3], cost0 = 0.0f;
-    bool have_terms = false;
+    float dirs[2][3];
+    if (live) {
+        tangent_dirs(nrm, dirs);
+        live = lane_pair_gn<TAPS, WT>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g,
+                                      cost0);
+    }
     for (int it = 0; it < steps; ++it) {
         if (!live) break;
-        float dirs[2][3];
-        tangent_dirs(nrm, dirs);
-        if (!have_terms) {
-            // J^T J, J^T r and the cost of the pair at the current plane
-            live = lane_pair_gn<TAPS, WT>(Ss[p0], Ss[p1], pr, rstep, hp.x, nrm, dirs, Ah, lane, Hm, g,
-                                      cost0);
-            if (!live) break;
-            have_terms = true;
-        }
         const float Hd[6] = {Hm[0] * (1.0f + lambda), Hm[1], Hm[2], Hm[3] * (1.0f + lambda),
                              Hm[4], Hm[5] * (1.0f + lambda)};
         const float rhs[3] = {-g[0], -g[1], -g[2]};
@@ -439,7 +438,27 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
         const bool cand = solved && (n1[0] * pr.rcx + n1[1] * pr.rcy + n1[2] < 0.0f) &&
                           d1 >= lo && d1 <= hi;
         bool better = false;
-        if (cand) {
+        if (cand && it + 1 < steps) {
+            // the trial's cost with its own terms (the NCC sums are the same
+            // arithmetic as the cost-only pass's)
+            float dirs1[2][3], Hm1[6], g1[3], cost1 = 0.0f;
+            tangent_dirs(n1, dirs1);
+            better = lane_pair_gn<TAPS, WT>(Ss[p0], Ss[p1], pr, rstep, d1, n1, dirs1, Ah, lane,
+                                            Hm1, g1, cost1) &&
+                     cost1 < cost0;
+            if (better) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) Hm[q] = Hm1[q];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) g[q] = g1[q];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    dirs[0][q] = dirs1[0][q];
+                    dirs[1][q] = dirs1[1][q];
+                }
+                cost0 = cost1;
+            }
+        } else if (cand) {
             const LTerms<0> CA =
                 lane_terms<TAPS, 0, WT>(Ss[p0], pr, rstep, d1, n1, nullptr, Ah, lane);
             const LTerms<0> CB =
@@ -453,7 +472,6 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
             nrm[2] = n1[2];
             lambda = fmaxf(lambda * 0.1f, 1e-6f);
             ++acc;
-            have_terms = false;
         } else {
             lambda = fminf(lambda * 10.0f, 1e6f);
         }
