@@ -223,6 +223,132 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
     return T;
 }
 
+// NCC of source S for the plane (d, n) at this lane's pixel (lane_terms with
+// NDIR = 0, the same definedness tests and sums), with the window's source
+// points from the centre-scaled homography: scaled by the centre tap's depth
+// dq_c, the source point of tap (dx, dy) is affine in (dx, dy),
+//     X_c + dx (R e_x dq_c + t (n . e_x) / (n . r_c)) + dy (...),
+// so a tap costs three multiply-adds and one reciprocal rather than the
+// plane depth's reciprocal, its scaling and the projection's reciprocal.
+// The taps clamp their positions only when some lane's window has a corner
+// within 1/256 pixel of the sampling range's edge (rounding moves an inner
+// tap by a few 1e-4 pixels at most).
+template <int TAPS, class WT>
+__device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, float d,
+                               const float n[3], const WT (*Ah)[64], int lane, bool act) {
+    constexpr int N = TAPS * TAPS, H = TAPS / 2;
+    constexpr bool F32 = std::is_same<WT, float>::value;
+    LTerms<0> T;
+    const float nc = fmaf(n[0], pr.rcx, fmaf(n[1], pr.rcy, n[2]));  // n . r_c
+    const float cc = d * nc;
+    const float ex = n[0] * rstep, ey = n[1] * rstep;
+    float wc[3], wx[3], wy[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        wc[i] = fmaf(S.R[3 * i], pr.rcx, fmaf(S.R[3 * i + 1], pr.rcy, S.R[3 * i + 2]));
+        wx[i] = S.R[3 * i] * rstep;
+        wy[i] = S.R[3 * i + 1] * rstep;
+    }
+    const float F = S.f, CXh = S.cx - 0.5f, CYh = S.cy - 0.5f;
+    const float wm1 = (float)(S.w - 1), hm1 = (float)(S.h - 1);
+    constexpr float kInner = 1.0f / 256.0f;
+    bool inner = true;
+#pragma unroll
+    for (int cr = 0; cr < 4; ++cr) {
+        const float fdx = (cr & 1) ? (float)H : -(float)H;
+        const float fdy = (cr & 2) ? (float)H : -(float)H;
+        const float den = fmaf(fdy, ey, fmaf(fdx, ex, nc));
+        act = act && fabsf(den) >= 1e-6f && cc * den > 0.0f;
+        const float dq = cc * __builtin_amdgcn_rcpf(den);
+        float X[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) X[i] = fmaf(dq, fmaf(fdy, wy[i], fmaf(fdx, wx[i], wc[i])), S.t[i]);
+        act = act && X[2] > 0.0f;
+        const float fi = F * __builtin_amdgcn_rcpf(X[2]);
+        const float u = fmaf(fi, X[0], CXh), v = fmaf(fi, X[1], CYh);
+        act = act && u >= 0.0f && v >= 0.0f && u < wm1 && v < hm1;
+        inner = inner && u >= kInner && v >= kInner && u < wm1 - kInner && v < hm1 - kInner;
+    }
+    // the centre tap's depth and the source-space steps per tap column / row
+    const float rnc = act ? __builtin_amdgcn_rcpf(nc) : 0.0f;
+    const float dqc = cc * rnc;
+    const float pc0 = pr.rcx * dqc, pc1 = pr.rcy * dqc;
+    const float sx = dqc * rstep;
+    const float tnx = ex * rnc, tny = ey * rnc;
+    float B0[3], B1[3], B2[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float xc =
+            fmaf(S.R[3 * i], pc0, fmaf(S.R[3 * i + 1], pc1, fmaf(S.R[3 * i + 2], dqc, S.t[i])));
+        const float c1 = fmaf(S.t[i], tnx, S.R[3 * i] * sx);
+        const float c2 = fmaf(S.t[i], tny, S.R[3 * i + 1] * sx);
+        const float sc = i < 2 ? F : 1.0f;
+        B0[i] = sc * xc;
+        B1[i] = sc * c1;
+        B2[i] = sc * c2;
+    }
+    const uint32_t sw = (uint32_t)S.w;
+    const u32x2 *Qp = (const u32x2 *)S.quadh;
+    const float4 *Qd = S.quadd;
+    float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
+    auto taps = [&](auto clamp_tag) {
+        constexpr bool CLAMP = decltype(clamp_tag)::value;
+#pragma unroll 1
+        for (int ty = 0; ty < TAPS; ++ty) {
+            const float fdy = (float)(ty - H);
+            const float Y0 = fmaf(fdy, B2[0], B0[0]);
+            const float Y1 = fmaf(fdy, B2[1], B0[1]);
+            const float Y2 = fmaf(fdy, B2[2], B0[2]);
+#pragma unroll
+            for (int tx = 0; tx < TAPS; ++tx) {
+                const float fdx = (float)(tx - H);
+                const float P0 = tx == H ? Y0 : fmaf(fdx, B1[0], Y0);
+                const float P1 = tx == H ? Y1 : fmaf(fdx, B1[1], Y1);
+                const float P2 = tx == H ? Y2 : fmaf(fdx, B1[2], Y2);
+                const float ip = __builtin_amdgcn_rcpf(P2);
+                float fx = fmaf(P0, ip, CXh), fy = fmaf(P1, ip, CYh);
+                if constexpr (CLAMP) {
+                    fx = __builtin_amdgcn_fmed3f(fx, 0.0f, wm1);
+                    fy = __builtin_amdgcn_fmed3f(fy, 0.0f, hm1);
+                }
+                const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+                const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
+                float top, bot;
+                if constexpr (F32) {
+                    const float4 e = Qd[__umul24(y0, sw) + x0];
+                    top = fmaf(ax, e.y, e.x);
+                    bot = fmaf(ax, e.w, e.z);
+                } else {
+                    const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
+                    const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
+                    const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
+                    top = fmaf(ax, (float)r0.y, (float)r0.x);
+                    bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                }
+                const float val = fmaf(ay, bot - top, top);
+                if (tx == 0 && ty == 0) c0 = val;
+                const float sv = val - c0;
+                s1 += sv;
+                s2 = fmaf(sv, sv, s2);
+                sa = fmaf((float)Ah[ty * TAPS + tx][lane], sv, sa);
+            }
+        }
+    };
+    if (act) {
+        if (__ballot(act && !inner) == 0ull)
+            taps(std::false_type{});
+        else
+            taps(std::true_type{});
+    }
+    const float ms = s1 * (1.0f / (float)N);
+    const float nb2 = fmaf(-ms, s1, s2);
+    if (!(act && nb2 > 1e-12f)) return T;
+    T.nb = sqrtf(nb2);
+    T.ncc = fmaf(-pr.delta, s1, sa) / (pr.na * T.nb);
+    T.ok = true;
+    return T;
+}
+
 // a ranks before b: higher NCC, then lower source index (dense.rs:255-256).
 __device__ inline bool lane_before(float na, int ia, float nb, int ib) {
     return na > nb || (na == nb && ia < ib);
@@ -321,8 +447,8 @@ __device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, fl
     i0 = 1 << 20;
     i1 = 1 << 20;
     for (int k = 0; k < nsrc; ++k) {
-        if (search) {
-            const LTerms<0> T = lane_terms<TAPS, 0, WT>(Ss[k], pr, rstep, d, n, nullptr, Ah, lane);
+        {
+            const LTerms<0> T = lane_cost<TAPS, WT>(Ss[k], pr, rstep, d, n, Ah, lane, search);
             if (T.ok) {
                 if (lane_before(T.ncc, k, b0, i0)) {
                     b1 = b0;
@@ -459,10 +585,8 @@ __global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
                 cost0 = cost1;
             }
         } else if (cand) {
-            const LTerms<0> CA =
-                lane_terms<TAPS, 0, WT>(Ss[p0], pr, rstep, d1, n1, nullptr, Ah, lane);
-            const LTerms<0> CB =
-                lane_terms<TAPS, 0, WT>(Ss[p1], pr, rstep, d1, n1, nullptr, Ah, lane);
+            const LTerms<0> CA = lane_cost<TAPS, WT>(Ss[p0], pr, rstep, d1, n1, Ah, lane, true);
+            const LTerms<0> CB = lane_cost<TAPS, WT>(Ss[p1], pr, rstep, d1, n1, Ah, lane, true);
             better = CA.ok && CB.ok && 1.0f - 0.5f * (CA.ncc + CB.ncc) < cost0;
         }
         if (better) {
