@@ -22,14 +22,15 @@ struct SrcCam {
     // 8-bit inputs only (else null): per pixel the bytes of (x, y), (x+1, y),
     // (x, y+1), (x+1, y+1), low to high — a bilinear sample's four corners
     const uint32_t *quad;
-    // 8-bit inputs only: per pixel {g00 | (g10 - g00) << 16, g01 | (g11 - g01)
-    // << 16} as f16 (integers, exact): a bilinear sample's corners and column
-    // differences in one 8-byte load
+    // 8-bit inputs only: per pixel the bilinear coefficients of the cell
+    // with top-left corner (x, y), {g00 | A << 16, B | C << 16} as f16
+    // (integers, exact), A = g10 - g00, B = g01 - g00, C = g11 - g10 - g01 +
+    // g00: a sample is g00 + ax A + ay (B + ax C), from one 8-byte load
     const uint2 *quadh;
     // per pixel the grey of the same four corners (every view)
     const float4 *quadf;
-    // f32 grey inputs only (else null): per pixel {g00, g10 - g00, g01,
-    // g11 - g01}, a bilinear sample's corners and column differences
+    // f32 grey inputs only (else null): per pixel the same coefficients in
+    // f32, {g00, A, B, C} with C = (g11 - g01) - A
     const float4 *quadd;
     int32_t w, h;
     int32_t pinhole_exact;

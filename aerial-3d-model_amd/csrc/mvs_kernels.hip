@@ -56,10 +56,14 @@ __global__ void k_quad_from_u8(const uint8_t *__restrict__ src, int32_t w, int32
              ((uint32_t)r1[x1] << 24);
 }
 
-// Corner-packed f16 plane for the one-lane-per-pixel kernels: pixel (x, y)
-// holds b(x, y), b(x+1, y) - b(x, y), b(x, y+1), b(x+1, y+1) - b(x, y+1) as f16
-// (small integers, exact; clamped at the last column/row), so one 8-byte load
-// gives a bilinear sample's corners in the form its two row interpolations use.
+// Bilinear-coefficient plane in f16 for the one-lane-per-pixel kernels: the
+// sample at (x + ax, y + ay) of the cell whose top-left corner is (x, y) is
+//     g00 + ax A + ay (B + ax C),
+// A = g10 - g00, B = g01 - g00, C = g11 - g10 - g01 + g00 (g_ij the byte at
+// (x + i, y + j), clamped at the last column/row). Pixel (x, y) holds
+// {g00, A} and {B, C} as f16 (integers below 2^11, exact), so one 8-byte load
+// gives a sample's three multiply-adds their operands; A + ay C and B + ax C
+// are the sample's image gradient.
 __global__ void k_quadh_from_u8(const uint8_t *__restrict__ src, int32_t w, int32_t h,
                                 int32_t stride, uint2 *__restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,14 +74,15 @@ __global__ void k_quadh_from_u8(const uint8_t *__restrict__ src, int32_t w, int3
     auto h16 = [](int v) -> uint32_t {
         return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(float)v);
     };
-    dst[i] = make_uint2(h16(r0[x]) | (h16((int)r0[x1] - (int)r0[x]) << 16),
-                        h16(r1[x]) | (h16((int)r1[x1] - (int)r1[x]) << 16));
+    const int g00 = r0[x], g10 = r0[x1], g01 = r1[x], g11 = r1[x1];
+    dst[i] = make_uint2(h16(g00) | (h16(g10 - g00) << 16),
+                        h16(g01 - g00) | (h16(g11 - g10 - g01 + g00) << 16));
 }
 
-// Corner-difference plane of f32 grey views: pixel (x, y) holds g(x, y),
-// g(x+1, y) - g(x, y), g(x, y+1), g(x+1, y+1) - g(x, y+1) (clamped at the
-// last column/row), the two row interpolations of a bilinear sample as two
-// multiply-adds from one 16-byte load.
+// Bilinear-coefficient plane of f32 grey views: pixel (x, y) holds {g00, A,
+// B, C} as f32 with A = g10 - g00, B = g01 - g00, C = (g11 - g01) - A, each
+// rounded once in that order (clamped at the last column/row); the kernels
+// that read the grey plane itself form the same coefficients the same way.
 __global__ void k_quadd(const float *__restrict__ g, int32_t w, int32_t h,
                         float4 *__restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -85,7 +90,8 @@ __global__ void k_quadd(const float *__restrict__ g, int32_t w, int32_t h,
     const int32_t y = (int32_t)(i / w), x = (int32_t)(i - (int64_t)y * w);
     const int32_t x1 = x + 1 < w ? x + 1 : x, y1 = y + 1 < h ? y + 1 : y;
     const float *r0 = g + (int64_t)y * w, *r1 = g + (int64_t)y1 * w;
-    dst[i] = make_float4(r0[x], r0[x1] - r0[x], r1[x], r1[x1] - r1[x]);
+    const float A = r0[x1] - r0[x];
+    dst[i] = make_float4(r0[x], A, r1[x] - r0[x], (r1[x1] - r1[x]) - A);
 }
 
 // Corner-packed grey plane: pixel (x, y) holds the grey of (x, y), (x+1, y),

@@ -68,7 +68,7 @@ constexpr int32_t kBufferWord3 = 0x00020000;
 constexpr int kSortIters = AMVS_SORT_ITERS;
 constexpr int kDepthBins = 16;
 
-// PL_F16T: the f16 corner plane staged in LDS per (batch, source) when the
+// PL_F16T: the f16 coefficient plane staged in LDS per (batch, source) when the
 // batch's windows fit a tile of kTileCap texels (8 bytes each) and lie
 // inside the source: a few coalesced loads replace the per-tap gathers,
 // and the taps read LDS. Batches that do not fit gather as PL_F16 does.
@@ -104,13 +104,13 @@ __device__ __forceinline__ float key_cost(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// PLANE: PL_F16 samples the 8-byte f16 corner plane (fewer ALU operations
-// per sample), PL_U8 the 4-byte 8-bit corner plane (half the bytes per
+// PLANE: PL_F16 samples the 8-byte f16 bilinear-coefficient plane (fewer ALU
+// operations per sample), PL_U8 the 4-byte 8-bit corner plane (half the bytes per
 // gather) — both give the same values, from 8-bit views, in byte units; PL_F32
-// samples the 16-byte f32 corner-difference plane of f32 grey views (the
+// samples the 16-byte f32 coefficient plane of f32 grey views (the
 // `.view` cache's own grey, dense.rs:104-111), in grey units; PL_F32G reads
 // the same views' 4-byte grey plane as two row pairs per sample (the same
-// corner values and column differences, so the same costs, from a quarter of
+// coefficients, formed the same way, so the same costs, from a quarter of
 // the plane bytes: the scattered first iterations miss the caches less).
 template <int TAPS, int MODE, bool PERSIST, int PLANE>
 __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     constexpr int H = TAPS / 2;
     constexpr bool F32 = PLANE == PL_F32 || PLANE == PL_F32G;
     constexpr bool TILE = PLANE == PL_F16T;
-    constexpr bool F16Q = PLANE == PL_F16 || TILE;  // samples the f16 corner plane
+    constexpr bool F16Q = PLANE == PL_F16 || TILE;  // samples the f16 coefficient plane
     // each lane's reference window as 8-bit values less an integer near their
     // mean (exact in f16; f32 views: the grey less the window mean, f32),
     // tap-major so that the per-tap reads of a wavefront are free of bank
@@ -449,7 +449,9 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                             const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
                                             const float ax = __builtin_amdgcn_fractf(fx),
                                                         ay = __builtin_amdgcn_fractf(fy);
-                                            float top, bot;
+                                            // the sample g00 + ax A + ay (B + ax C) as
+                                            // t1 + ay u, t1 = g00 + ax A, u = B + ax C
+                                            float t1, u;
                                             if constexpr (TL) {
                                                 // the staged texel (index clamped into
                                                 // the tile: a safety bound only)
@@ -459,14 +461,16 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                 const u32x2 qv = tile[ti];
                                                 const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
                                                 const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
-                                                top = fmaf(ax, (float)r0.y, (float)r0.x);
-                                                bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                                t1 = fmaf(ax, (float)r0.y, (float)r0.x);
+                                                u = fmaf(ax, (float)r1.y, (float)r1.x);
                                             } else if constexpr (PLANE == PL_F32G) {
                                                 // g00, g10 and g01, g11 from the
-                                                // rows y0 and y0 + 1; at the last
-                                                // column or row the pair's second
-                                                // value (weight 0) is the next
-                                                // row's first or the zero padding
+                                                // rows y0 and y0 + 1, and their
+                                                // coefficients as the f32 plane
+                                                // forms them; at the last column or
+                                                // row the pair's second value (weight
+                                                // 0) is the next row's first or the
+                                                // zero padding
                                                 const int32_t off = (int32_t)(
                                                     __umul24(y0, (uint32_t)row4) + (x0 << 2));
                                                 const amvs_u32x2 r0 =
@@ -477,36 +481,39 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                 const float g10 = __uint_as_float(r0.y);
                                                 const float g01 = __uint_as_float(r1.x);
                                                 const float g11 = __uint_as_float(r1.y);
-                                                top = fmaf(ax, g10 - g00, g00);
-                                                bot = fmaf(ax, g11 - g01, g01);
+                                                const float cA = g10 - g00;
+                                                t1 = fmaf(ax, cA, g00);
+                                                u = fmaf(ax, (g11 - g01) - cA, g01 - g00);
                                             } else if constexpr (PLANE == PL_F32) {
                                                 const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
-                                                // g00, g10 - g00, g01, g11 - g01
+                                                // g00, A, B, C
                                                 const amvs_f32x4 e =
                                                     amvs_struct_load_f4(Qh, idx, 0, 0, 0);
-                                                top = fmaf(ax, e.y, e.x);
-                                                bot = fmaf(ax, e.w, e.z);
+                                                t1 = fmaf(ax, e.y, e.x);
+                                                u = fmaf(ax, e.w, e.z);
                                             } else if constexpr (F16Q) {
                                                 const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
-                                                // corners and column differences, f16
+                                                // {g00, A} and {B, C}, f16
                                                 const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
-                                                const uint32_t q0 = qv.x, q1 = qv.y;
-                                                const h2 r0 = __builtin_bit_cast(h2, q0);
-                                                const h2 r1 = __builtin_bit_cast(h2, q1);
-                                                top = fmaf(ax, (float)r0.y, (float)r0.x);
-                                                bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                                                const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
+                                                const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
+                                                t1 = fmaf(ax, (float)r0.y, (float)r0.x);
+                                                u = fmaf(ax, (float)r1.y, (float)r1.x);
                                             } else {
                                                 const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
-                                                // the four corners as bytes
+                                                // the four corners as bytes; the
+                                                // coefficients are exact integers, as
+                                                // in the f16 plane
                                                 const uint32_t qd = amvs_struct_load_b32(Qh, idx, 0, 0, 0);
                                                 const float g00 = (float)(qd & 0xffu);
                                                 const float g10 = (float)((qd >> 8) & 0xffu);
                                                 const float g01 = (float)((qd >> 16) & 0xffu);
                                                 const float g11 = (float)(qd >> 24);
-                                                top = fmaf(ax, g10 - g00, g00);
-                                                bot = fmaf(ax, g11 - g01, g01);
+                                                const float cA = g10 - g00;
+                                                t1 = fmaf(ax, cA, g00);
+                                                u = fmaf(ax, (g11 - g01) - cA, g01 - g00);
                                             }
-                                            const float val = fmaf(ay, bot - top, top);
+                                            const float val = fmaf(ay, u, t1);
                                             if (tx == 0 && ty == 0) c0 = val;
                                             const float dd = val - c0;
                                             s1 += dd;

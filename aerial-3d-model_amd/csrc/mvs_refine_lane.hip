@@ -1,7 +1,7 @@
 // Gauss-Newton refinement with one lane per reference pixel, for views on
-// undistorted cameras: 8-bit views sample the f16 corner plane with the
-// reference window as bytes (lane_ok 1), f32 grey views the f32
-// corner-difference plane with the window in f32 (lane_ok 2). It performs
+// undistorted cameras: 8-bit views sample the f16 bilinear-coefficient plane
+// with the reference window as bytes (lane_ok 1), f32 grey views the f32
+// coefficient plane with the window in f32 (lane_ok 2). It performs
 // the same refinement as k_refine in mvs_refine.hip: LM-damped Gauss-Newton
 // on (depth, two normal angles) of the 1 - NCC cost over the pair of sources
 // that is best at the PatchMatch plane,
@@ -15,8 +15,9 @@
 // window's rays are affine in the tap offsets (undistorted reference), so the
 // plane depth dq = c / (n . r), the source-frame ray w = R r and the
 // derivative factors are affine per tap rather than loaded. Samples come from
-// the f16 corner plane (corners and column differences of the 8-bit image,
-// which also give the image gradient); the reference window lives in LDS as
+// the f16 bilinear-coefficient plane (g00 + ax A + ay (B + ax C) of the 8-bit
+// image, whose A + ay C and B + ax C are the image gradient); the reference
+// window lives in LDS as
 // f16 integers b - mi with the exact correction delta = mean(b) - mi. All
 // sums are in byte units; NCC and its derivatives are scale-invariant.
 //
@@ -144,27 +145,28 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
             const float fy = __builtin_amdgcn_fmed3f(fmaf(F, q1, CYh), 0.0f, hm1);
             const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
             const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
-            // a sample's top and bottom rows as corner and column difference
-            // (f16 bytes of 8-bit views, or f32 grey of f32 views)
-            float t0v, t0d, b0v, b0d;
+            // the cell's bilinear coefficients g00, A, B, C (f16 bytes of 8-bit
+            // views, or f32 grey of f32 views): the sample is t1 + ay u with
+            // t1 = g00 + ax A, u = B + ax C
+            float cg, cA, cB, cC;
             if constexpr (F32) {
                 const float4 e = Qd[__umul24(y0, sw) + x0];
-                t0v = e.x;
-                t0d = e.y;
-                b0v = e.z;
-                b0d = e.w;
+                cg = e.x;
+                cA = e.y;
+                cB = e.z;
+                cC = e.w;
             } else {
                 const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
                 const uint32_t qa = qv.x, qb = qv.y;
                 const h2 r0 = __builtin_bit_cast(h2, qa), r1 = __builtin_bit_cast(h2, qb);
-                t0v = (float)r0.x;
-                t0d = (float)r0.y;
-                b0v = (float)r1.x;
-                b0d = (float)r1.y;
+                cg = (float)r0.x;
+                cA = (float)r0.y;
+                cB = (float)r1.x;
+                cC = (float)r1.y;
             }
-            const float top = fmaf(ax, t0d, t0v);
-            const float bot = fmaf(ax, b0d, b0v);
-            const float val = fmaf(ay, bot - top, top);
+            const float t1 = fmaf(ax, cA, cg);
+            const float u = fmaf(ax, cC, cB);
+            const float val = fmaf(ay, u, t1);
             if (tx == 0 && ty == 0) c0 = val;
             const float sv = val - c0;  // shifted sample
             const float a = (float)Ah[ty * TAPS + tx][lane];
@@ -173,8 +175,8 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
             sa = fmaf(a, sv, sa);
             if constexpr (NDIR > 0) {
                 // image gradient at the sample (per source pixel)
-                const float Iv = bot - top;
-                const float Iu = fmaf(ay, b0d - t0d, t0d);
+                const float Iv = u;
+                const float Iu = fmaf(ay, cC, cA);
                 // d(u, v)/d dq and the chain to depth and the normal angles
                 const float fi = F * inv;
                 const float du = fmaf(-q0, w[2], w[0]) * fi;
@@ -313,19 +315,20 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
                 }
                 const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
                 const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
-                float top, bot;
+                // the sample t1 + ay u, t1 = g00 + ax A, u = B + ax C
+                float t1, u;
                 if constexpr (F32) {
                     const float4 e = Qd[__umul24(y0, sw) + x0];
-                    top = fmaf(ax, e.y, e.x);
-                    bot = fmaf(ax, e.w, e.z);
+                    t1 = fmaf(ax, e.y, e.x);
+                    u = fmaf(ax, e.w, e.z);
                 } else {
                     const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
                     const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
                     const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
-                    top = fmaf(ax, (float)r0.y, (float)r0.x);
-                    bot = fmaf(ax, (float)r1.y, (float)r1.x);
+                    t1 = fmaf(ax, (float)r0.y, (float)r0.x);
+                    u = fmaf(ax, (float)r1.y, (float)r1.x);
                 }
-                const float val = fmaf(ay, bot - top, top);
+                const float val = fmaf(ay, u, t1);
                 if (tx == 0 && ty == 0) c0 = val;
                 const float sv = val - c0;
                 s1 += sv;
