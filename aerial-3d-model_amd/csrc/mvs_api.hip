@@ -535,13 +535,21 @@ std::vector<Schedule> candidates(const KClass &k) {
     return cand;
 }
 
-// Time the half-sweep (it, parity) of the first job of a class per candidate
-// schedule on a copy of its state, keep the fastest. With the exact arithmetic
-// all schedules give bit-identical results; with the fast arithmetic they sum
-// in different orders, so costs differ within the fast tolerance. Speed
-// differs between the first iterations (random planes, scattered gathers) and
-// the converged ones, so the choice is made per iteration.
-int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_t it,
+// Jobs of a class the autotuner times its candidates on, and the runs per
+// candidate (the fastest counts): a few views keep the launch in the
+// many-chunks regime of a batch, and a second run discounts a first run's
+// cold caches.
+constexpr int32_t kTuneJobs = 4, kTuneRuns = 2;
+
+// Time the half-sweep (it, parity) of the first kTuneJobs jobs of a class per
+// candidate schedule on copies of their states, keep the fastest. With the
+// exact arithmetic all schedules give bit-identical results; with the fast
+// arithmetic they sum in different orders, so costs differ within the fast
+// tolerance. Speed differs between the first iterations (random planes,
+// scattered gathers) and the converged ones, so the choice is made per
+// iteration.
+
+int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_t it,
                    int32_t parity) {
     const std::vector<Schedule> cand = candidates(k);
     if (cand.empty()) return fail(c, MVS_ERR_ARG, "no schedule for this class");
@@ -549,44 +557,55 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob &j0, int32_
         k.sch = cand[0];
         return MVS_OK;
     }
-    const size_t n = (size_t)j0.w * j0.h;
+    const int32_t nt = std::min(k.count, kTuneJobs);
+    std::vector<size_t> off((size_t)nt + 1, 0);
+    for (int32_t j = 0; j < nt; ++j) off[(size_t)j + 1] = off[(size_t)j] + (size_t)j0[j].w * j0[j].h;
+    const size_t n = off[(size_t)nt];
     HIP_TRY(c, c->save_hyp.ensure(n * sizeof(float4)));
     HIP_TRY(c, c->save_cost.ensure(n * sizeof(float)));
-    HIP_TRY(c, hipMemcpyAsync(c->save_hyp.p, j0.hyp, n * sizeof(float4), hipMemcpyDeviceToDevice,
-                              c->st));
-    HIP_TRY(c, hipMemcpyAsync(c->save_cost.p, j0.cost, n * sizeof(float), hipMemcpyDeviceToDevice,
-                              c->st));
+    // the tuning jobs' states, saved before and restored before every run
+    auto copy_states = [&](bool save) -> int {
+        for (int32_t j = 0; j < nt; ++j) {
+            const size_t m = off[(size_t)j + 1] - off[(size_t)j];
+            float4 *sh = c->save_hyp.as<float4>() + off[(size_t)j];
+            float *sc = c->save_cost.as<float>() + off[(size_t)j];
+            HIP_TRY(c, hipMemcpyAsync(save ? (void *)sh : (void *)j0[j].hyp,
+                                      save ? (const void *)j0[j].hyp : (const void *)sh,
+                                      m * sizeof(float4), hipMemcpyDeviceToDevice, c->st));
+            HIP_TRY(c, hipMemcpyAsync(save ? (void *)sc : (void *)j0[j].cost,
+                                      save ? (const void *)j0[j].cost : (const void *)sc,
+                                      m * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+        }
+        return MVS_OK;
+    };
+    TRY(copy_states(true));
     hipEvent_t ea = take_event(c), eb = take_event(c);
     float best_ms = 1e30f;
     Schedule best = cand[0];
     KClass one = k;
-    one.count = 1;
-    one.caps.assign(1, (int64_t)j0.w * j0.h);
+    one.count = nt;
+    one.caps.assign(k.caps.begin(), k.caps.begin() + nt);
     for (const Schedule &s : cand) {
         one.sch = s;
         PmArgs a = base_args(c, one);
         a.it = it;
         a.parity = parity;
         a.range = ldexpf(1.0f, -it);
-        HIP_TRY(c, hipMemcpyAsync(j0.hyp, c->save_hyp.p, n * sizeof(float4),
-                                  hipMemcpyDeviceToDevice, c->st));
-        HIP_TRY(c, hipMemcpyAsync(j0.cost, c->save_cost.p, n * sizeof(float),
-                                  hipMemcpyDeviceToDevice, c->st));
-        HIP_TRY(c, hipEventRecord(ea, c->st));
-        TRY(pm_launch(c, PM_SWEEP, taps, one, a));
-        HIP_TRY(c, hipEventRecord(eb, c->st));
-        HIP_TRY(c, hipEventSynchronize(eb));
-        float ms = 0.0f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, ea, eb));
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = s;
+        for (int32_t r = 0; r < kTuneRuns; ++r) {
+            TRY(copy_states(false));
+            HIP_TRY(c, hipEventRecord(ea, c->st));
+            TRY(pm_launch(c, PM_SWEEP, taps, one, a));
+            HIP_TRY(c, hipEventRecord(eb, c->st));
+            HIP_TRY(c, hipEventSynchronize(eb));
+            float ms = 0.0f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, ea, eb));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = s;
+            }
         }
     }
-    HIP_TRY(c, hipMemcpyAsync(j0.hyp, c->save_hyp.p, n * sizeof(float4), hipMemcpyDeviceToDevice,
-                              c->st));
-    HIP_TRY(c, hipMemcpyAsync(j0.cost, c->save_cost.p, n * sizeof(float), hipMemcpyDeviceToDevice,
-                              c->st));
+    TRY(copy_states(false));
     c->free_events.push_back(ea);
     c->free_events.push_back(eb);
     k.sch = best;
@@ -929,7 +948,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
                 if (f != c->tuned.end()) {
                     k.sch = f->second;
                 } else if (c->autotune) {
-                    TRY(autotune_class(c, k.taps, k, table[(size_t)k.first], it, parity));
+                    TRY(autotune_class(c, k.taps, k, table.data() + k.first, it, parity));
                     c->tuned[key] = k.sch;
                     known = false;
                 } else {

@@ -419,8 +419,7 @@ def run_uneven(args):
                                 refine_steps=args.refine_steps, fast=args.mode == "fast"))
     ctx = capi.Context(0)
     sv = capi.Survey(ctx, surv.views)
-    for _ in range(max(1, args.warmup)):
-        sv.depth_maps(jobs, prms)
+    sv.depth_maps(jobs, prms)  # warm-up: one untimed pass tunes the schedules
     ctx.synchronize()
     ctx.reset_stats()
     ctx.set_timing(True)
@@ -441,7 +440,7 @@ def run_uneven(args):
     line = dict(
         metric="depth Mpix/s, batched uneven survey (one call, per-view K and window)",
         value=pixels / elapsed / 1e6, unit="depth Mpix/s", n_gpus=1, steps=n,
-        warmup=max(1, args.warmup), ms_per_step=elapsed * 1e3 / n, higher_is_better=True,
+        warmup=n, ms_per_step=elapsed * 1e3 / n, higher_is_better=True,
         scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(workload=(f"{spec.nadir_lines * spec.nadir_per_line} nadir "
                               f"{spec.nadir_size[0]}x{spec.nadir_size[1]} + {spec.oblique} "
@@ -600,7 +599,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0,
                     help="reference views timed (default 0: every view of the survey)")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed warm-up views, run as one batch (it tunes the schedules)")
     ap.add_argument("--config", default="c2")
     ap.add_argument("--mode", choices=("fast", "exact"), default="fast",
                     help="arithmetic of the batched path's PatchMatch kernels")
@@ -670,9 +670,12 @@ def main():
     ctx.synchronize()
     upload_s = time.perf_counter() - tu
     log(f"views resident in HBM (upload and planes: {upload_s:.2f} s)")
-    for w in range(args.warmup):
-        j = jobs[w % len(jobs)]
-        sv.depth_maps([j], prm)
+    # warm-up: the first W views as one batch (the autotuner times its
+    # candidate schedules on up to four of them, per iteration), and one view
+    # of the plain path
+    if args.warmup > 0:
+        sv.depth_maps(jobs[:min(args.warmup, len(jobs))], prm)
+        j = jobs[0]
         sv.depth_map(j[0], j[1], j[2][0], j[2][1], prm_exact)
     ctx.synchronize()
     ctx.reset_stats()

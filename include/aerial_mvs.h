@@ -189,7 +189,7 @@ int mvs_survey_set_view(mvs_survey *sv, int32_t i, const mvs_view *v);
  * of thousands of views stay in one GPU's memory). They apply to views set
  * and maps computed after the call.
  * MVS_SURVEY_LEAN_VIEWS: an 8-bit view keeps its f32 grey and its 8-bit and
- *   f16 corner planes only (16 bytes per pixel instead of 32); jobs that
+ *   f16 coefficient planes only (16 bytes per pixel instead of 32); jobs that
  *   touch such a view must run in fast mode on undistorted views (the
  *   one-lane-per-pixel kernels), others fail with MVS_ERR_ARG.
  * MVS_SURVEY_DEPTH_ONLY: maps keep the depth only (4 bytes per pixel instead
@@ -298,7 +298,7 @@ typedef struct {
     int64_t init_evals;       /* of which in the initialisation */
     int64_t sweep_evals;      /* of which in the half-sweeps */
     int32_t split;            /* schedule of the last batch or cost list: lanes
-                               * per source (-1: one lane per pixel, f16 corner
+                               * per source (-1: one lane per pixel, f16 coefficient
                                * plane; -2: one lane per pixel, 8-bit plane;
                                * -3: one lane per pixel, f32 plane) */
     int32_t gpc;              /* and pixel groups per work chunk */
@@ -323,15 +323,15 @@ int mvs_set_timing(mvs_ctx *ctx, int32_t enable);
  * not depend on it, with MVS_MODE_FAST they vary within the fast tolerance):
  * split = lanes sharing one source's window (0 = choose automatically; -1 =
  * one lane per reference pixel, for MVS_MODE_FAST with undistorted 8-bit
- * reference and source views, sampling an f16 corner plane; -2 = the same
+ * reference and source views, sampling an f16 coefficient plane; -2 = the same
  * sampling the 8-bit corner plane; -3 = the same for undistorted f32 grey
- * views, sampling the f32 corner-difference plane; -4 = the same sampling
+ * views, sampling the f32 coefficient plane; -4 = the same sampling
  * the f32 grey plane itself; a class the forced split does not apply to
  * takes its first candidate), gpc = pixel
  * groups per work chunk (0 = default), per_cu = persistent workgroups per
  * compute unit (0 = one workgroup per chunk). autotune = 1 times the candidate
- * schedules per kernel class and iteration on the first batch and keeps the
- * fastest. */
+ * schedules per kernel class and iteration on up to four views of the first
+ * batch that has the class (twice each) and keeps the fastest. */
 int mvs_set_schedule(mvs_ctx *ctx, int32_t split, int32_t gpc, int32_t per_cu,
                      int32_t autotune);
 /* Half-sweep graphs (no reference counterpart; results do not depend on it):
