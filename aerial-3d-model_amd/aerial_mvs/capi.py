@@ -75,7 +75,8 @@ class MvsStats(C.Structure):
                 ("refine_launches", C.c_int64), ("refine_ms", C.c_double),
                 ("refine_pixels", C.c_int64), ("refine_accepted", C.c_int64),
                 ("graph_captures", C.c_int64), ("graph_replays", C.c_int64),
-                ("fuse_passes", C.c_int64), ("sweep_kernels", C.c_int64)]
+                ("fuse_passes", C.c_int64), ("sweep_kernels", C.c_int64),
+                ("sweep_skipped", C.c_int64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32)

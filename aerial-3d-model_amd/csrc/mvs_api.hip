@@ -76,7 +76,7 @@ struct MapDev {
 
 // Work buffers of one job slot of a batch.
 struct JobWork {
-    DevBuf hyp, cost, textured, list, counts, srcs;
+    DevBuf hyp, cost, textured, changed, list, counts, srcs;
 };
 
 // One reference view to process.
@@ -151,7 +151,7 @@ struct mvs_ctx {
     DevBuf up_u8;                    // staging of u8 uploads
     std::vector<ViewDev> tmp_views;  // host-view calls
     MapDev tmp_map;
-    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost;
+    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost, save_changed;
     DevBuf prep_box, prep_rgb, prep_grey;  // view preparation
     DevBuf dsm_xyz, dsm_h, dsm_a, dsm_b, dsm_c, dsm_d, dsm_snap, dsm_parent, dsm_la, dsm_lb,
         dsm_ctr, dsm_sel;  // surface model
@@ -247,10 +247,15 @@ void collect_timing(mvs_ctx *c) {
     c->pending.clear();
 }
 
+// Device counters (c->evals): [0] initial-cost evaluations, [1] half-sweep
+// evaluations, [2] accepted refinement steps, [3] refined pixels, [4]
+// half-sweep candidates skipped as scored and not taken before.
+constexpr int kEvalSlots = 5;
+
 int sync_ctx(mvs_ctx *c) {
     HIP_TRY(c, hipStreamSynchronize(c->st));
     collect_timing(c);
-    unsigned long long ev[4] = {0, 0, 0, 0};
+    unsigned long long ev[kEvalSlots] = {0, 0, 0, 0, 0};
     if (c->evals.p) {
         HIP_TRY(c, hipMemcpy(ev, c->evals.p, sizeof ev, hipMemcpyDeviceToHost));
         HIP_TRY(c, hipMemset(c->evals.p, 0, sizeof ev));
@@ -260,6 +265,7 @@ int sync_ctx(mvs_ctx *c) {
     c->stats.evals += (int64_t)(ev[0] + ev[1]);
     c->stats.refine_accepted += (int64_t)ev[2];
     c->stats.refine_pixels += (int64_t)ev[3];
+    c->stats.sweep_skipped += (int64_t)ev[4];
     c->host_jobs.clear();
     c->host_tables.clear();
     return MVS_OK;
@@ -550,7 +556,7 @@ constexpr int32_t kTuneJobs = 4, kTuneRuns = 2;
 // iteration.
 
 int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_t it,
-                   int32_t parity) {
+                   int32_t parity, bool skip_unchanged) {
     const std::vector<Schedule> cand = candidates(k);
     if (cand.empty()) return fail(c, MVS_ERR_ARG, "no schedule for this class");
     if (cand.size() == 1) {
@@ -563,18 +569,20 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_
     const size_t n = off[(size_t)nt];
     HIP_TRY(c, c->save_hyp.ensure(n * sizeof(float4)));
     HIP_TRY(c, c->save_cost.ensure(n * sizeof(float)));
-    // the tuning jobs' states, saved before and restored before every run
+    HIP_TRY(c, c->save_changed.ensure(n));
+    // the tuning jobs' states (hypotheses, costs, change flags), saved before
+    // and restored before every run
     auto copy_states = [&](bool save) -> int {
         for (int32_t j = 0; j < nt; ++j) {
             const size_t m = off[(size_t)j + 1] - off[(size_t)j];
-            float4 *sh = c->save_hyp.as<float4>() + off[(size_t)j];
-            float *sc = c->save_cost.as<float>() + off[(size_t)j];
-            HIP_TRY(c, hipMemcpyAsync(save ? (void *)sh : (void *)j0[j].hyp,
-                                      save ? (const void *)j0[j].hyp : (const void *)sh,
-                                      m * sizeof(float4), hipMemcpyDeviceToDevice, c->st));
-            HIP_TRY(c, hipMemcpyAsync(save ? (void *)sc : (void *)j0[j].cost,
-                                      save ? (const void *)j0[j].cost : (const void *)sc,
-                                      m * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+            void *sv[3] = {c->save_hyp.as<float4>() + off[(size_t)j],
+                           c->save_cost.as<float>() + off[(size_t)j],
+                           c->save_changed.as<uint8_t>() + off[(size_t)j]};
+            void *jv[3] = {j0[j].hyp, j0[j].cost, j0[j].changed};
+            const size_t bytes[3] = {m * sizeof(float4), m * sizeof(float), m};
+            for (int q = 0; q < 3; ++q)
+                HIP_TRY(c, hipMemcpyAsync(save ? sv[q] : jv[q], save ? jv[q] : sv[q], bytes[q],
+                                          hipMemcpyDeviceToDevice, c->st));
         }
         return MVS_OK;
     };
@@ -591,6 +599,7 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_
         a.it = it;
         a.parity = parity;
         a.range = ldexpf(1.0f, -it);
+        a.skip_unchanged = skip_unchanged ? 1 : 0;
         for (int32_t r = 0; r < kTuneRuns; ++r) {
             TRY(copy_states(false));
             HIP_TRY(c, hipEventRecord(ea, c->st));
@@ -649,7 +658,8 @@ struct SweepLaunch {
 };
 
 int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32_t it,
-                   int32_t parity, unsigned long long *evals, std::vector<SweepLaunch> &out) {
+                   int32_t parity, bool skip_unchanged, unsigned long long *evals,
+                   unsigned long long *skipped, std::vector<SweepLaunch> &out) {
     out.clear();
     for (const KClass *kp : classes) {
         const KClass &k = *kp;
@@ -660,6 +670,8 @@ int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32
         L.a.parity = parity;
         L.a.range = ldexpf(1.0f, -it);  // 0.5f32.powi(it), dense.rs:414
         L.a.evals = evals;
+        L.a.skip_unchanged = skip_unchanged ? 1 : 0;
+        L.a.skipped = skipped;
         L.persist = k.sch.per_cu > 0;
         L.grid = grid_for(c, k, L.persist != 0,
                           [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
@@ -697,6 +709,8 @@ int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
         put(&L.a.range, sizeof L.a.range);
         put(&L.a.gpc, sizeof L.a.gpc);
         put(&L.a.evals, sizeof L.a.evals);
+        put(&L.a.skip_unchanged, sizeof L.a.skip_unchanged);
+        put(&L.a.skipped, sizeof L.a.skipped);
         const int32_t sel[7] = {L.grid, L.srcl, L.split, L.pinhole, L.persist, L.taps, L.fast};
         put(sel, sizeof sel);
     }
@@ -758,8 +772,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
     }
     if ((int32_t)c->work.size() < nj) c->work.resize((size_t)nj);
     if (!c->evals.p) {
-        HIP_TRY(c, c->evals.ensure(4 * sizeof(unsigned long long)));
-        HIP_TRY(c, hipMemset(c->evals.p, 0, 4 * sizeof(unsigned long long)));
+        HIP_TRY(c, c->evals.ensure(kEvalSlots * sizeof(unsigned long long)));
+        HIP_TRY(c, hipMemset(c->evals.p, 0, kEvalSlots * sizeof(unsigned long long)));
     }
     HIP_TRY(c, c->prefix.ensure(sizeof(int32_t) * 9 * ((size_t)nj + 1)));
     HIP_TRY(c, c->counters.ensure(sizeof(int32_t) * 8));
@@ -778,6 +792,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         HIP_TRY(c, wk.hyp.ensure((size_t)n * sizeof(float4)));
         HIP_TRY(c, wk.cost.ensure((size_t)n * sizeof(float)));
         HIP_TRY(c, wk.textured.ensure((size_t)n));
+        HIP_TRY(c, wk.changed.ensure((size_t)n));
         HIP_TRY(c, wk.list.ensure((size_t)(2 * cap) * sizeof(int32_t)));
         HIP_TRY(c, wk.counts.ensure((4 + 2 * (size_t)list_waves(ref.w, ref.h)) * sizeof(int32_t)));
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
@@ -804,6 +819,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         job.hyp = wk.hyp.as<float4>();
         job.cost = wk.cost.as<float>();
         job.textured = wk.textured.as<uint8_t>();
+        job.changed = wk.changed.as<uint8_t>();
         job.list = wk.list.as<int32_t>();
         job.counts = wk.counts.as<int32_t>();
         job.srcs = wk.srcs.as<SrcCam>();
@@ -948,7 +964,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
                 if (f != c->tuned.end()) {
                     k.sch = f->second;
                 } else if (c->autotune) {
-                    TRY(autotune_class(c, k.taps, k, table.data() + k.first, it, parity));
+                    TRY(autotune_class(c, k.taps, k, table.data() + k.first, it, parity,
+                                       s >= s0 + 2));
                     c->tuned[key] = k.sch;
                     known = false;
                 } else {
@@ -962,7 +979,10 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
             eb = take_event(c);
             HIP_TRY(c, hipEventRecord(ea, c->st));
         }
-        TRY(sweep_launches(c, active, it, parity, evals + 1, ls));
+        // both colours have had a half-sweep of this call from s0 + 2 on: the
+        // candidates scored two half-sweeps before are known from then
+        const bool skip_unchanged = s >= s0 + 2;
+        TRY(sweep_launches(c, active, it, parity, skip_unchanged, evals + 1, evals + 4, ls));
         c->stats.sweep_kernels += (int64_t)ls.size();
         if (c->graphs && known)
             TRY(run_sweeps_graph(c, ls));
@@ -1158,12 +1178,13 @@ void mvs_destroy(mvs_ctx *c) {
         delete kv.second;
     }
     for (auto &wk : c->work) {
-        DevBuf *b[] = {&wk.hyp, &wk.cost, &wk.textured, &wk.list, &wk.counts, &wk.srcs};
+        DevBuf *b[] = {&wk.hyp, &wk.cost, &wk.textured, &wk.changed, &wk.list, &wk.counts,
+                       &wk.srcs};
         for (DevBuf *x : b) x->release();
     }
     DevBuf *bufs[] = {&c->jobs_dev, &c->prefix, &c->counters, &c->evals, &c->ok_count,
                       &c->up_u8,    &c->px,     &c->py,       &c->hyp_in, &c->cost_out,
-                      &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost,
+                      &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost, &c->save_changed,
                       &c->pairs,    &c->grad_io, &c->gout,
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost,
                       &c->prep_box, &c->prep_rgb, &c->prep_grey,

@@ -44,6 +44,9 @@ struct RefJob {
     float4 *hyp;          // w*h {d, n0, n1, n2}
     float *cost;          // w*h
     uint8_t *textured;    // w*h
+    // w*h: 1 where the pixel's last half-sweep changed its hypothesis (0 from
+    // the initialisation on; untextured pixels never change)
+    uint8_t *changed;
     int32_t *list;        // 2 * list_cap textured pixel indices, by parity
     int32_t *counts;      // [2] list lengths
     const SrcCam *srcs;   // nsrc
@@ -71,6 +74,13 @@ struct PmArgs {
     float range;          // 0.5^it
     int32_t gpc;          // pixel groups per work chunk
     unsigned long long *evals;
+    // Half-sweeps: skip_unchanged = 1 when both colours have had a half-sweep
+    // since the state was set, so a propagation candidate from a neighbour
+    // whose last half-sweep left it unchanged was scored by this pixel two
+    // half-sweeps before (and kept or rejected); such candidates are counted
+    // into *skipped rather than scored (one-lane-per-pixel kernels).
+    int32_t skip_unchanged;
+    unsigned long long *skipped;
     const int32_t *px, *py;  // explicit (pixel, hypothesis) list
     const float4 *hyp_in;
     float *out;

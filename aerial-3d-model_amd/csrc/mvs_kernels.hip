@@ -207,6 +207,7 @@ __global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t 
             tex = sqrtf(var) >= job.texture_min;
         }
         job.textured[i] = tex ? 1 : 0;
+        job.changed[i] = 0;
     }
     const int parity = (x + y) & 1;
     const uint64_t m0 = __ballot(valid && tex && parity == 0);

@@ -459,7 +459,8 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                     const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
                     if (have && r == 0) job.cost[pidx] = cst;
                 } else {
-                    float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    const float4 orig = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                    float4 best = orig;
                     float bcost = have ? job.cost[pidx] : 2.0f;
                     // Ten proposals in the reference's order, one evaluation
                     // site. k < 8: propagation from the other colour
@@ -504,6 +505,9 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                     if (have && r == 0) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
+                        // (float ==: a NaN component counts as a change)
+                        job.changed[pidx] = !(best.x == orig.x && best.y == orig.y &&
+                                              best.z == orig.z && best.w == orig.w);
                     }
                 }
             }

@@ -138,6 +138,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     const int32_t nj = a.njobs;
     const int home = blockIdx.x % kNumXcd;
     unsigned long long n_evals = 0;
+    uint32_t n_skipped = 0;  // wave-uniform (a scalar register)
 
     for (int attempt = 0; attempt < (PERSIST ? kNumXcd : 1); ++attempt) {
         const int xcd = (home + attempt) % kNumXcd;
@@ -425,15 +426,24 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 auto taps = [&](auto clamp_tag, auto tile_tag) {
                                     constexpr bool CLAMP = decltype(clamp_tag)::value;
                                     constexpr bool TL = decltype(tile_tag)::value;
-                                    // one tap row per trip: a row's gathers are in
-                                    // flight together while the registers they hold
-                                    // leave room for several wavefronts per SIMD
+                                    // one tap row per trip: the row's positions,
+                                    // then all its gathers, then its samples and
+                                    // sums, so that a row's gathers are in flight
+                                    // together (a scheduling barrier keeps the
+                                    // compiler from sinking each gather to its
+                                    // use, which it otherwise does in some
+                                    // instantiations); the registers they hold
+                                    // leave room for four wavefronts per SIMD
+                                    constexpr int QW = PLANE == PL_F32 ? 4 : (PLANE == PL_U8 ? 1 : 2);
+                                    constexpr int QN = PLANE == PL_F32G ? 2 : 1;  // loads per tap
 #pragma unroll 1
                                     for (int ty = 0; ty < TAPS; ++ty) {
                                         const float fdy = (float)(ty - H);
                                         const float Y0 = fmaf(fdy, B2[0], B0[0]);
                                         const float Y1 = fmaf(fdy, B2[1], B0[1]);
                                         const float Y2 = fmaf(fdy, B2[2], B0[2]);
+                                        float axs[TAPS], ays[TAPS];
+                                        uint32_t q[TAPS][QN][QW];
 #pragma unroll
                                         for (int tx = 0; tx < TAPS; ++tx) {
                                             const float fdx = (float)(tx - H);
@@ -447,11 +457,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                 fy = __builtin_amdgcn_fmed3f(fy, 0.0f, hm1);
                                             }
                                             const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
-                                            const float ax = __builtin_amdgcn_fractf(fx),
-                                                        ay = __builtin_amdgcn_fractf(fy);
-                                            // the sample g00 + ax A + ay (B + ax C) as
-                                            // t1 + ay u, t1 = g00 + ax A, u = B + ax C
-                                            float t1, u;
+                                            axs[tx] = __builtin_amdgcn_fractf(fx);
+                                            ays[tx] = __builtin_amdgcn_fractf(fy);
                                             if constexpr (TL) {
                                                 // the staged texel (index clamped into
                                                 // the tile: a safety bound only)
@@ -459,52 +466,73 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                                     (uint32_t)((int32_t)(__umul24(y0, tw) + x0) - tbase),
                                                     (uint32_t)(tcnt - 1));
                                                 const u32x2 qv = tile[ti];
-                                                const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
-                                                const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
+                                                q[tx][0][0] = qv.x;
+                                                q[tx][0][QW - 1] = qv.y;
+                                            } else if constexpr (PLANE == PL_F32G) {
+                                                // rows y0 and y0 + 1: g00, g10 and
+                                                // g01, g11 (at the last column or row
+                                                // the pair's second value, weight 0,
+                                                // is the next row's first or the
+                                                // zero padding)
+                                                const int32_t off = (int32_t)(
+                                                    __umul24(y0, (uint32_t)row4) + (x0 << 2));
+                                                const amvs_u32x2 r0 = amvs_raw_load_b64(Qh, off, 0, 0);
+                                                const amvs_u32x2 r1 = amvs_raw_load_b64(Qh, off, row4, 0);
+                                                q[tx][0][0] = r0.x;
+                                                q[tx][0][QW - 1] = r0.y;
+                                                q[tx][QN - 1][0] = r1.x;
+                                                q[tx][QN - 1][QW - 1] = r1.y;
+                                            } else if constexpr (PLANE == PL_F32) {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
+                                                const amvs_f32x4 e = amvs_struct_load_f4(Qh, idx, 0, 0, 0);
+                                                q[tx][0][0] = __float_as_uint(e.x);
+                                                q[tx][0][1 % QW] = __float_as_uint(e.y);
+                                                q[tx][0][2 % QW] = __float_as_uint(e.z);
+                                                q[tx][0][3 % QW] = __float_as_uint(e.w);
+                                            } else if constexpr (F16Q) {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
+                                                const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
+                                                q[tx][0][0] = qv.x;
+                                                q[tx][0][QW - 1] = qv.y;
+                                            } else {
+                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
+                                                q[tx][0][0] = amvs_struct_load_b32(Qh, idx, 0, 0, 0);
+                                            }
+                                        }
+                                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                                        for (int tx = 0; tx < TAPS; ++tx) {
+                                            const float ax = axs[tx], ay = ays[tx];
+                                            // the sample g00 + ax A + ay (B + ax C) as
+                                            // t1 + ay u, t1 = g00 + ax A, u = B + ax C
+                                            float t1, u;
+                                            if constexpr (TL || F16Q) {
+                                                // {g00, A} and {B, C}, f16
+                                                const h2 r0 = __builtin_bit_cast(h2, q[tx][0][0]);
+                                                const h2 r1 = __builtin_bit_cast(h2, q[tx][0][QW - 1]);
                                                 t1 = fmaf(ax, (float)r0.y, (float)r0.x);
                                                 u = fmaf(ax, (float)r1.y, (float)r1.x);
                                             } else if constexpr (PLANE == PL_F32G) {
-                                                // g00, g10 and g01, g11 from the
-                                                // rows y0 and y0 + 1, and their
-                                                // coefficients as the f32 plane
-                                                // forms them; at the last column or
-                                                // row the pair's second value (weight
-                                                // 0) is the next row's first or the
-                                                // zero padding
-                                                const int32_t off = (int32_t)(
-                                                    __umul24(y0, (uint32_t)row4) + (x0 << 2));
-                                                const amvs_u32x2 r0 =
-                                                    amvs_raw_load_b64(Qh, off, 0, 0);
-                                                const amvs_u32x2 r1 =
-                                                    amvs_raw_load_b64(Qh, off, row4, 0);
-                                                const float g00 = __uint_as_float(r0.x);
-                                                const float g10 = __uint_as_float(r0.y);
-                                                const float g01 = __uint_as_float(r1.x);
-                                                const float g11 = __uint_as_float(r1.y);
+                                                // the coefficients as the f32 plane
+                                                // forms them
+                                                const float g00 = __uint_as_float(q[tx][0][0]);
+                                                const float g10 = __uint_as_float(q[tx][0][QW - 1]);
+                                                const float g01 = __uint_as_float(q[tx][QN - 1][0]);
+                                                const float g11 = __uint_as_float(q[tx][QN - 1][QW - 1]);
                                                 const float cA = g10 - g00;
                                                 t1 = fmaf(ax, cA, g00);
                                                 u = fmaf(ax, (g11 - g01) - cA, g01 - g00);
                                             } else if constexpr (PLANE == PL_F32) {
-                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // g00, A, B, C
-                                                const amvs_f32x4 e =
-                                                    amvs_struct_load_f4(Qh, idx, 0, 0, 0);
-                                                t1 = fmaf(ax, e.y, e.x);
-                                                u = fmaf(ax, e.w, e.z);
-                                            } else if constexpr (F16Q) {
-                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
-                                                // {g00, A} and {B, C}, f16
-                                                const u32x2 qv = amvs_struct_load_b64(Qh, idx, 0, 0, 0);
-                                                const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
-                                                const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
-                                                t1 = fmaf(ax, (float)r0.y, (float)r0.x);
-                                                u = fmaf(ax, (float)r1.y, (float)r1.x);
+                                                t1 = fmaf(ax, __uint_as_float(q[tx][0][1 % QW]),
+                                                          __uint_as_float(q[tx][0][0]));
+                                                u = fmaf(ax, __uint_as_float(q[tx][0][3 % QW]),
+                                                         __uint_as_float(q[tx][0][2 % QW]));
                                             } else {
-                                                const int32_t idx = (int32_t)(__umul24(y0, sw) + x0);
                                                 // the four corners as bytes; the
                                                 // coefficients are exact integers, as
                                                 // in the f16 plane
-                                                const uint32_t qd = amvs_struct_load_b32(Qh, idx, 0, 0, 0);
+                                                const uint32_t qd = q[tx][0][0];
                                                 const float g00 = (float)(qd & 0xffu);
                                                 const float g10 = (float)((qd >> 8) & 0xffu);
                                                 const float g01 = (float)((qd >> 16) & 0xffu);
@@ -583,6 +611,9 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // compiler schedules that way.
                     float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     float bcost = have ? job.cost[pidx] : 2.0f;
+                    // a candidate is taken only at a strictly lower cost, so
+                    // the hypothesis changed exactly when one was taken
+                    bool taken = false;
                     uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
                     const float span = job.imax - job.imin;
 #pragma unroll 1
@@ -596,6 +627,11 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             hp = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                             if (inb) hp = job.hyp[(int64_t)ny * w + nx];
                             act = inb && !same_hyp(hp, best);
+                            // scored and not taken two half-sweeps before
+                            const bool skip =
+                                act && a.skip_unchanged && !job.changed[(int64_t)ny * w + nx];
+                            n_skipped += (uint32_t)__popcll(__ballot(skip));
+                            act = act && !skip;
                         } else {
                             float stp = (rnd_next(s) - 0.5f) * a.range;
                             stp = stp * 0.25f;
@@ -612,11 +648,13 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         if (act && cst < bcost) {
                             best = hp;
                             bcost = cst;
+                            taken = true;
                         }
                     }
                     if (have) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
+                        job.changed[pidx] = taken;
                     }
                 } else {
                     // the ten proposals of dense.rs:427-462: the eight
@@ -624,6 +662,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // refinements
                     float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     float bcost = have ? job.cost[pidx] : 2.0f;
+                    bool taken = false;  // (at a strictly lower cost: a change)
                     // A candidate equal to the current best is not scored
                     // (dense.rs:436-438); one equal to an earlier candidate
                     // cannot be accepted either (its cost is that candidate's,
@@ -635,6 +674,25 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // and keeps each pixel's lowest (cost, index) in LDS.
                     uint32_t amask = 0, bins = 0;
                     const bool by_depth = a.it < kSortIters;  // uniform
+                    // A candidate from a neighbour that its last half-sweep
+                    // left unchanged was scored at this pixel two half-sweeps
+                    // before and not taken, or taken (and is the best or was
+                    // beaten since): its cost is what it was, and the best
+                    // cost can only have fallen, so it cannot be taken now.
+                    // (The flags are read in a loop of their own: read among
+                    // the candidates' loads, the compiler serialises the
+                    // taps' gathers.)
+                    uint32_t unchanged = 0;
+                    if (a.skip_unchanged && pixel_ok) {
+#pragma unroll 1
+                        for (int k = 0; k < 8; ++k) {
+                            int nx, ny;
+                            cand_pos(k, x, y, nx, ny);
+                            if (nx >= 0 && ny >= 0 && nx < w && ny < h &&
+                                !job.changed[(int64_t)ny * w + nx])
+                                unchanged |= 1u << k;
+                        }
+                    }
                     const float bscale = (float)kDepthBins / (job.imax - job.imin);
                     {
                         float4 cand[8];
@@ -650,6 +708,9 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
 #pragma unroll
                             for (int j = 0; j < k; ++j)
                                 act = act && !(((inbm >> j) & 1u) && same_hyp(cand[j], cand[k]));
+                            const bool skip = act && ((unchanged >> k) & 1u);
+                            n_skipped += (uint32_t)__popcll(__ballot(skip));
+                            act = act && !skip;
                             inbm |= (inb ? 1u : 0u) << k;
                             amask |= (act ? 1u : 0u) << k;
                             if (by_depth) {
@@ -734,6 +795,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             cand_pos((int)(pm & 7ull), x, y, nx, ny);
                             best = job.hyp[(int64_t)ny * w + nx];
                             bcost = cst;
+                            taken = true;
                         }
                     }
                     // the two random refinements from the best after
@@ -756,11 +818,13 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         if (act && cst < bcost) {
                             best = hp;
                             bcost = cst;
+                            taken = true;
                         }
                     }
                     if (have) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
+                        job.changed[pidx] = taken;
                     }
                 }
             }
@@ -773,6 +837,8 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
         for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off);
         if (lane == 0 && e) atomicAdd(a.evals, e);
     }
+    if (MODE == MODE_SWEEP && a.skipped && lane == 0 && n_skipped)
+        atomicAdd(a.skipped, (unsigned long long)n_skipped);
 }
 
 template <int TAPS, int MODE, bool PERSIST, int PLANE>

@@ -315,6 +315,13 @@ typedef struct {
                                * class (window, projection, mode, and for the
                                * source-lane kernels the neighbour-count
                                * bucket) per half-sweep of a batch */
+    int64_t sweep_skipped;    /* propagation candidates the one-lane-per-pixel
+                               * half-sweeps did not score: the neighbour's
+                               * last half-sweep left it unchanged, so this
+                               * pixel scored it two half-sweeps before and did
+                               * not take it (or took it and has it or a better
+                               * one): it cannot be taken now, and the state
+                               * is what scoring it would leave */
 } mvs_stats;
 
 int mvs_set_timing(mvs_ctx *ctx, int32_t enable);

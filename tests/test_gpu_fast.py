@@ -253,6 +253,43 @@ def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
     assert (maps[splits[0]][0] > 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("grey,split", [("u8", -1), ("u8", -2), ("f32", -3)])
+def test_skipped_candidates_change_nothing(gpu_ctx, grey, split):
+    """A propagation candidate from a neighbour that its last half-sweep left
+    unchanged is not scored again by the one-lane-per-pixel kernels (it was
+    scored two half-sweeps before and could not be taken now). Half-sweeps
+    run in one call (where that applies from the third on) must land on the
+    states of the same half-sweeps run one call each (each from a state the
+    caller sets, where it never applies), bit for bit, and the candidates
+    skipped plus those scored must be the ones scored there."""
+    s = survey_cached("small")
+    spec = s.spec
+    ref = s.reference_index()
+    views = s.views if grey == "u8" else _f32_views(s, 43)
+    v = views[ref]
+    srcs = [views[j] for j in s.neighbours[ref][:8]]
+    iters = 4
+    prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=iters, fast=True)
+    gpu_ctx.set_schedule(split, 1, 16, False)
+    try:
+        _say(f"skip: small {grey} split={split}, {2 * iters} half-sweeps in one call")
+        gpu_ctx.reset_stats()
+        st_a, _, ev_a = gpu_ctx.patchmatch_steps(v, srcs, *spec.depth_range, prm, -1, 2 * iters)
+        skipped = gpu_ctx.stats()["sweep_skipped"]
+        _say("skip: the same half-sweeps one call each")
+        st_b, _, ev_b = gpu_ctx.patchmatch_steps(v, srcs, *spec.depth_range, prm, -1, 0)
+        for step in range(2 * iters):
+            st_b, _, e = gpu_ctx.patchmatch_steps(v, srcs, *spec.depth_range, prm, step, 1,
+                                                  state=st_b)
+            ev_b += e
+    finally:
+        gpu_ctx.set_schedule(0, 0, 16, True)
+    _say(f"  scored {ev_a} + skipped {skipped}, one call each {ev_b}")
+    assert skipped > 0
+    assert np.array_equal(st_a, st_b)
+    assert ev_a + skipped == ev_b
+
+
 def test_pixel_lane_schedule_leaves_exact_mode_exact(gpu_ctx, tiny_survey):
     """Forcing split -1 applies to fast classes only; exact-mode costs stay
     bit-identical to the oracle's."""
