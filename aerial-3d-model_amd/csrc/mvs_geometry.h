@@ -20,13 +20,17 @@ __device__ inline int32_t agree_pixel(const RefCam64 &rf, const float *__restric
     const int64_t y = i / w, x = i - y * w;
     const double qx = ((double)x + 0.5 - rf.cx) / rf.f;
     const double qy = ((double)y + 0.5 - rf.cy) / rf.f;
+    // (undistorted camera: every fixed-point step divides by exactly 1.0 and
+    // returns q itself, so the steps are skipped without changing a bit)
     double px = qx, py = qy;
-    for (int k = 0; k < 8; ++k) {
-        const double r2 = px * px + py * py;
-        double d = 1.0 + rf.k1 * r2;
-        d = d + (rf.k2 * r2) * r2;
-        px = qx / d;
-        py = qy / d;
+    if (rf.k1 != 0.0 || rf.k2 != 0.0) {
+        for (int k = 0; k < 8; ++k) {
+            const double r2 = px * px + py * py;
+            double d = 1.0 + rf.k1 * r2;
+            d = d + (rf.k2 * r2) * r2;
+            px = qx / d;
+            py = qy / d;
+        }
     }
     const double dd = (double)di;
     const double xc0 = px * dd, xc1 = py * dd, xc2 = dd;
@@ -68,12 +72,14 @@ __device__ inline int32_t agree_pixel(const RefCam64 &rf, const float *__restric
         // back-project the neighbour pixel centre at its own depth
         const double nqx = (fx + 0.5 - v.cx) / v.f, nqy = (fy + 0.5 - v.cy) / v.f;
         double npx = nqx, npy = nqy;
-        for (int k2 = 0; k2 < 8; ++k2) {
-            const double r2n = npx * npx + npy * npy;
-            double dd = 1.0 + v.k1 * r2n;
-            dd = dd + (v.k2 * r2n) * r2n;
-            npx = nqx / dd;
-            npy = nqy / dd;
+        if (v.k1 != 0.0 || v.k2 != 0.0) {
+            for (int k2 = 0; k2 < 8; ++k2) {
+                const double r2n = npx * npx + npy * npy;
+                double dd = 1.0 + v.k1 * r2n;
+                dd = dd + (v.k2 * r2n) * r2n;
+                npx = nqx / dd;
+                npy = nqy / dd;
+            }
         }
         const double xn0 = npx * dn, xn1 = npy * dn, xn2 = dn;
         double er[3];

@@ -81,16 +81,20 @@ PEAK_HBM_GBS = 8000.0
 # profiles/k_pml_sweep_fast_8x2000x1500_k8_7x7_counters.md (fast arithmetic,
 # one lane per pixel) and profiles/k_pm_sweep_exact_8x2000x1500_k8_7x7_counters.md
 # hold the per-launch figures.
-BYTES_PER_EVAL = {"exact": (442.3, 6.55), "fast": (154.9, 5.94)}
+BYTES_PER_EVAL = {"exact": (442.3, 6.55), "fast": (173.6, 7.61)}
 
 
 # The roofline object's "bound" names the compute roofline ("mfma": the FP32
 # peak of 157.3 TFLOP/s, MI355X's figure for vector and matrix FP32 alike);
-# the half-sweep kernel itself runs on the vector ALUs.
-LIMITER = "valu"
+# the half-sweep kernel itself computes on the vector ALUs, and what limits it
+# is the texture-address unit of its sample gathers.
+LIMITER = "texture-address unit (gathers)"
 BOUND_NOTE = ("compute roofline at MI355X's FP32 peak (157.3 TFLOP/s); the kernel issues no "
-              "matrix instructions: it runs on the vector ALUs (limiter 'valu'), and its "
-              "counters show the texture-address unit of the sample gathers busy beside them")
+              "matrix instructions: it computes on the vector ALUs (0.55-0.88 wave-instructions "
+              "per CU per clock), and its sample gathers keep the texture-address unit 87-97% "
+              "busy, a third to a half of it stalled on the L1 cache (the limiter); achieved "
+              "counts the evaluations performed, not the propagation candidates skipped as "
+              "scored before")
 
 
 def log(*a):
@@ -554,6 +558,7 @@ def run_large(args):
     roofline = dict(bound="mfma", achieved=achieved, peak=157.3, unit="TFLOP/s",
                     frac=achieved / 157.3, traffic=None, avg_launch_ms=avg_ms,
                     launches=st["sweep_launches"], flops_per_eval=fpe,
+                    skipped_per_launch=st["sweep_skipped"] / max(1, st["sweep_launches"]),
                     limiter=LIMITER, bound_note=BOUND_NOTE)
     cpu = None
     if not args.no_cpu:
@@ -752,6 +757,7 @@ def main():
                     hbm_gbs=(traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
                     kernel=kname,
                     avg_launch_ms=avg_ms, evals_per_launch=evals_per_launch, flops_per_eval=fpe,
+                    skipped_per_launch=st["sweep_skipped"] / max(1, st["sweep_launches"]),
                     launches=st["sweep_launches"],
                     samples_per_s=samples_per_eval * evals_per_launch / (avg_ms * 1e-3),
                     samples_note=("bilinear samples gathered per second by the half-sweeps "
