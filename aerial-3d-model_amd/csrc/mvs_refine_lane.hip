@@ -150,7 +150,7 @@ __device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rst
             // t1 = g00 + ax A, u = B + ax C
             float cg, cA, cB, cC;
             if constexpr (F32) {
-                const float4 e = Qd[__umul24(y0, sw) + x0];
+                const f32x4 e = as_global(Qd)[__umul24(y0, sw) + x0];
                 cg = e.x;
                 cA = e.y;
                 cB = e.z;
@@ -318,7 +318,7 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
                 // the sample t1 + ay u, t1 = g00 + ax A, u = B + ax C
                 float t1, u;
                 if constexpr (F32) {
-                    const float4 e = Qd[__umul24(y0, sw) + x0];
+                    const f32x4 e = as_global(Qd)[__umul24(y0, sw) + x0];
                     t1 = fmaf(ax, e.y, e.x);
                     u = fmaf(ax, e.w, e.z);
                 } else {
@@ -489,12 +489,26 @@ __device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr
     return true;
 }
 
+// Waves per SIMD the refinement kernel is built for: two, which leaves the
+// registers to issue a tap row's gathers together in the Gauss-Newton steps
+// (at three waves the compiler issues them one at a time there: 77-view
+// refinement 302 -> 281 ms at 7x7, 842 -> 585 ms at 11x11); the 11x11 window
+// on f32 views, which needs more registers than two waves have, keeps the
+// compiler's own choice.
+template <int TAPS, class WT>
+constexpr bool kRefineTwoWaves = TAPS <= 7 || !std::is_same<WT, float>::value;
+template <int TAPS, class WT>
+constexpr int kRefineWavesMin = kRefineTwoWaves<TAPS, WT> ? 2 : 1;
+template <int TAPS, class WT>
+constexpr int kRefineWavesMax = kRefineTwoWaves<TAPS, WT> ? 2 : 8;
+
 // One lane per pixel of the job's textured-pixel lists (64 per wavefront);
 // pixels whose PatchMatch cost is below cost_max are refined in place.
 // counters (nullable): [0] += accepted steps, [1] += pixels refined.
 template <int TAPS, class WT>
-__global__ __launch_bounds__(64) void k_refine_lane(RefJob job, int32_t steps,
-                                                    unsigned long long *counters) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(kRefineWavesMin<TAPS, WT>, kRefineWavesMax<TAPS, WT>)))
+void k_refine_lane(RefJob job, int32_t steps, unsigned long long *counters) {
     constexpr int N = TAPS * TAPS;
     __shared__ WT Ah[N][64];
     __shared__ SrcCam Ss[kMaxSrc];
