@@ -13,8 +13,10 @@
 // coherent like the PatchMatch sweep's; the pairs of neighbouring pixels
 // mostly agree, which keeps the Gauss-Newton gathers coherent too. The
 // window's rays are affine in the tap offsets (undistorted reference), so the
-// plane depth dq = c / (n . r), the source-frame ray w = R r and the
-// derivative factors are affine per tap rather than loaded. Samples come from
+// scaled source points are affine per tap (a homography), and so are the
+// factors that carry a sample's derivative along the plane vector to depth
+// and the normal angles: the taps sum moments, and the derivative sums follow
+// once per window (lane_pass). Samples come from
 // the f16 bilinear-coefficient plane (g00 + ax A + ay (B + ax C) of the 8-bit
 // image, whose A + ay C and B + ax C are the image gradient); the reference
 // window lives in LDS as
@@ -56,195 +58,48 @@ struct LaneRef {
     bool ok = false;
 };
 
-// NCC terms of source S for the plane (d, n) at this lane's pixel; NDIR = 3
-// adds the derivatives along depth and the two normal directions dirs.
-template <int TAPS, int NDIR, class WT>
-__device__ LTerms<NDIR> lane_terms(const SrcCam &S, const LaneRef &pr, float rstep, float d,
-                                   const float n[3], const float (*dirs)[3],
-                                   const WT (*Ah)[64], int lane) {
+// One window pass of source S for the plane (d, n) at this lane's pixel: the
+// NCC, and with DERIV its derivatives along depth and the two normal angles
+// (tangent directions dirs). Lanes with act false (or whose window fails the
+// definedness tests) return ok = false.
+//
+// The window must lie on the plane's visible side (|n . r| >= 1e-6, dq > 0)
+// and project into the source (X_s.z > 0, inside the image) at every tap;
+// n . r and the scaled source point are affine over the window, so checking
+// the four corner taps suffices. The source points come from the
+// centre-scaled homography: scaled by the centre tap's depth dq_c, the source
+// point of tap (dx, dy) is affine in (dx, dy),
+//     X_c + dx (R e_x dq_c + t (n . e_x) / (n . r_c)) + dy (...),
+// so a tap costs three multiply-adds and one reciprocal. The taps clamp their
+// positions only when some lane's window has a corner within 1/256 pixel of
+// the sampling range's edge (rounding moves an inner tap by a few 1e-4 pixels
+// at most).
+//
+// Derivatives. With the plane written m . X = 1 (m = n / (d n . r_c)), a tap
+// samples the source at the projection of R r + t (m . r), so
+//     ds/dm = d b r,  b = (Iu (Kx - u tz) + Iv (Ky - v tz)) / P_z,
+// with (u, v) the source pixel position, (Iu, Iv) the image gradient there,
+// Kx = F tx + (cx - 1/2) tz, Ky likewise and P_z the homography's third
+// coordinate. The chain to theta = (depth, angle 1, angle 2) multiplies b by
+// a factor affine in the tap offsets,
+//     ds/dtheta_j = b g_j(dx, dy),  g_0 = -(n . r) / c,
+//     g_q = (dx (dir_q.x rstep - (n.x rstep)(dir_q . r_c) / nc)
+//          + dy (dir_q.y rstep - (n.y rstep)(dir_q . r_c) / nc)) / nc
+// (c = d nc, nc = n . r_c; the angles move the plane about the centre
+// point, so g_q vanishes there). The taps therefore only sum b, b^2, s b
+// and a b against 1, dx, dy (and b^2 against their products); the
+// derivative sums follow once per window from these moments.
+template <int TAPS, class WT, bool DERIV>
+__device__ LTerms<DERIV ? 3 : 0> lane_pass(const SrcCam &S, const LaneRef &pr, float rstep,
+                                           float d, const float n[3], const float (*dirs)[3],
+                                           const WT (*Ah)[64], int lane, bool act) {
     constexpr int N = TAPS * TAPS, H = TAPS / 2;
-    constexpr int ND = NDIR > 0 ? NDIR : 1;
-    constexpr int NP = NDIR > 0 ? pk(NDIR) : 1;
-    LTerms<NDIR> T;
+    constexpr bool F32 = std::is_same<WT, float>::value;
+    LTerms<DERIV ? 3 : 0> T;
     const float nc = fmaf(n[0], pr.rcx, fmaf(n[1], pr.rcy, n[2]));  // n . r_c
     const float cc = d * nc;
     const float ex = n[0] * rstep, ey = n[1] * rstep;  // n . r per tap column / row
     float wc[3], wx[3], wy[3];  // w = R r at the centre tap, per column, per row
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        wc[i] = fmaf(S.R[3 * i], pr.rcx, fmaf(S.R[3 * i + 1], pr.rcy, S.R[3 * i + 2]));
-        wx[i] = S.R[3 * i] * rstep;
-        wy[i] = S.R[3 * i + 1] * rstep;
-    }
-    const float F = S.f, CXh = S.cx - 0.5f, CYh = S.cy - 0.5f;
-    const float wm1 = (float)(S.w - 1), hm1 = (float)(S.h - 1);
-    // The window must lie on the plane's visible side (|n . r| >= 1e-6,
-    // dq > 0) and project into the source (X_s.z > 0, inside the image) at
-    // every tap; n . r and the scaled source point are affine over the window,
-    // so checking the four corner taps suffices.
-#pragma unroll
-    for (int cr = 0; cr < 4; ++cr) {
-        const float fdx = (cr & 1) ? (float)H : -(float)H;
-        const float fdy = (cr & 2) ? (float)H : -(float)H;
-        const float den = fmaf(fdy, ey, fmaf(fdx, ex, nc));
-        if (!(fabsf(den) >= 1e-6f && cc * den > 0.0f)) return T;
-        const float dq = cc * __builtin_amdgcn_rcpf(den);
-        float X[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) X[i] = fmaf(dq, fmaf(fdy, wy[i], fmaf(fdx, wx[i], wc[i])), S.t[i]);
-        if (!(X[2] > 0.0f)) return T;
-        const float fi = F * __builtin_amdgcn_rcpf(X[2]);
-        const float u = fmaf(fi, X[0], CXh), v = fmaf(fi, X[1], CYh);
-        if (!(u >= 0.0f && v >= 0.0f && u < wm1 && v < hm1)) return T;
-    }
-    // r . dir per normal direction (affine) and d (r_c . dir)
-    float rdc[ND], rdx[ND], rdy[ND], drc[ND];
-#pragma unroll
-    for (int q = 1; q < ND; ++q) {
-        const float *dv = dirs[q - 1];
-        rdc[q] = fmaf(dv[0], pr.rcx, fmaf(dv[1], pr.rcy, dv[2]));
-        rdx[q] = dv[0] * rstep;
-        rdy[q] = dv[1] * rstep;
-        drc[q] = d * rdc[q];
-    }
-    constexpr bool F32 = std::is_same<WT, float>::value;
-    const uint32_t sw = (uint32_t)S.w;
-    const u32x2 *Qp = (const u32x2 *)S.quadh;
-    const float4 *Qd = S.quadd;
-    float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
-    float sD[ND], sbD[ND], saD[ND], sDD[NP];
-#pragma unroll
-    for (int q = 0; q < ND; ++q) sD[q] = sbD[q] = saD[q] = 0.0f;
-#pragma unroll
-    for (int q = 0; q < NP; ++q) sDD[q] = 0.0f;
-#pragma unroll 1
-    for (int ty = 0; ty < TAPS; ++ty) {
-        const float fdy = (float)(ty - H);
-        const float denY = fmaf(fdy, ey, nc);
-        float wY[3], rdY[ND];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) wY[i] = fmaf(fdy, wy[i], wc[i]);
-#pragma unroll
-        for (int q = 1; q < ND; ++q) rdY[q] = fmaf(fdy, rdy[q], rdc[q]);
-#pragma unroll
-        for (int tx = 0; tx < TAPS; ++tx) {
-            const float fdx = (float)(tx - H);
-            const float den = fmaf(fdx, ex, denY);
-            const float rden = __builtin_amdgcn_rcpf(den);
-            const float dq = cc * rden;
-            float w[3], X[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                w[i] = fmaf(fdx, wx[i], wY[i]);
-                X[i] = fmaf(dq, w[i], S.t[i]);
-            }
-            const float inv = __builtin_amdgcn_rcpf(X[2]);
-            const float q0 = X[0] * inv, q1 = X[1] * inv;
-            // inside up to rounding; the clamp keeps every address in bounds
-            const float fx = __builtin_amdgcn_fmed3f(fmaf(F, q0, CXh), 0.0f, wm1);
-            const float fy = __builtin_amdgcn_fmed3f(fmaf(F, q1, CYh), 0.0f, hm1);
-            const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
-            const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
-            // the cell's bilinear coefficients g00, A, B, C (f16 bytes of 8-bit
-            // views, or f32 grey of f32 views): the sample is t1 + ay u with
-            // t1 = g00 + ax A, u = B + ax C
-            float cg, cA, cB, cC;
-            if constexpr (F32) {
-                const f32x4 e = as_global(Qd)[__umul24(y0, sw) + x0];
-                cg = e.x;
-                cA = e.y;
-                cB = e.z;
-                cC = e.w;
-            } else {
-                const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
-                const uint32_t qa = qv.x, qb = qv.y;
-                const h2 r0 = __builtin_bit_cast(h2, qa), r1 = __builtin_bit_cast(h2, qb);
-                cg = (float)r0.x;
-                cA = (float)r0.y;
-                cB = (float)r1.x;
-                cC = (float)r1.y;
-            }
-            const float t1 = fmaf(ax, cA, cg);
-            const float u = fmaf(ax, cC, cB);
-            const float val = fmaf(ay, u, t1);
-            if (tx == 0 && ty == 0) c0 = val;
-            const float sv = val - c0;  // shifted sample
-            const float a = (float)Ah[ty * TAPS + tx][lane];
-            s1 += sv;
-            s2 = fmaf(sv, sv, s2);
-            sa = fmaf(a, sv, sa);
-            if constexpr (NDIR > 0) {
-                // image gradient at the sample (per source pixel)
-                const float Iv = u;
-                const float Iu = fmaf(ay, cC, cA);
-                // d(u, v)/d dq and the chain to depth and the normal angles
-                const float fi = F * inv;
-                const float du = fmaf(-q0, w[2], w[0]) * fi;
-                const float dv = fmaf(-q1, w[2], w[1]) * fi;
-                const float t0 = fmaf(Iu, du, Iv * dv) * rden;
-                float Dk[ND];
-                Dk[0] = t0 * nc;
-#pragma unroll
-                for (int q = 1; q < ND; ++q) {
-                    const float rq = fmaf(fdx, rdx[q], rdY[q]);
-                    Dk[q] = t0 * fmaf(-dq, rq, drc[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < ND; ++q) {
-                    sD[q] += Dk[q];
-                    sbD[q] = fmaf(sv, Dk[q], sbD[q]);
-                    saD[q] = fmaf(a, Dk[q], saD[q]);
-                }
-                int m = 0;
-#pragma unroll
-                for (int p = 0; p < ND; ++p)
-#pragma unroll
-                    for (int q = p; q < ND; ++q, ++m) sDD[m] = fmaf(Dk[p], Dk[q], sDD[m]);
-            }
-        }
-    }
-    const float ms = s1 * (1.0f / (float)N);
-    const float nb2 = fmaf(-ms, s1, s2);
-    if (!(nb2 > 1e-12f)) return T;
-    T.nb = sqrtf(nb2);
-    T.ncc = fmaf(-pr.delta, s1, sa) / (pr.na * T.nb);
-    if constexpr (NDIR > 0) {
-#pragma unroll
-        for (int q = 0; q < ND; ++q) {
-            T.Dta[q] = fmaf(-pr.delta, sD[q], saD[q]);
-            T.Dtb[q] = fmaf(-ms, sD[q], sbD[q]);
-        }
-        int m = 0;
-#pragma unroll
-        for (int p = 0; p < ND; ++p)
-#pragma unroll
-            for (int q = p; q < ND; ++q, ++m)
-                T.DD[m] = sDD[m] - sD[p] * sD[q] * (1.0f / (float)N) - T.Dtb[p] * T.Dtb[q] / nb2;
-    }
-    T.ok = true;
-    return T;
-}
-
-// NCC of source S for the plane (d, n) at this lane's pixel (lane_terms with
-// NDIR = 0, the same definedness tests and sums), with the window's source
-// points from the centre-scaled homography: scaled by the centre tap's depth
-// dq_c, the source point of tap (dx, dy) is affine in (dx, dy),
-//     X_c + dx (R e_x dq_c + t (n . e_x) / (n . r_c)) + dy (...),
-// so a tap costs three multiply-adds and one reciprocal rather than the
-// plane depth's reciprocal, its scaling and the projection's reciprocal.
-// The taps clamp their positions only when some lane's window has a corner
-// within 1/256 pixel of the sampling range's edge (rounding moves an inner
-// tap by a few 1e-4 pixels at most).
-template <int TAPS, class WT>
-__device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, float d,
-                               const float n[3], const WT (*Ah)[64], int lane, bool act) {
-    constexpr int N = TAPS * TAPS, H = TAPS / 2;
-    constexpr bool F32 = std::is_same<WT, float>::value;
-    LTerms<0> T;
-    const float nc = fmaf(n[0], pr.rcx, fmaf(n[1], pr.rcy, n[2]));  // n . r_c
-    const float cc = d * nc;
-    const float ex = n[0] * rstep, ey = n[1] * rstep;
-    float wc[3], wx[3], wy[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         wc[i] = fmaf(S.R[3 * i], pr.rcx, fmaf(S.R[3 * i + 1], pr.rcy, S.R[3 * i + 2]));
@@ -289,10 +144,16 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
         B1[i] = sc * c1;
         B2[i] = sc * c2;
     }
+    const float tz = S.t[2];
+    const float Kx = fmaf(F, S.t[0], CXh * tz), Ky = fmaf(F, S.t[1], CYh * tz);
     const uint32_t sw = (uint32_t)S.w;
     const u32x2 *Qp = (const u32x2 *)S.quadh;
     const float4 *Qd = S.quadd;
     float c0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
+    // moments of b against (1, dx, dy) (m1), of s b (mb) and a b (ma), and of
+    // b^2 against 1, dx, dy, dx^2, dx dy, dy^2 (m2)
+    float m1[3] = {0.0f, 0.0f, 0.0f}, mb[3] = {0.0f, 0.0f, 0.0f}, ma[3] = {0.0f, 0.0f, 0.0f};
+    float m2[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     auto taps = [&](auto clamp_tag) {
         constexpr bool CLAMP = decltype(clamp_tag)::value;
 #pragma unroll 1
@@ -301,6 +162,9 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
             const float Y0 = fmaf(fdy, B2[0], B0[0]);
             const float Y1 = fmaf(fdy, B2[1], B0[1]);
             const float Y2 = fmaf(fdy, B2[2], B0[2]);
+            // this row's moments against 1 and dx (and dx^2 for b^2)
+            float rb0 = 0.0f, rbx = 0.0f, rq0 = 0.0f, rqx = 0.0f, rqxx = 0.0f;
+            float rs0 = 0.0f, rsx = 0.0f, ra0 = 0.0f, rax = 0.0f;
 #pragma unroll
             for (int tx = 0; tx < TAPS; ++tx) {
                 const float fdx = (float)(tx - H);
@@ -315,25 +179,63 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
                 }
                 const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
                 const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
-                // the sample t1 + ay u, t1 = g00 + ax A, u = B + ax C
-                float t1, u;
+                // the cell's bilinear coefficients g00, A, B, C (f16 bytes of
+                // 8-bit views, or f32 grey of f32 views): the sample is
+                // t1 + ay u with t1 = g00 + ax A, u = B + ax C; the gradient
+                // is (A + ay C, u)
+                float t1, u, Iu;
                 if constexpr (F32) {
                     const f32x4 e = as_global(Qd)[__umul24(y0, sw) + x0];
                     t1 = fmaf(ax, e.y, e.x);
                     u = fmaf(ax, e.w, e.z);
+                    if constexpr (DERIV) Iu = fmaf(ay, e.w, e.y);
                 } else {
                     const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
                     const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
                     const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
                     t1 = fmaf(ax, (float)r0.y, (float)r0.x);
                     u = fmaf(ax, (float)r1.y, (float)r1.x);
+                    if constexpr (DERIV) Iu = fmaf(ay, (float)r1.y, (float)r0.y);
                 }
                 const float val = fmaf(ay, u, t1);
                 if (tx == 0 && ty == 0) c0 = val;
-                const float sv = val - c0;
+                const float sv = val - c0;  // shifted sample
+                const float a = (float)Ah[ty * TAPS + tx][lane];
                 s1 += sv;
                 s2 = fmaf(sv, sv, s2);
-                sa = fmaf((float)Ah[ty * TAPS + tx][lane], sv, sa);
+                sa = fmaf(a, sv, sa);
+                if constexpr (DERIV) {
+                    const float b = fmaf(Iu, fmaf(-fx, tz, Kx), u * fmaf(-fy, tz, Ky)) * ip;
+                    const float bb = b * b, sb = sv * b, ab = a * b;
+                    rb0 += b;
+                    rq0 += bb;
+                    rs0 += sb;
+                    ra0 += ab;
+                    if (tx != H) {
+                        rbx = fmaf(b, fdx, rbx);
+                        rqx = fmaf(bb, fdx, rqx);
+                        rqxx = fmaf(bb, fdx * fdx, rqxx);
+                        rsx = fmaf(sb, fdx, rsx);
+                        rax = fmaf(ab, fdx, rax);
+                    }
+                }
+            }
+            if constexpr (DERIV) {
+                m1[0] += rb0;
+                m1[1] += rbx;
+                m1[2] = fmaf(fdy, rb0, m1[2]);
+                mb[0] += rs0;
+                mb[1] += rsx;
+                mb[2] = fmaf(fdy, rs0, mb[2]);
+                ma[0] += ra0;
+                ma[1] += rax;
+                ma[2] = fmaf(fdy, ra0, ma[2]);
+                m2[0] += rq0;                         // 1
+                m2[1] += rqx;                         // dx
+                m2[2] = fmaf(fdy, rq0, m2[2]);        // dy
+                m2[3] += rqxx;                        // dx^2
+                m2[4] = fmaf(fdy, rqx, m2[4]);        // dx dy
+                m2[5] = fmaf(fdy * fdy, rq0, m2[5]);  // dy^2
             }
         }
     };
@@ -348,8 +250,64 @@ __device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, 
     if (!(act && nb2 > 1e-12f)) return T;
     T.nb = sqrtf(nb2);
     T.ncc = fmaf(-pr.delta, s1, sa) / (pr.na * T.nb);
+    if constexpr (DERIV) {
+        // g_j = G[j][0] + dx G[j][1] + dy G[j][2]
+        const float rcc = rnc / d;
+        float G[3][3];
+        G[0][0] = -1.0f / d;
+        G[0][1] = -ex * rcc;
+        G[0][2] = -ey * rcc;
+#pragma unroll
+        for (int q = 1; q < 3; ++q) {
+            const float *dv = dirs[q - 1];
+            const float k = fmaf(dv[0], pr.rcx, fmaf(dv[1], pr.rcy, dv[2])) * rnc;  // (dir . r_c) / nc
+            G[q][0] = 0.0f;
+            G[q][1] = fmaf(-ex, k, dv[0] * rstep) * rnc;
+            G[q][2] = fmaf(-ey, k, dv[1] * rstep) * rnc;
+        }
+        const float M2[3][3] = {{m2[0], m2[1], m2[2]}, {m2[1], m2[3], m2[4]}, {m2[2], m2[4], m2[5]}};
+        float sD[3], sbD[3], saD[3], GM[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            sD[j] = fmaf(G[j][0], m1[0], fmaf(G[j][1], m1[1], G[j][2] * m1[2]));
+            sbD[j] = fmaf(G[j][0], mb[0], fmaf(G[j][1], mb[1], G[j][2] * mb[2]));
+            saD[j] = fmaf(G[j][0], ma[0], fmaf(G[j][1], ma[1], G[j][2] * ma[2]));
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                GM[j][c] = fmaf(G[j][0], M2[0][c], fmaf(G[j][1], M2[1][c], G[j][2] * M2[2][c]));
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            T.Dta[q] = fmaf(-pr.delta, sD[q], saD[q]);
+            T.Dtb[q] = fmaf(-ms, sD[q], sbD[q]);
+        }
+        int m = 0;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = p; q < 3; ++q, ++m) {
+                const float sdd = fmaf(GM[p][0], G[q][0], fmaf(GM[p][1], G[q][1], GM[p][2] * G[q][2]));
+                T.DD[m] = sdd - sD[p] * sD[q] * (1.0f / (float)N) - T.Dtb[p] * T.Dtb[q] / nb2;
+            }
+    }
     T.ok = true;
     return T;
+}
+
+// The NCC terms of source S at the plane (d, n) with their derivatives along
+// depth and the two normal angles dirs.
+template <int TAPS, class WT>
+__device__ LTerms<3> lane_terms(const SrcCam &S, const LaneRef &pr, float rstep, float d,
+                                const float n[3], const float (*dirs)[3], const WT (*Ah)[64],
+                                int lane) {
+    return lane_pass<TAPS, WT, true>(S, pr, rstep, d, n, dirs, Ah, lane, true);
+}
+
+// The NCC of source S at the plane (d, n) alone.
+template <int TAPS, class WT>
+__device__ LTerms<0> lane_cost(const SrcCam &S, const LaneRef &pr, float rstep, float d,
+                               const float n[3], const WT (*Ah)[64], int lane, bool act) {
+    return lane_pass<TAPS, WT, false>(S, pr, rstep, d, n, nullptr, Ah, lane, act);
 }
 
 // a ranks before b: higher NCC, then lower source index (dense.rs:255-256).
@@ -447,24 +405,19 @@ __device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, fl
                                float d, const float n[3], const WT (*Ah)[64], int lane,
                                bool search, int &i0, int &i1) {
     float b0 = -INFINITY, b1 = -INFINITY;
-    i0 = 1 << 20;
-    i1 = 1 << 20;
+    int j0 = 1 << 20, j1 = 1 << 20;
     for (int k = 0; k < nsrc; ++k) {
-        {
-            const LTerms<0> T = lane_cost<TAPS, WT>(Ss[k], pr, rstep, d, n, Ah, lane, search);
-            if (T.ok) {
-                if (lane_before(T.ncc, k, b0, i0)) {
-                    b1 = b0;
-                    i1 = i0;
-                    b0 = T.ncc;
-                    i0 = k;
-                } else if (lane_before(T.ncc, k, b1, i1)) {
-                    b1 = T.ncc;
-                    i1 = k;
-                }
-            }
-        }
+        const LTerms<0> T = lane_cost<TAPS, WT>(Ss[k], pr, rstep, d, n, Ah, lane, search);
+        // (selects rather than branches keep the pair in registers)
+        const bool first = T.ok && lane_before(T.ncc, k, b0, j0);
+        const bool second = T.ok && !first && lane_before(T.ncc, k, b1, j1);
+        b1 = first ? b0 : (second ? T.ncc : b1);
+        j1 = first ? j0 : (second ? k : j1);
+        b0 = first ? T.ncc : b0;
+        j0 = first ? k : j0;
     }
+    i0 = j0;
+    i1 = j1;
 }
 
 // Gauss-Newton terms of the pair (A, B) at the plane (d, n): J^T J (packed
@@ -476,8 +429,8 @@ __device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr
                              float d, const float n[3], const float dirs[2][3],
                              const WT (*Ah)[64], int lane, float Hm[6], float g[3],
                              float &cost) {
-    const LTerms<3> TA = lane_terms<TAPS, 3, WT>(A, pr, rstep, d, n, dirs, Ah, lane);
-    const LTerms<3> TB = lane_terms<TAPS, 3, WT>(B, pr, rstep, d, n, dirs, Ah, lane);
+    const LTerms<3> TA = lane_terms<TAPS, WT>(A, pr, rstep, d, n, dirs, Ah, lane);
+    const LTerms<3> TB = lane_terms<TAPS, WT>(B, pr, rstep, d, n, dirs, Ah, lane);
     if (!(TA.ok && TB.ok)) return false;
 #pragma unroll
     for (int q = 0; q < 6; ++q) Hm[q] = TA.DD[q] / (TA.nb * TA.nb) + TB.DD[q] / (TB.nb * TB.nb);
