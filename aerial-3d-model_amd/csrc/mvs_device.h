@@ -93,6 +93,8 @@ typedef const __attribute__((address_space(1))) float gfloat;
 __device__ __forceinline__ gfloat *as_global(const float *p) { return (gfloat *)p; }
 typedef const __attribute__((address_space(1))) uint32_t gu32;
 __device__ __forceinline__ gu32 *as_global(const uint32_t *p) { return (gu32 *)p; }
+typedef const __attribute__((address_space(1))) uint8_t gu8;
+__device__ __forceinline__ gu8 *as_global(const uint8_t *p) { return (gu8 *)p; }
 // A copy of *p read through the constant address space, for tables that no
 // lane writes while the kernel runs: at a wave-uniform address the reads
 // become scalar loads shared by the wave.

@@ -208,7 +208,65 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                 // the costs use a = b - mi (mi an integer near the byte mean)
                 // and the exact correction delta = mean(b) - mi.
                 float mr = 0.0f, sr = 0.0f, delta = 0.0f;
-                if (inside) {
+                // (In the sweep the window's loads issue together, the
+                // first pass's all at once and the second pass's a row at a
+                // time: 28 ms less per 77-view f32 run; the initial-cost
+                // pass, one evaluation per pixel, keeps plain row loops,
+                // which measured faster there.)
+                if (inside && MODE == MODE_SWEEP) {
+                    float s = 0.0f, sb = 0.0f, mi = 0.0f, var = 0.0f;
+                    auto tap_grey = [&](int ty, int tx) -> float {
+                        return as_global(job.gray)[pidx + (int64_t)((ty - H) * stride) * w +
+                                                   (tx - H) * stride];
+                    };
+                    auto second_pass = [&](int ty, int tx, float g) {
+                        const float d = g - mr;
+                        var = var + d * d;
+                        if constexpr (F32)
+                            Af[ty * TAPS + tx][lane] = d;
+                        else
+                            Ah[ty * TAPS + tx][lane] = (_Float16)(rintf(g * 255.0f) - mi);
+                    };
+                    if constexpr (N <= 49) {
+                        // the first pass's loads all issue together
+                        float gw[N];
+#pragma unroll
+                        for (int t = 0; t < N; ++t) gw[t] = tap_grey(t / TAPS, t % TAPS);
+#pragma unroll
+                        for (int t = 0; t < N; ++t) {
+                            s = s + gw[t];
+                            sb += rintf(gw[t] * 255.0f);
+                        }
+                    } else {
+#pragma unroll 1
+                        for (int ty = 0; ty < TAPS; ++ty) {
+                            float gr[TAPS];
+#pragma unroll
+                            for (int tx = 0; tx < TAPS; ++tx) gr[tx] = tap_grey(ty, tx);
+#pragma unroll
+                            for (int tx = 0; tx < TAPS; ++tx) {
+                                s = s + gr[tx];
+                                sb += rintf(gr[tx] * 255.0f);
+                            }
+                        }
+                    }
+                    mr = s / (float)N;
+                    mi = rintf(sb * (1.0f / (float)N));
+                    // the second pass reads the window again, a row's loads
+                    // together (holding the first pass's values instead
+                    // costs registers the sampling loop needs)
+#pragma unroll 1
+                    for (int ty = 0; ty < TAPS; ++ty) {
+                        float gr[TAPS];
+#pragma unroll
+                        for (int tx = 0; tx < TAPS; ++tx) gr[tx] = tap_grey(ty, tx);
+#pragma unroll
+                        for (int tx = 0; tx < TAPS; ++tx) second_pass(ty, tx, gr[tx]);
+                    }
+                    delta = (sb - (float)N * mi) * (1.0f / (float)N);
+                    var = var / (float)N;
+                    sr = sqrtf(var);
+                } else if (inside) {
                     float s = 0.0f, sb = 0.0f;
 #pragma unroll 1
                     for (int ty = 0; ty < TAPS; ++ty) {
@@ -679,11 +737,35 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // before and not taken, or taken (and is the best or was
                     // beaten since): its cost is what it was, and the best
                     // cost can only have fallen, so it cannot be taken now.
-                    // (The flags are read in a loop of their own: read among
-                    // the candidates' loads, the compiler serialises the
-                    // taps' gathers.)
+                    // (On f32 views the neighbours' flags and planes are
+                    // loaded before the candidate list, at in-bounds
+                    // positions — the pixel's own where the neighbour is
+                    // outside — so that their loads issue together; their
+                    // values are used only where inside: 14 ms less per
+                    // 77-view run there, 16 ms more on 8-bit views. Read
+                    // among the candidates' loads, the flags made the
+                    // compiler serialise the taps' gathers.)
+                    int64_t nidx[8];
+                    uint32_t inbm0 = 0;
                     uint32_t unchanged = 0;
-                    if (a.skip_unchanged && pixel_ok) {
+                    if constexpr (F32) {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            int nx, ny;
+                            cand_pos(k, x, y, nx, ny);
+                            const bool inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
+                            nidx[k] = inb ? (int64_t)ny * w + nx : pidx;
+                            inbm0 |= (inb ? 1u : 0u) << k;
+                        }
+                        if (a.skip_unchanged && pixel_ok) {
+                            uint8_t fl[8];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) fl[k] = as_global(job.changed)[nidx[k]];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (((inbm0 >> k) & 1u) && !fl[k]) unchanged |= 1u << k;
+                        }
+                    } else if (a.skip_unchanged && pixel_ok) {
 #pragma unroll 1
                         for (int k = 0; k < 8; ++k) {
                             int nx, ny;
@@ -697,13 +779,26 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     {
                         float4 cand[8];
                         uint32_t inbm = 0;
+                        if constexpr (F32) {
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                const f32x4 c = as_global(job.hyp)[nidx[k]];
+                                cand[k] = make_float4(c.x, c.y, c.z, c.w);
+                            }
+                        }
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            int nx, ny;
-                            cand_pos(k, x, y, nx, ny);
-                            const bool inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
-                            cand[k] = inb ? job.hyp[(int64_t)ny * w + nx]
-                                          : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                            bool inb;
+                            if constexpr (F32) {
+                                inb = (inbm0 >> k) & 1u;
+                                if (!inb) cand[k] = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                            } else {
+                                int nx, ny;
+                                cand_pos(k, x, y, nx, ny);
+                                inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
+                                cand[k] = inb ? job.hyp[(int64_t)ny * w + nx]
+                                              : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
+                            }
                             bool act = inb && !same_hyp(cand[k], best);
 #pragma unroll
                             for (int j = 0; j < k; ++j)
