@@ -623,12 +623,13 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_
 
 // Schedule of the initial-cost evaluation (one cost per pixel, scattered
 // random planes): the one-lane-per-pixel kernels where they apply, whose
-// gathers each read one source image, on the 8-bit corner plane (scattered
-// gathers favour the smaller plane; f32 views: the grey plane, a quarter of
-// the corner-difference plane's bytes — 92 against 126 ms on 77 views
-// 2000x1500), else the first source-lane candidate.
+// gathers each read one source image. 8-bit views: the 8-bit corner plane,
+// one 4-byte gather per sample (105 ms on 77 views 2000x1500, 143 ms on the
+// f16 plane); f32 views: the f32 coefficient plane, one 16-byte gather per
+// sample (140 ms; 214 ms on the grey plane's two 8-byte row gathers). Else the
+// first source-lane candidate.
 Schedule eval_schedule(const KClass &k) {
-    if (k.fast == 2) return Schedule{-4, 1, 0};
+    if (k.fast == 2) return Schedule{-3, 1, 0};
     if (k.fast && k.pinhole == 2) return Schedule{-2, 1, 0};
     return candidates(k)[0];
 }
