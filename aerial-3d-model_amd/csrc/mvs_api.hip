@@ -76,7 +76,7 @@ struct MapDev {
 
 // Work buffers of one job slot of a batch.
 struct JobWork {
-    DevBuf hyp, cost, textured, changed, list, counts, srcs;
+    DevBuf hyp, cost, textured, changed, pair, list, counts, srcs;
 };
 
 // One reference view to process.
@@ -151,7 +151,8 @@ struct mvs_ctx {
     DevBuf up_u8;                    // staging of u8 uploads
     std::vector<ViewDev> tmp_views;  // host-view calls
     MapDev tmp_map;
-    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost, save_changed;
+    DevBuf px, py, hyp_in, cost_out, nbr_cams, agree, save_hyp, save_cost, save_changed,
+        save_pair;
     DevBuf prep_box, prep_rgb, prep_grey;  // view preparation
     DevBuf dsm_xyz, dsm_h, dsm_a, dsm_b, dsm_c, dsm_d, dsm_snap, dsm_parent, dsm_la, dsm_lb,
         dsm_ctr, dsm_sel;  // surface model
@@ -570,17 +571,19 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_
     HIP_TRY(c, c->save_hyp.ensure(n * sizeof(float4)));
     HIP_TRY(c, c->save_cost.ensure(n * sizeof(float)));
     HIP_TRY(c, c->save_changed.ensure(n));
-    // the tuning jobs' states (hypotheses, costs, change flags), saved before
-    // and restored before every run
+    HIP_TRY(c, c->save_pair.ensure(n));
+    // the tuning jobs' states (hypotheses, costs, change flags, pairs), saved
+    // before and restored before every run
     auto copy_states = [&](bool save) -> int {
         for (int32_t j = 0; j < nt; ++j) {
             const size_t m = off[(size_t)j + 1] - off[(size_t)j];
-            void *sv[3] = {c->save_hyp.as<float4>() + off[(size_t)j],
+            void *sv[4] = {c->save_hyp.as<float4>() + off[(size_t)j],
                            c->save_cost.as<float>() + off[(size_t)j],
-                           c->save_changed.as<uint8_t>() + off[(size_t)j]};
-            void *jv[3] = {j0[j].hyp, j0[j].cost, j0[j].changed};
-            const size_t bytes[3] = {m * sizeof(float4), m * sizeof(float), m};
-            for (int q = 0; q < 3; ++q)
+                           c->save_changed.as<uint8_t>() + off[(size_t)j],
+                           c->save_pair.as<uint8_t>() + off[(size_t)j]};
+            void *jv[4] = {j0[j].hyp, j0[j].cost, j0[j].changed, j0[j].pair};
+            const size_t bytes[4] = {m * sizeof(float4), m * sizeof(float), m, m};
+            for (int q = 0; q < 4; ++q)
                 HIP_TRY(c, hipMemcpyAsync(save ? sv[q] : jv[q], save ? jv[q] : sv[q], bytes[q],
                                           hipMemcpyDeviceToDevice, c->st));
         }
@@ -794,6 +797,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         HIP_TRY(c, wk.cost.ensure((size_t)n * sizeof(float)));
         HIP_TRY(c, wk.textured.ensure((size_t)n));
         HIP_TRY(c, wk.changed.ensure((size_t)n));
+        HIP_TRY(c, wk.pair.ensure((size_t)n));
         HIP_TRY(c, wk.list.ensure((size_t)(2 * cap) * sizeof(int32_t)));
         HIP_TRY(c, wk.counts.ensure((4 + 2 * (size_t)list_waves(ref.w, ref.h)) * sizeof(int32_t)));
         HIP_TRY(c, wk.srcs.ensure(sizeof(SrcCam) * (size_t)K));
@@ -821,6 +825,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         job.cost = wk.cost.as<float>();
         job.textured = wk.textured.as<uint8_t>();
         job.changed = wk.changed.as<uint8_t>();
+        job.pair = wk.pair.as<uint8_t>();
         job.list = wk.list.as<int32_t>();
         job.counts = wk.counts.as<int32_t>();
         job.srcs = wk.srcs.as<SrcCam>();
@@ -1179,14 +1184,14 @@ void mvs_destroy(mvs_ctx *c) {
         delete kv.second;
     }
     for (auto &wk : c->work) {
-        DevBuf *b[] = {&wk.hyp, &wk.cost, &wk.textured, &wk.changed, &wk.list, &wk.counts,
-                       &wk.srcs};
+        DevBuf *b[] = {&wk.hyp, &wk.cost, &wk.textured, &wk.changed, &wk.pair, &wk.list,
+                       &wk.counts, &wk.srcs};
         for (DevBuf *x : b) x->release();
     }
     DevBuf *bufs[] = {&c->jobs_dev, &c->prefix, &c->counters, &c->evals, &c->ok_count,
                       &c->up_u8,    &c->px,     &c->py,       &c->hyp_in, &c->cost_out,
                       &c->nbr_cams, &c->agree,  &c->save_hyp, &c->save_cost, &c->save_changed,
-                      &c->pairs,    &c->grad_io, &c->gout,
+                      &c->save_pair, &c->pairs,    &c->grad_io, &c->gout,
                       &c->tmp_map.depth, &c->tmp_map.normal, &c->tmp_map.cost,
                       &c->prep_box, &c->prep_rgb, &c->prep_grey,
                       &c->dsm_xyz, &c->dsm_h, &c->dsm_a, &c->dsm_b, &c->dsm_c, &c->dsm_d,

@@ -47,6 +47,10 @@ struct RefJob {
     // w*h: 1 where the pixel's last half-sweep changed its hypothesis (0 from
     // the initialisation on; untextured pixels never change)
     uint8_t *changed;
+    // w*h: the two sources best at the pixel's plane (first | second << 4),
+    // as the one-lane-per-pixel PatchMatch kernels scored that plane; 0xff
+    // where the plane was set by any other kernel or the host (unknown)
+    uint8_t *pair;
     int32_t *list;        // 2 * list_cap textured pixel indices, by parity
     int32_t *counts;      // [2] list lengths
     const SrcCam *srcs;   // nsrc

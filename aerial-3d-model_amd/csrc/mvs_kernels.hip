@@ -208,6 +208,7 @@ __global__ __launch_bounds__(256) void k_init(RefJob job, int32_t taps, int32_t 
         }
         job.textured[i] = tex ? 1 : 0;
         job.changed[i] = 0;
+        job.pair[i] = 0xff;  // unknown until a pixel-lane kernel scores the plane
     }
     const int parity = (x + y) & 1;
     const uint64_t m0 = __ballot(valid && tex && parity == 0);

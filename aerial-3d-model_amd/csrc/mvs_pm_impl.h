@@ -457,7 +457,10 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                 } else if (MODE == MODE_EVAL_CURRENT) {
                     const float4 hp = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     const float cst = eval(have, hp.x, hp.y, hp.z, hp.w);
-                    if (have && r == 0) job.cost[pidx] = cst;
+                    if (have && r == 0) {
+                        job.cost[pidx] = cst;
+                        job.pair[pidx] = 0xff;  // (this kernel stores no pairs)
+                    }
                 } else {
                     const float4 orig = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     float4 best = orig;
@@ -506,8 +509,10 @@ __global__ __launch_bounds__(64) void k_pm(PmArgs a) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
                         // (float ==: a NaN component counts as a change)
-                        job.changed[pidx] = !(best.x == orig.x && best.y == orig.y &&
-                                              best.z == orig.z && best.w == orig.w);
+                        const bool ch = !(best.x == orig.x && best.y == orig.y &&
+                                          best.z == orig.z && best.w == orig.w);
+                        job.changed[pidx] = ch;
+                        if (ch) job.pair[pidx] = 0xff;  // (this kernel stores no pairs)
                     }
                 }
             }

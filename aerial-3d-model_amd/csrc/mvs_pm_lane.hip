@@ -310,7 +310,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                 // (rcx, rcy), N sd of its bytes nsrv, byte-mean correction dlt,
                 // reference window in column col of Ah). Every lane calls it;
                 // lanes without a hypothesis to score pass act = false (act
-                // includes the pixel's own test, pixel_ok).
+                // includes the pixel's own test, pixel_ok). It leaves the two
+                // best sources in epair (first | second << 4; 0xff unless
+                // act), which the refinement takes as the plane's pair.
+                uint32_t epair = 0xffu;
                 auto eval = [&](bool act, float rcx, float rcy, float nsrv, float dlt, int col,
                                 float d, float n0, float n1, float n2) -> float {
                     if (act) ++n_evals;
@@ -336,6 +339,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     const float sx = dqc * rstep;
                     const float tnx = (n0 * rstep) * rden, tny = (n1 * rstep) * rden;
                     float m1 = INFINITY, m2 = INFINITY;
+                    uint32_t jj = 0xffu;  // their sources, first | second << 4
                     // (PL_F16T: every lane walks the sources, so that the whole
                     // wavefront can stage each source's tile)
                     if (TILE || act) {
@@ -638,13 +642,19 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                 if (!(ss < (F32 ? 1e-3f : 1e-3f * 255.0f)))
                                     cs = 1.0f - cov * __builtin_amdgcn_rcpf(nsrv * ss);
                             }
-                            // best two (dense.rs:255-262)
+                            // best two (dense.rs:255-262), and their sources
+                            // (ties to the lower index, as the sort's)
+                            const bool b1 = cs < m1, b2 = !b1 && cs < m2;
+                            jj = b1   ? ((jj << 4) & 0xf0u) | (uint32_t)k
+                                 : b2 ? (jj & 0x0fu) | ((uint32_t)k << 4)
+                                      : jj;
                             const float lo = fminf(m1, cs);
                             m2 = fminf(fmaxf(m1, cs), m2);
                             m1 = lo;
                         }
                     }
                     float cst = K >= 2 ? (m1 + m2) * 0.5f : m1;
+                    epair = act && K >= 2 ? jj : 0xffu;
                     if (!act) cst = 2.0f;
                     return cst;
                 };
@@ -658,7 +668,10 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     const float4 hp = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     const float cst = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane,
                                            hp.x, hp.y, hp.z, hp.w);
-                    if (have) job.cost[pidx] = cst;
+                    if (have) {
+                        job.cost[pidx] = cst;
+                        job.pair[pidx] = (uint8_t)epair;
+                    }
                 } else if constexpr (PLANE == PL_U8) {
                     // the ten proposals of dense.rs:427-462 in the reference's
                     // order (propagation in CAND order, then two random
@@ -672,6 +685,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // a candidate is taken only at a strictly lower cost, so
                     // the hypothesis changed exactly when one was taken
                     bool taken = false;
+                    uint32_t bpair = 0xffu;
                     uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
                     const float span = job.imax - job.imin;
 #pragma unroll 1
@@ -707,12 +721,14 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             best = hp;
                             bcost = cst;
                             taken = true;
+                            bpair = epair;
                         }
                     }
                     if (have) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
                         job.changed[pidx] = taken;
+                        if (taken) job.pair[pidx] = (uint8_t)bpair;
                     }
                 } else {
                     // the ten proposals of dense.rs:427-462: the eight
@@ -721,6 +737,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     float4 best = have ? job.hyp[pidx] : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                     float bcost = have ? job.cost[pidx] : 2.0f;
                     bool taken = false;  // (at a strictly lower cost: a change)
+                    uint32_t bpair = 0xffu;
                     // A candidate equal to the current best is not scored
                     // (dense.rs:436-438); one equal to an earlier candidate
                     // cannot be accepted either (its cost is that candidate's,
@@ -877,9 +894,12 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                                               : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                         const float cst = eval(act, orx, ory, onsr, odl, o, hp.x,
                                                hp.y, hp.z, hp.w);
+                        // (cost, candidate) orders the keys; the pair rides
+                        // below them
                         if (act)
                             atomicMin(&pmin[o], ((unsigned long long)cost_key(cst) << 32) |
-                                                    (unsigned long long)k);
+                                                    ((unsigned long long)k << 8) |
+                                                    (unsigned long long)epair);
                     }
                     __syncthreads();
                     const unsigned long long pm = pmin[lane];
@@ -887,10 +907,11 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         const float cst = key_cost((uint32_t)(pm >> 32));
                         if (cst < bcost) {
                             int nx, ny;
-                            cand_pos((int)(pm & 7ull), x, y, nx, ny);
+                            cand_pos((int)((pm >> 8) & 7ull), x, y, nx, ny);
                             best = job.hyp[(int64_t)ny * w + nx];
                             bcost = cst;
                             taken = true;
+                            bpair = (uint32_t)(pm & 0xffull);
                         }
                     }
                     // the two random refinements from the best after
@@ -914,12 +935,14 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             best = hp;
                             bcost = cst;
                             taken = true;
+                            bpair = epair;
                         }
                     }
                     if (have) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
                         job.changed[pidx] = taken;
+                        if (taken) job.pair[pidx] = (uint8_t)bpair;
                     }
                 }
             }

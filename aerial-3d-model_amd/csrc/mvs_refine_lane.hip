@@ -406,6 +406,7 @@ __device__ void lane_best_pair(const SrcCam *Ss, int nsrc, const LaneRef &pr, fl
                                bool search, int &i0, int &i1) {
     float b0 = -INFINITY, b1 = -INFINITY;
     int j0 = 1 << 20, j1 = 1 << 20;
+    if (__ballot(search) == 0ull) nsrc = 0;  // no lane of the wavefront searches
     for (int k = 0; k < nsrc; ++k) {
         const LTerms<0> T = lane_cost<TAPS, WT>(Ss[k], pr, rstep, d, n, Ah, lane, search);
         // (selects rather than branches keep the pair in registers)
@@ -487,11 +488,20 @@ void k_refine_lane(RefJob job, int32_t steps, unsigned long long *counters) {
     if (have) pr = lane_ref<TAPS, WT>(job, pix, Ah, lane);
     const float rstep = job.ray_step;
     float nrm[3] = {hp.y, hp.z, hp.w};
-    // best pair at the PatchMatch plane: every source in step
+    // best pair at the PatchMatch plane: the one the sweep stored with the
+    // plane (its two lowest source costs), else every source in step
     int i0, i1;
-    const bool search = have && pr.ok;
-    lane_best_pair<TAPS, WT>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, search, i0, i1);
-    bool live = search && i1 < job.nsrc;
+    const bool want = have && pr.ok;
+    const uint32_t sp = want ? (uint32_t)job.pair[pix] : 0xffu;
+    const int q0 = (int)(sp & 15u), q1 = (int)(sp >> 4);
+    const bool stored = sp != 0xffu && q0 < job.nsrc && q1 < job.nsrc && q0 != q1;
+    lane_best_pair<TAPS, WT>(Ss, job.nsrc, pr, rstep, hp.x, nrm, Ah, lane, want && !stored, i0,
+                             i1);
+    if (stored) {
+        i0 = q0;
+        i1 = q1;
+    }
+    bool live = want && i1 < job.nsrc;
     const bool refined = live;
     const int p0 = live ? i0 : 0, p1 = live ? i1 : 0;
     const float lo = job.dmin * 0.7f, hi = job.dmax * 1.4f;
@@ -570,7 +580,10 @@ void k_refine_lane(RefJob job, int32_t steps, unsigned long long *counters) {
             lambda = fminf(lambda * 10.0f, 1e6f);
         }
     }
-    if (have && acc) job.hyp[pix] = hp;
+    if (have && acc) {
+        job.hyp[pix] = hp;
+        job.pair[pix] = 0xff;  // (the refined plane's pair is not known)
+    }
     if (counters) {
         unsigned long long e = acc, p = refined ? 1 : 0;
 #pragma unroll
