@@ -860,6 +860,25 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             if ((amask >> k) & 1u)
                                 plist[atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1)] =
                                     (uint16_t)(lane | (k << 6));
+                    } else if constexpr (F32) {
+                        // f32 views: the list pixel-major, each pixel's
+                        // candidates together (after the first iterations
+                        // nearly one plane, so neighbouring lanes gather the
+                        // same cells): 31 ms less per 77-view run than
+                        // candidate-major, which is 37 ms faster on the f16
+                        // plane
+                        const int cnt = __popc(amask);
+                        int incl = cnt;
+#pragma unroll
+                        for (int off = 1; off < 64; off <<= 1) {
+                            const int t = __shfl_up(incl, off);
+                            if (lane >= off) incl += t;
+                        }
+                        total = __shfl(incl, 63);
+                        int at = incl - cnt;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            if ((amask >> k) & 1u) plist[at++] = (uint16_t)(lane | (k << 6));
                     } else {
                         // the list, candidate-major (neighbouring pixels' k-th
                         // candidates together, as they come from neighbouring
