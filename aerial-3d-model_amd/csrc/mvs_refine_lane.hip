@@ -165,6 +165,12 @@ __device__ LTerms<DERIV ? 3 : 0> lane_pass(const SrcCam &S, const LaneRef &pr, f
             // this row's moments against 1 and dx (and dx^2 for b^2)
             float rb0 = 0.0f, rbx = 0.0f, rq0 = 0.0f, rqx = 0.0f, rqxx = 0.0f;
             float rs0 = 0.0f, rsx = 0.0f, ra0 = 0.0f, rax = 0.0f;
+            // one tap row per trip: the row's positions and gathers, then
+            // (a scheduling barrier keeps the compiler from sinking each
+            // gather to its use) its samples and sums
+            float fxs[TAPS], fys[TAPS], ips[TAPS];
+            typedef std::conditional_t<F32, f32x4, u32x2> QT;
+            QT qs[TAPS];
 #pragma unroll
             for (int tx = 0; tx < TAPS; ++tx) {
                 const float fdx = (float)(tx - H);
@@ -178,6 +184,19 @@ __device__ LTerms<DERIV ? 3 : 0> lane_pass(const SrcCam &S, const LaneRef &pr, f
                     fy = __builtin_amdgcn_fmed3f(fy, 0.0f, hm1);
                 }
                 const uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+                if constexpr (F32)
+                    qs[tx] = as_global(Qd)[__umul24(y0, sw) + x0];
+                else
+                    qs[tx] = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
+                fxs[tx] = fx;
+                fys[tx] = fy;
+                ips[tx] = ip;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int tx = 0; tx < TAPS; ++tx) {
+                const float fdx = (float)(tx - H);
+                const float fx = fxs[tx], fy = fys[tx], ip = ips[tx];
                 const float ax = __builtin_amdgcn_fractf(fx), ay = __builtin_amdgcn_fractf(fy);
                 // the cell's bilinear coefficients g00, A, B, C (f16 bytes of
                 // 8-bit views, or f32 grey of f32 views): the sample is
@@ -185,12 +204,12 @@ __device__ LTerms<DERIV ? 3 : 0> lane_pass(const SrcCam &S, const LaneRef &pr, f
                 // is (A + ay C, u)
                 float t1, u, Iu;
                 if constexpr (F32) {
-                    const f32x4 e = as_global(Qd)[__umul24(y0, sw) + x0];
+                    const f32x4 e = qs[tx];
                     t1 = fmaf(ax, e.y, e.x);
                     u = fmaf(ax, e.w, e.z);
                     if constexpr (DERIV) Iu = fmaf(ay, e.w, e.y);
                 } else {
-                    const u32x2 qv = as_global_u32x2(Qp)[__umul24(y0, sw) + x0];
+                    const u32x2 qv = qs[tx];
                     const h2 r0 = __builtin_bit_cast(h2, (uint32_t)qv.x);
                     const h2 r1 = __builtin_bit_cast(h2, (uint32_t)qv.y);
                     t1 = fmaf(ax, (float)r0.y, (float)r0.x);
@@ -443,18 +462,19 @@ __device__ bool lane_pair_gn(const SrcCam &A, const SrcCam &B, const LaneRef &pr
     return true;
 }
 
-// Waves per SIMD the refinement kernel is built for: two, which leaves the
-// registers to issue a tap row's gathers together in the Gauss-Newton steps
-// (at three waves the compiler issues them one at a time there: 77-view
-// refinement 302 -> 281 ms at 7x7, 842 -> 585 ms at 11x11); the 11x11 window
-// on f32 views, which needs more registers than two waves have, keeps the
-// compiler's own choice.
+// Waves per SIMD the refinement kernel is built for, with each tap row's
+// gathers issued together (lane_pass): three up to 7x7 windows (77-view 7x7
+// refinement 192 -> 164 ms against two), two for 11x11 on 8-bit views (at
+// three its registers spill: 365 -> 370 ms); the 11x11 window on f32 views,
+// which needs more registers than two waves have, keeps the compiler's own
+// choice. (Left to itself the compiler aimed at three waves and issued the
+// Gauss-Newton steps' gathers one at a time.)
 template <int TAPS, class WT>
-constexpr bool kRefineTwoWaves = TAPS <= 7 || !std::is_same<WT, float>::value;
+constexpr int kRefineWaves = TAPS <= 7 ? 3 : (std::is_same<WT, float>::value ? 0 : 2);
 template <int TAPS, class WT>
-constexpr int kRefineWavesMin = kRefineTwoWaves<TAPS, WT> ? 2 : 1;
+constexpr int kRefineWavesMin = kRefineWaves<TAPS, WT> > 0 ? kRefineWaves<TAPS, WT> : 1;
 template <int TAPS, class WT>
-constexpr int kRefineWavesMax = kRefineTwoWaves<TAPS, WT> ? 2 : 8;
+constexpr int kRefineWavesMax = kRefineWaves<TAPS, WT> > 0 ? kRefineWaves<TAPS, WT> : 8;
 
 // One lane per pixel of the job's textured-pixel lists (64 per wavefront);
 // pixels whose PatchMatch cost is below cost_max are refined in place.
