@@ -452,6 +452,9 @@ struct KClass {
     int64_t cap_sweep = 0, cap_eval = 0;  // upper bounds of the list lengths
     std::vector<int64_t> caps;            // pixels of each job
     int32_t size = 0;                     // log2 pixels of the first job
+    // the first half-sweep of each colour scores the initial planes itself
+    // (one-lane-per-pixel kernels in iteration 0) instead of a separate pass
+    bool own_init = false;
     Schedule sch;
 };
 
@@ -512,8 +515,9 @@ int pm_launch(mvs_ctx *c, int mode, int32_t taps, const KClass &k, const PmArgs 
 // Candidate schedules of a class: tap split x chunk size, and for the fast
 // arithmetic on undistorted cameras the one-lane-per-pixel kernels (split -1
 // on the f16 corner plane, -2 on the 8-bit one). A class without source lanes
-// (srcl 0: jobs of mixed neighbour counts) has only the latter.
-std::vector<Schedule> candidates(const KClass &k) {
+// (srcl 0: jobs of mixed neighbour counts) has only the latter, and so has a
+// half-sweep that scores the initial planes itself (pixel_lanes).
+std::vector<Schedule> candidates(const KClass &k, bool pixel_lanes = false) {
     if (k.fast == 2) {
         // f32 views: only the one-lane-per-pixel kernels, on the f32
         // corner-difference plane or the grey plane (identical values)
@@ -521,7 +525,7 @@ std::vector<Schedule> candidates(const KClass &k) {
                 Schedule{-4, 1, 8}};
     }
     std::vector<Schedule> cand;
-    if (k.srcl > 0) {
+    if (k.srcl > 0 && !pixel_lanes) {
         int32_t sp[4];
         const int ns = splits_for(k.taps, sp);
         for (int i = 0; i < ns; ++i) {
@@ -557,8 +561,8 @@ constexpr int32_t kTuneJobs = 4, kTuneRuns = 2;
 // iteration.
 
 int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_t it,
-                   int32_t parity, bool skip_unchanged) {
-    const std::vector<Schedule> cand = candidates(k);
+                   int32_t parity, bool skip_unchanged, bool eval_own) {
+    const std::vector<Schedule> cand = candidates(k, eval_own);
     if (cand.empty()) return fail(c, MVS_ERR_ARG, "no schedule for this class");
     if (cand.size() == 1) {
         k.sch = cand[0];
@@ -603,6 +607,7 @@ int autotune_class(mvs_ctx *c, int32_t taps, KClass &k, const RefJob *j0, int32_
         a.parity = parity;
         a.range = ldexpf(1.0f, -it);
         a.skip_unchanged = skip_unchanged ? 1 : 0;
+        a.eval_own = eval_own ? 1 : 0;
         for (int32_t r = 0; r < kTuneRuns; ++r) {
             TRY(copy_states(false));
             HIP_TRY(c, hipEventRecord(ea, c->st));
@@ -662,8 +667,9 @@ struct SweepLaunch {
 };
 
 int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32_t it,
-                   int32_t parity, bool skip_unchanged, unsigned long long *evals,
-                   unsigned long long *skipped, std::vector<SweepLaunch> &out) {
+                   int32_t parity, bool skip_unchanged, bool first_pair,
+                   unsigned long long *evals, unsigned long long *skipped,
+                   std::vector<SweepLaunch> &out) {
     out.clear();
     for (const KClass *kp : classes) {
         const KClass &k = *kp;
@@ -676,6 +682,7 @@ int sweep_launches(mvs_ctx *c, const std::vector<const KClass *> &classes, int32
         L.a.evals = evals;
         L.a.skip_unchanged = skip_unchanged ? 1 : 0;
         L.a.skipped = skipped;
+        L.a.eval_own = k.own_init && first_pair ? 1 : 0;
         L.persist = k.sch.per_cu > 0;
         L.grid = grid_for(c, k, L.persist != 0,
                           [&](int32_t j) { return (k.caps[(size_t)j] + 1) / 2; });
@@ -715,6 +722,7 @@ int run_sweeps_graph(mvs_ctx *c, const std::vector<SweepLaunch> &ls) {
         put(&L.a.evals, sizeof L.a.evals);
         put(&L.a.skip_unchanged, sizeof L.a.skip_unchanged);
         put(&L.a.skipped, sizeof L.a.skipped);
+        put(&L.a.eval_own, sizeof L.a.eval_own);
         const int32_t sel[7] = {L.grid, L.srcl, L.split, L.pinhole, L.persist, L.taps, L.fast};
         put(sel, sizeof sel);
     }
@@ -916,6 +924,13 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
                               sizeof(RefJob) * table.size(), hipMemcpyHostToDevice, c->st));
 
     unsigned long long *evals = c->evals.as<unsigned long long>();
+    // Classes on the one-lane-per-pixel kernels that make at least a
+    // half-sweep of each colour score the initial planes in those half-sweeps
+    // (each pixel's own plane first, ahead of its candidates: the same costs,
+    // and the pixel's window set up once); the others in a pass of their own.
+    for (KClass &k : classes)
+        k.own_init = first_step < 0 && k.fast && k.pinhole == 2 && k.steps >= 2 &&
+                     c->forced.split <= 0;
     // initialisation: hypotheses, texture mask, parity lists, then the cost of
     // every textured pixel's initial hypothesis
     {
@@ -931,6 +946,7 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         }
         if (first_step < 0) {
             for (KClass &k : classes) {
+                if (k.own_init) continue;
                 k.sch = eval_schedule(k);
                 PmArgs a = base_args(c, k);
                 a.evals = evals;
@@ -957,6 +973,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
             step_report->fn(step_report->user, "patchmatch", s - s0, max_steps))
             return fail(c, MVS_ERR_CANCELLED, "cancelled");
         const int32_t it = s / 2, parity = s % 2;
+        // the first half-sweep of each colour of this call
+        const bool first_pair = s < s0 + 2;
         bool known = true;
         active.clear();
         for (KClass &k : classes) {
@@ -971,12 +989,16 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
                     k.sch = f->second;
                 } else if (c->autotune) {
                     TRY(autotune_class(c, k.taps, k, table.data() + k.first, it, parity,
-                                       s >= s0 + 2));
+                                       s >= s0 + 2, k.own_init && first_pair));
                     c->tuned[key] = k.sch;
                     known = false;
                 } else {
-                    k.sch = candidates(k)[0];
+                    k.sch = candidates(k, k.own_init && first_pair)[0];
                 }
+                // (a cached choice of a call that scored the initial planes
+                // apart may be a source-lane schedule)
+                if (k.own_init && first_pair && k.sch.split > 0)
+                    k.sch = candidates(k, true)[0];
             }
         }
         hipEvent_t ea = nullptr, eb = nullptr;
@@ -988,7 +1010,8 @@ int run_batch(mvs_ctx *c, const std::vector<JobSpec> &jobs, int32_t first_step,
         // both colours have had a half-sweep of this call from s0 + 2 on: the
         // candidates scored two half-sweeps before are known from then
         const bool skip_unchanged = s >= s0 + 2;
-        TRY(sweep_launches(c, active, it, parity, skip_unchanged, evals + 1, evals + 4, ls));
+        TRY(sweep_launches(c, active, it, parity, skip_unchanged, first_pair, evals + 1,
+                           evals + 4, ls));
         c->stats.sweep_kernels += (int64_t)ls.size();
         if (c->graphs && known)
             TRY(run_sweeps_graph(c, ls));

@@ -85,6 +85,10 @@ struct PmArgs {
     // into *skipped rather than scored (one-lane-per-pixel kernels).
     int32_t skip_unchanged;
     unsigned long long *skipped;
+    // Half-sweeps: eval_own = 1 in the first half-sweep of each colour when
+    // the pixels' initial planes have no cost yet: each pixel scores its own
+    // plane first, ahead of every candidate (one-lane-per-pixel kernels).
+    int32_t eval_own;
     const int32_t *px, *py;  // explicit (pixel, hypothesis) list
     const float4 *hyp_in;
     float *out;

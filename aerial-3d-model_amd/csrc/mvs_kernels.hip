@@ -438,6 +438,7 @@ int launch_pm(int mode, int32_t taps, int32_t srcl, int32_t split, int32_t pinho
                                         : PL_F16T;
         return launch_pml(mode, taps, plane, persist, a, grid, st);
     }
+    if (a.eval_own) return -1;  // only the one-lane-per-pixel kernels score own planes
     if (fast) {
         if (taps == 5) return launch_pm_t5f(mode, srcl, split, pinhole != 0, persist, a, grid, st);
         if (taps == 7) return launch_pm_t7f(mode, srcl, split, pinhole != 0, persist, a, grid, st);

@@ -112,8 +112,16 @@ __device__ __forceinline__ float key_cost(uint32_t k) {
 // the same views' 4-byte grey plane as two row pairs per sample (the same
 // coefficients, formed the same way, so the same costs, from a quarter of
 // the plane bytes: the scattered first iterations miss the caches less).
+// Waves per SIMD the 7x7 and 5x5 kernels on the 8-bit and f16 planes are
+// built for at least: four (128 registers), which the half-sweep's register
+// allocation would otherwise miss by a register or two; the others keep the
+// compiler's choice.
+template <int TAPS, int PLANE>
+constexpr int kPmlWavesMin = (TAPS <= 7 && (PLANE == PL_F16 || PLANE == PL_U8)) ? 4 : 1;
+
 template <int TAPS, int MODE, bool PERSIST, int PLANE>
-__global__ __launch_bounds__(64) void k_pml(PmArgs a) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArgs a) {
     using namespace dev;
     constexpr int N = TAPS * TAPS;
     constexpr int H = TAPS / 2;
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
     __shared__ float Af[F32 ? N : 1][64];
     // half-sweeps: the wavefront's propagation candidates as one list of
     // (pixel lane, candidate) entries, and each pixel's lowest (cost, index)
-    __shared__ uint16_t plist[8 * 64];
+    __shared__ uint16_t plist[9 * 64];  // up to eight candidates and the own plane per pixel
     __shared__ unsigned long long pmin[64];
     __shared__ int32_t pbin[kDepthBins];
     __shared__ u32x2 tile[TILE ? kTileCap : 1];
@@ -686,6 +694,17 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     // the hypothesis changed exactly when one was taken
                     bool taken = false;
                     uint32_t bpair = 0xffu;
+                    bool pair_set = false;
+                    if (a.eval_own) {
+                        // the pixel's own plane has no cost yet: scored first
+                        const float c0 = eval(have && pixel_ok, rc.x, rc.y, nsr, delta, lane,
+                                              best.x, best.y, best.z, best.w);
+                        if (have && pixel_ok) {
+                            bcost = c0;
+                            bpair = epair;
+                            pair_set = true;
+                        }
+                    }
                     uint64_t s = seed_refine((uint64_t)pidx, (uint64_t)a.it, (uint64_t)a.parity);
                     const float span = job.imax - job.imin;
 #pragma unroll 1
@@ -728,7 +747,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
                         job.changed[pidx] = taken;
-                        if (taken) job.pair[pidx] = (uint8_t)bpair;
+                        if (taken || pair_set) job.pair[pidx] = (uint8_t)bpair;
                     }
                 } else {
                     // the ten proposals of dense.rs:427-462: the eight
@@ -738,6 +757,12 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                     float bcost = have ? job.cost[pidx] : 2.0f;
                     bool taken = false;  // (at a strictly lower cost: a change)
                     uint32_t bpair = 0xffu;
+                    // eval_own: the pixel's own plane has no cost yet and is
+                    // scored first, as list entry 8 ordered ahead of the
+                    // candidates (its cost wins ties, so a candidate is taken
+                    // only at a strictly lower cost); its pair is then known
+                    const bool own = a.eval_own && pixel_ok;
+                    bool pair_set = false;
                     // A candidate equal to the current best is not scored
                     // (dense.rs:436-438); one equal to an earlier candidate
                     // cannot be accepted either (its cost is that candidate's,
@@ -840,9 +865,15 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         // within a bin does not matter: the minimum is)
                         if (lane < kDepthBins) pbin[lane] = 0;
                         __syncthreads();
+                        uint32_t bin_own = 0u;
+                        if (own)
+                            bin_own = (uint32_t)__builtin_amdgcn_fmed3f(
+                                (__builtin_amdgcn_rcpf(best.x) - job.imin) * bscale, 0.0f,
+                                (float)(kDepthBins - 1));
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
                             if ((amask >> k) & 1u) atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1);
+                        if (own) atomicAdd(&pbin[bin_own], 1);
                         __syncthreads();
                         const int cnt = lane < kDepthBins ? pbin[lane] : 0;
                         int incl = cnt;
@@ -860,6 +891,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                             if ((amask >> k) & 1u)
                                 plist[atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1)] =
                                     (uint16_t)(lane | (k << 6));
+                        if (own) plist[atomicAdd(&pbin[bin_own], 1)] = (uint16_t)(lane | (8 << 6));
                     } else if constexpr (F32) {
                         // f32 views: the list pixel-major, each pixel's
                         // candidates together (after the first iterations
@@ -867,7 +899,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         // same cells): 31 ms less per 77-view run than
                         // candidate-major, which is 37 ms faster on the f16
                         // plane
-                        const int cnt = __popc(amask);
+                        const int cnt = __popc(amask) + (own ? 1 : 0);
                         int incl = cnt;
 #pragma unroll
                         for (int off = 1; off < 64; off <<= 1) {
@@ -876,16 +908,18 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         }
                         total = __shfl(incl, 63);
                         int at = incl - cnt;
+                        if (own) plist[at++] = (uint16_t)(lane | (8 << 6));
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
                             if ((amask >> k) & 1u) plist[at++] = (uint16_t)(lane | (k << 6));
                     } else {
                         // the list, candidate-major (neighbouring pixels' k-th
                         // candidates together, as they come from neighbouring
-                        // pixels themselves)
+                        // pixels themselves; own planes first)
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            const bool on = (amask >> k) & 1u;
+                        for (int j = 0; j < 9; ++j) {
+                            const int k = j == 0 ? 8 : j - 1;
+                            const bool on = j == 0 ? own : ((amask >> k) & 1u);
                             const uint64_t bal = __ballot(on);
                             if (on)
                                 plist[total + (int32_t)__builtin_amdgcn_mbcnt_hi(
@@ -907,26 +941,34 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         const int ox = __shfl(x, o), oy = __shfl(y, o);
                         const float orx = __shfl(rc.x, o), ory = __shfl(rc.y, o);
                         const float onsr = __shfl(nsr, o), odl = __shfl(delta, o);
-                        int nx, ny;
-                        cand_pos(k, ox, oy, nx, ny);
+                        int nx = ox, ny = oy;  // (entry 8: the pixel's own plane)
+                        if (k < 8) cand_pos(k, ox, oy, nx, ny);
                         const float4 hp = act ? job.hyp[(int64_t)ny * w + nx]
                                               : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                         const float cst = eval(act, orx, ory, onsr, odl, o, hp.x,
                                                hp.y, hp.z, hp.w);
-                        // (cost, candidate) orders the keys; the pair rides
-                        // below them
+                        // (cost, order) orders the keys, the own plane first
+                        // and then the candidates in CAND order; the pair
+                        // rides below them
+                        const uint32_t ord = k == 8 ? 0u : (uint32_t)k + 1u;
                         if (act)
                             atomicMin(&pmin[o], ((unsigned long long)cost_key(cst) << 32) |
-                                                    ((unsigned long long)k << 8) |
+                                                    ((unsigned long long)ord << 8) |
                                                     (unsigned long long)epair);
                     }
                     __syncthreads();
                     const unsigned long long pm = pmin[lane];
                     if (pm != ~0ull) {
                         const float cst = key_cost((uint32_t)(pm >> 32));
-                        if (cst < bcost) {
+                        const int ord = (int)((pm >> 8) & 15ull);
+                        if (ord == 0) {
+                            // the own plane scored lowest: its cost, no change
+                            bcost = cst;
+                            bpair = (uint32_t)(pm & 0xffull);
+                            pair_set = true;
+                        } else if (cst < bcost) {
                             int nx, ny;
-                            cand_pos((int)((pm >> 8) & 7ull), x, y, nx, ny);
+                            cand_pos(ord - 1, x, y, nx, ny);
                             best = job.hyp[(int64_t)ny * w + nx];
                             bcost = cst;
                             taken = true;
@@ -961,7 +1003,7 @@ __global__ __launch_bounds__(64) void k_pml(PmArgs a) {
                         job.hyp[pidx] = best;
                         job.cost[pidx] = bcost;
                         job.changed[pidx] = taken;
-                        if (taken) job.pair[pidx] = (uint8_t)bpair;
+                        if (taken || pair_set) job.pair[pidx] = (uint8_t)bpair;
                     }
                 }
             }
