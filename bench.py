@@ -370,14 +370,16 @@ def reference_grey(views, seed=0xF32):
 def f32_input_leg(ctx, surv, timed, prm):
     """The batched refined path on f32 grey input (MVS_GRAY_F32, the `.view`
     cache's own grey): same views, neighbours and parameters as the headline
-    run; one untimed batch tunes the schedules."""
+    run; one untimed batch of the first four views tunes the schedules (as
+    the headline run's warm-up does: tuned on a single view, the schedules
+    suit a one-view launch rather than the batch)."""
     from aerial_mvs import capi
     t = time.perf_counter()
     views = reference_grey(surv.views)
     sv = capi.Survey(ctx, views)
     ctx.synchronize()
     setup = time.perf_counter() - t
-    sv.depth_maps(timed[:1], prm)
+    sv.depth_maps(timed[:4], prm)
     ctx.synchronize()
     ctx.reset_stats()
     t0 = time.perf_counter()
