@@ -257,11 +257,14 @@ def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
 def test_skipped_candidates_change_nothing(gpu_ctx, grey, split):
     """A propagation candidate from a neighbour that its last half-sweep left
     unchanged is not scored again by the one-lane-per-pixel kernels (it was
-    scored two half-sweeps before and could not be taken now). Half-sweeps
-    run in one call (where that applies from the third on) must land on the
-    states of the same half-sweeps run one call each (each from a state the
-    caller sets, where it never applies), bit for bit, and the candidates
-    skipped plus those scored must be the ones scored there."""
+    scored two half-sweeps before and could not be taken now), and a call
+    that makes a half-sweep of each colour scores the initial planes inside
+    those half-sweeps rather than in a pass of its own. Half-sweeps run in
+    one call (initial costs in the first two, skipping from the third on)
+    must land on the states of the initialisation and the same half-sweeps
+    run one call each (each from a state the caller sets, where neither
+    applies), bit for bit, and the candidates skipped plus those scored must
+    be the ones scored there."""
     s = survey_cached("small")
     spec = s.spec
     ref = s.reference_index()
