@@ -66,7 +66,20 @@ constexpr int32_t kBufferWord3 = 0x00020000;
 #define AMVS_SORT_ITERS 2
 #endif
 constexpr int kSortIters = AMVS_SORT_ITERS;
-constexpr int kDepthBins = 16;
+// The finer the bins, the closer neighbouring lanes' depths (77-view
+// half-sweeps: 16 bins 1807 ms, 64 bins 1757, 128 bins 1749, 256 bins 1747
+// with a register spilled).
+#ifndef AMVS_DEPTH_BINS
+#define AMVS_DEPTH_BINS 128
+#endif
+constexpr int kDepthBins = AMVS_DEPTH_BINS;  // a power of two, at most 256
+constexpr int kBinBits = kDepthBins <= 16    ? 4
+                         : kDepthBins <= 32  ? 5
+                         : kDepthBins <= 64  ? 6
+                         : kDepthBins <= 128 ? 7
+                                             : 8;  // per candidate in a 64-bit word
+constexpr int kBinsPerLane = kDepthBins > 64 ? kDepthBins / 64 : 1;
+static_assert(kDepthBins == 1 << kBinBits || kDepthBins < 16, "bins: a power of two");
 
 // PL_F16T: the f16 coefficient plane staged in LDS per (batch, source) when the
 // batch's windows fit a tile of kTileCap texels (8 bytes each) and lie
@@ -772,7 +785,8 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
                     // the wavefront scores the candidates of all its pixels as
                     // one compacted list, no lane idle while others score,
                     // and keeps each pixel's lowest (cost, index) in LDS.
-                    uint32_t amask = 0, bins = 0;
+                    uint32_t amask = 0;
+                    uint64_t bins = 0;  // kBinBits per candidate
                     const bool by_depth = a.it < kSortIters;  // uniform
                     // A candidate from a neighbour that its last half-sweep
                     // left unchanged was scored at this pixel two half-sweeps
@@ -854,7 +868,7 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
                                 const float iv = __builtin_amdgcn_rcpf(cand[k].x);
                                 const int b = (int)__builtin_amdgcn_fmed3f(
                                     (iv - job.imin) * bscale, 0.0f, (float)(kDepthBins - 1));
-                                bins |= (uint32_t)b << (4 * k);
+                                bins |= (uint64_t)b << (kBinBits * k);
                             }
                         }
                     }
@@ -863,33 +877,49 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
                         // the list by depth bin: counts, their scan, then
                         // each candidate at its bin's next slot (the order
                         // within a bin does not matter: the minimum is)
-                        if (lane < kDepthBins) pbin[lane] = 0;
+                        for (int i = lane; i < kDepthBins; i += 64) pbin[i] = 0;
                         __syncthreads();
                         uint32_t bin_own = 0u;
                         if (own)
                             bin_own = (uint32_t)__builtin_amdgcn_fmed3f(
                                 (__builtin_amdgcn_rcpf(best.x) - job.imin) * bscale, 0.0f,
                                 (float)(kDepthBins - 1));
+                        auto bin_of = [&](int k) {
+                            return (uint32_t)(bins >> (kBinBits * k)) & (uint32_t)(kDepthBins - 1);
+                        };
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
-                            if ((amask >> k) & 1u) atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1);
+                            if ((amask >> k) & 1u) atomicAdd(&pbin[bin_of(k)], 1);
                         if (own) atomicAdd(&pbin[bin_own], 1);
                         __syncthreads();
-                        const int cnt = lane < kDepthBins ? pbin[lane] : 0;
+                        // each lane holds kBinsPerLane consecutive bins
+                        int loc[kBinsPerLane], cnt = 0;
+#pragma unroll
+                        for (int j = 0; j < kBinsPerLane; ++j) {
+                            const int bi = lane * kBinsPerLane + j;
+                            loc[j] = bi < kDepthBins ? pbin[bi] : 0;
+                            cnt += loc[j];
+                        }
                         int incl = cnt;
 #pragma unroll
-                        for (int off = 1; off < kDepthBins; off <<= 1) {
+                        for (int off = 1; off < 64; off <<= 1) {
                             const int t = __shfl_up(incl, off);
                             if (lane >= off) incl += t;
                         }
-                        total = __shfl(incl, kDepthBins - 1);
+                        total = __shfl(incl, 63);
                         __syncthreads();
-                        if (lane < kDepthBins) pbin[lane] = incl - cnt;
+                        int run = incl - cnt;
+#pragma unroll
+                        for (int j = 0; j < kBinsPerLane; ++j) {
+                            const int bi = lane * kBinsPerLane + j;
+                            if (bi < kDepthBins) pbin[bi] = run;
+                            run += loc[j];
+                        }
                         __syncthreads();
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
                             if ((amask >> k) & 1u)
-                                plist[atomicAdd(&pbin[(bins >> (4 * k)) & 15u], 1)] =
+                                plist[atomicAdd(&pbin[bin_of(k)], 1)] =
                                     (uint16_t)(lane | (k << 6));
                         if (own) plist[atomicAdd(&pbin[bin_own], 1)] = (uint16_t)(lane | (8 << 6));
                     } else if constexpr (F32) {
