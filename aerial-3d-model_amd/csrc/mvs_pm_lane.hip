@@ -61,7 +61,8 @@ constexpr int32_t kBufferWord3 = 0x00020000;
 // from the wavefront's block of pixels, whose windows land close together in
 // each source (shared cache lines, and in or out of the source's coverage
 // together). Later iterations keep the candidate-major order, whose
-// neighbouring pixels' k-th candidates already agree.
+// neighbouring pixels' k-th candidates already agree (pixel-major on f32
+// views, whose pixels' candidates agree among themselves).
 #ifndef AMVS_SORT_ITERS
 #define AMVS_SORT_ITERS 2
 #endif
