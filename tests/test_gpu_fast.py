@@ -253,8 +253,9 @@ def test_pixel_lane_planes_give_identical_depth_maps(gpu_ctx, grey):
     assert (maps[splits[0]][0] > 0).mean() > 0.5
 
 
-@pytest.mark.parametrize("grey,split", [("u8", -1), ("u8", -2), ("f32", -3)])
-def test_skipped_candidates_change_nothing(gpu_ctx, grey, split):
+@pytest.mark.parametrize("grey,split,iters", [("u8", -1, 4), ("u8", -2, 4), ("f32", -3, 4),
+                                              ("u8", -1, 1), ("u8", -5, 1), ("f32", -4, 1)])
+def test_skipped_candidates_change_nothing(gpu_ctx, grey, split, iters):
     """A propagation candidate from a neighbour that its last half-sweep left
     unchanged is not scored again by the one-lane-per-pixel kernels (it was
     scored two half-sweeps before and could not be taken now), and a call
@@ -271,7 +272,6 @@ def test_skipped_candidates_change_nothing(gpu_ctx, grey, split):
     views = s.views if grey == "u8" else _f32_views(s, 43)
     v = views[ref]
     srcs = [views[j] for j in s.neighbours[ref][:8]]
-    iters = 4
     prm = capi.params(taps=spec.taps, stride=spec.stride, iterations=iters, fast=True)
     gpu_ctx.set_schedule(split, 1, 16, False)
     try:
@@ -288,7 +288,7 @@ def test_skipped_candidates_change_nothing(gpu_ctx, grey, split):
     finally:
         gpu_ctx.set_schedule(0, 0, 16, True)
     _say(f"  scored {ev_a} + skipped {skipped}, one call each {ev_b}")
-    assert skipped > 0
+    assert skipped > 0 or iters == 1  # (one iteration: nothing to skip yet)
     assert np.array_equal(st_a, st_b)
     assert ev_a + skipped == ev_b
 
