@@ -295,7 +295,10 @@ typedef struct {
     double init_ms;
     int64_t evals;            /* cost evaluations counted on the device */
     int64_t pixels;           /* reference pixels processed */
-    int64_t init_evals;       /* of which in the initialisation */
+    int64_t init_evals;       /* of which in the initialisation (0 where the
+                               * first half-sweep of each colour scores the
+                               * initial planes: fast arithmetic on one lane
+                               * per pixel, calls of at least two half-sweeps) */
     int64_t sweep_evals;      /* of which in the half-sweeps */
     int32_t split;            /* schedule of the last batch or cost list: lanes
                                * per source (-1: one lane per pixel, f16 coefficient
