@@ -60,9 +60,8 @@ constexpr int32_t kBufferWord3 = 0x00020000;
 // over the job's range: a batch of 64 then holds candidates of similar depth
 // from the wavefront's block of pixels, whose windows land close together in
 // each source (shared cache lines, and in or out of the source's coverage
-// together). Later iterations keep the candidate-major order, whose
-// neighbouring pixels' k-th candidates already agree (pixel-major on f32
-// views, whose pixels' candidates agree among themselves).
+// together). Later iterations list each pixel's candidates together (they
+// agree among themselves by then).
 #ifndef AMVS_SORT_ITERS
 #define AMVS_SORT_ITERS 2
 #endif
@@ -923,13 +922,13 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
                                 plist[atomicAdd(&pbin[bin_of(k)], 1)] =
                                     (uint16_t)(lane | (k << 6));
                         if (own) plist[atomicAdd(&pbin[bin_own], 1)] = (uint16_t)(lane | (8 << 6));
-                    } else if constexpr (F32) {
-                        // f32 views: the list pixel-major, each pixel's
-                        // candidates together (after the first iterations
-                        // nearly one plane, so neighbouring lanes gather the
-                        // same cells): 31 ms less per 77-view run than
-                        // candidate-major, which is 37 ms faster on the f16
-                        // plane
+                    } else {
+                        // later iterations: the list pixel-major, each pixel's
+                        // candidates (nearly one plane once the first
+                        // iterations are done) in neighbouring lanes, own
+                        // plane first, the pixels in walk order (77-view
+                        // half-sweeps: 1711 ms against 1745 with the pixels'
+                        // k-th candidates together)
                         const int cnt = __popc(amask) + (own ? 1 : 0);
                         int incl = cnt;
 #pragma unroll
@@ -943,22 +942,6 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
 #pragma unroll
                         for (int k = 0; k < 8; ++k)
                             if ((amask >> k) & 1u) plist[at++] = (uint16_t)(lane | (k << 6));
-                    } else {
-                        // the list, candidate-major (neighbouring pixels' k-th
-                        // candidates together, as they come from neighbouring
-                        // pixels themselves; own planes first)
-#pragma unroll
-                        for (int j = 0; j < 9; ++j) {
-                            const int k = j == 0 ? 8 : j - 1;
-                            const bool on = j == 0 ? own : ((amask >> k) & 1u);
-                            const uint64_t bal = __ballot(on);
-                            if (on)
-                                plist[total + (int32_t)__builtin_amdgcn_mbcnt_hi(
-                                                  (uint32_t)(bal >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] =
-                                    (uint16_t)(lane | (k << 6));
-                            total += __popcll(bal);
-                        }
                     }
                     pmin[lane] = ~0ull;
                     __syncthreads();  // the list, pmin and every lane's window in Ah
