@@ -793,68 +793,45 @@ __attribute__((amdgpu_waves_per_eu(kPmlWavesMin<TAPS, PLANE>))) void k_pml(PmArg
                     // before and not taken, or taken (and is the best or was
                     // beaten since): its cost is what it was, and the best
                     // cost can only have fallen, so it cannot be taken now.
-                    // (On f32 views the neighbours' flags and planes are
-                    // loaded before the candidate list, at in-bounds
-                    // positions — the pixel's own where the neighbour is
-                    // outside — so that their loads issue together; their
-                    // values are used only where inside: 14 ms less per
-                    // 77-view run there, 16 ms more on 8-bit views. Read
-                    // among the candidates' loads, the flags made the
-                    // compiler serialise the taps' gathers.)
+                    // (The neighbours' flags and planes are loaded before the
+                    // candidate list, at in-bounds positions — the pixel's
+                    // own where the neighbour is outside — so that their
+                    // loads issue together; their values are used only
+                    // where inside: 23 ms less per 77-view run. Read among
+                    // the candidates' loads, the flags made the compiler
+                    // serialise the taps' gathers.)
                     int64_t nidx[8];
                     uint32_t inbm0 = 0;
                     uint32_t unchanged = 0;
-                    if constexpr (F32) {
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            int nx, ny;
-                            cand_pos(k, x, y, nx, ny);
-                            const bool inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
-                            nidx[k] = inb ? (int64_t)ny * w + nx : pidx;
-                            inbm0 |= (inb ? 1u : 0u) << k;
-                        }
-                        if (a.skip_unchanged && pixel_ok) {
-                            uint8_t fl[8];
+                    for (int k = 0; k < 8; ++k) {
+                        int nx, ny;
+                        cand_pos(k, x, y, nx, ny);
+                        const bool inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
+                        nidx[k] = inb ? (int64_t)ny * w + nx : pidx;
+                        inbm0 |= (inb ? 1u : 0u) << k;
+                    }
+                    if (a.skip_unchanged && pixel_ok) {
+                        uint8_t fl[8];
 #pragma unroll
-                            for (int k = 0; k < 8; ++k) fl[k] = as_global(job.changed)[nidx[k]];
+                        for (int k = 0; k < 8; ++k) fl[k] = as_global(job.changed)[nidx[k]];
 #pragma unroll
-                            for (int k = 0; k < 8; ++k)
-                                if (((inbm0 >> k) & 1u) && !fl[k]) unchanged |= 1u << k;
-                        }
-                    } else if (a.skip_unchanged && pixel_ok) {
-#pragma unroll 1
-                        for (int k = 0; k < 8; ++k) {
-                            int nx, ny;
-                            cand_pos(k, x, y, nx, ny);
-                            if (nx >= 0 && ny >= 0 && nx < w && ny < h &&
-                                !job.changed[(int64_t)ny * w + nx])
-                                unchanged |= 1u << k;
-                        }
+                        for (int k = 0; k < 8; ++k)
+                            if (((inbm0 >> k) & 1u) && !fl[k]) unchanged |= 1u << k;
                     }
                     const float bscale = (float)kDepthBins / (job.imax - job.imin);
                     {
                         float4 cand[8];
                         uint32_t inbm = 0;
-                        if constexpr (F32) {
 #pragma unroll
-                            for (int k = 0; k < 8; ++k) {
-                                const f32x4 c = as_global(job.hyp)[nidx[k]];
-                                cand[k] = make_float4(c.x, c.y, c.z, c.w);
-                            }
+                        for (int k = 0; k < 8; ++k) {
+                            const f32x4 c = as_global(job.hyp)[nidx[k]];
+                            cand[k] = make_float4(c.x, c.y, c.z, c.w);
                         }
 #pragma unroll
                         for (int k = 0; k < 8; ++k) {
-                            bool inb;
-                            if constexpr (F32) {
-                                inb = (inbm0 >> k) & 1u;
-                                if (!inb) cand[k] = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                            } else {
-                                int nx, ny;
-                                cand_pos(k, x, y, nx, ny);
-                                inb = pixel_ok && nx >= 0 && ny >= 0 && nx < w && ny < h;
-                                cand[k] = inb ? job.hyp[(int64_t)ny * w + nx]
-                                              : make_float4(1.0f, 0.0f, 0.0f, -1.0f);
-                            }
+                            const bool inb = (inbm0 >> k) & 1u;
+                            if (!inb) cand[k] = make_float4(1.0f, 0.0f, 0.0f, -1.0f);
                             bool act = inb && !same_hyp(cand[k], best);
 #pragma unroll
                             for (int j = 0; j < k; ++j)
