@@ -81,7 +81,7 @@ PEAK_HBM_GBS = 8000.0
 # profiles/k_pml_sweep_fast_8x2000x1500_k8_7x7_counters.md (fast arithmetic,
 # one lane per pixel) and profiles/k_pm_sweep_exact_8x2000x1500_k8_7x7_counters.md
 # hold the per-launch figures.
-BYTES_PER_EVAL = {"exact": (442.3, 6.55), "fast": (169.0, 8.05)}
+BYTES_PER_EVAL = {"exact": (442.3, 6.55), "fast": (165.6, 8.06)}
 
 
 # The roofline object's "bound" names the compute roofline ("mfma": the FP32
